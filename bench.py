@@ -1,0 +1,156 @@
+"""Benchmark: full binary pursuit on the 64-channel recording, one MI355X.
+
+One step = one drop-in binary_pursuit() call over the whole recording
+(64 ch, 30 kHz, 600 s, 120 units, 48-sample templates on 7-channel
+neighbourhoods, n_max_shift_inds = L - 1, overlap recheck on), with the voltage
+already resident in HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--duration S]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pursuit Msamples·ch/s at 64ch; mixed-prec vs FP32 GPU at identical spike set"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="probe64")
+    ap.add_argument("--duration", type=float, default=None, help="recording length in s (default: full)")
+    ap.add_argument("--literal", action="store_true", help="evaluate every overlap item literally")
+    ap.add_argument("--cpu-sample-s", type=float, default=0.1)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the time-sharded multi-GPU design is described in DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    from spikesorting_fullpursuit_amd import synthetic
+    from spikesorting_fullpursuit_amd.overlap import binary_pursuit_parallel as bpp
+
+    t0 = time.time()
+    d = synthetic.generate_on_device(args.workload, duration_s=args.duration)
+    V = d["voltage"]
+    C, N = V.shape
+    w = d["workload"]
+    U, _, L = d["templates"].shape
+    log(f"generated {args.workload}: C={C} N={N} U={U} L={L} spikes={len(d['truth_onsets'])} "
+        f"in {time.time() - t0:.1f}s")
+
+    def step(stats=None, literal=args.literal):
+        m = dict(d["separability_metrics"])
+        return bpp.binary_pursuit(V, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"],
+                                  recheck_literal=literal, stats=stats)
+
+    for i in range(args.warmup):
+        t1 = time.time()
+        out = step()
+        torch.cuda.synchronize()
+        log(f"warmup {i}: {time.time() - t1:.3f}s spikes={len(out[0])}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for i in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = N * C / (elapsed / args.steps) / 1e6
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.1f} Msamples*ch/s, spikes={len(out[0])}")
+
+    # One extra (untimed) run with per-kernel HIP-event timing for the roofline.
+    from spikesorting_fullpursuit_amd import _native
+    h = bpp._handle(0)
+    T3, bias, thr = synthetic.kernel_inputs(d)
+    h.set_templates(T3, bias, thr)
+    flags = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_TIMING | (_native.FLAG_RECHECK_LITERAL if args.literal else 0)
+    h.run(None, d["n_max_shift_inds"], flags, ld=V.stride(0), n_samples=N, device_ptr=V.data_ptr())
+    st = h.stats()
+    log("kernel stats:", json.dumps(st))
+    classes = {"likelihood": st["ms_likelihood"], "recheck": st["ms_recheck"], "window_max": st["ms_window_max"],
+               "pursuit": st["ms_pursuit"], "residual": st["ms_residual"], "other": st["ms_other"]}
+    dominant = max(classes, key=classes.get)
+    lik_avg_ms = st["ms_likelihood"] / max(1, st["n_likelihood_launches"])
+    lik_tflops = st["likelihood_flops"] / (st["ms_likelihood"] * 1e-3) / 1e12 if st["ms_likelihood"] > 0 else 0.0
+    roofline = {
+        "kernel": "k_likelihood",
+        "bound": "mfma",
+        "achieved": round(lik_tflops, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(lik_tflops / FP32_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(lik_avg_ms, 4),
+        "flops_per_launch": st["likelihood_flops"] / max(1, st["n_likelihood_launches"]),
+        "dominant_class": dominant,
+        "ms_by_class": {k: round(v, 3) for k, v in classes.items()},
+        "passes": st["passes"],
+    }
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        from oracle import fbp_oracle
+        ns = int(round(args.cpu_sample_s * w.fs))
+        Vs = V[:, :ns].cpu().numpy()
+        nthr = min(args.cpu_threads, os.cpu_count() or 1)
+        t2 = time.perf_counter()
+        oi, ol = fbp_oracle.pursuit(Vs, T3, bias, thr, d["n_max_shift_inds"], nthreads=nthr)
+        tc = time.perf_counter() - t2
+        cpu = {"value": round(ns * C / tc / 1e6, 6), "unit": "Msamples·ch/s", "cores": nthr, "kind": "port",
+               "sample": f"first {args.cpu_sample_s} s of the same recording ({ns} samples x {C} ch), "
+                         f"C restatement with OpenMP, {tc:.1f} s"}
+        h.run(None, d["n_max_shift_inds"], _native.FLAG_OVERLAP_RECHECK | (_native.FLAG_RECHECK_LITERAL if args.literal else 0),
+              ld=V.stride(0), n_samples=ns, device_ptr=V.data_ptr())
+        gi, gl = h.result()
+        parity = bool(np.array_equal(gi, oi) and np.array_equal(gl, ol))
+        log(f"cpu baseline {cpu['value']} Msamples*ch/s on {nthr} threads; slice parity {parity} ({len(oi)} spikes)")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Msamples·ch/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {C} ch, {w.fs / 1000:g} kHz, {N / w.fs:g} s, {U} units, L={L}, "
+                        f"n_max_shift_inds={d['n_max_shift_inds']}, overlap recheck on, voltage resident in HBM",
+            "spikes_found": int(len(out[0])),
+            "spikes_planted": int(len(d["truth_onsets"])),
+            "recheck": "literal" if args.literal else "screened",
+            "slice_parity_vs_oracle": parity,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,100 @@
+/*
+ * fbp.h -- C ABI of the MI355X full-binary-pursuit library (libfbp_hip.so).
+ *
+ * This boundary replaces the device half of the reference's
+ *   binary_pursuit(voltage, v_dtype, sort_info, separability_metrics,
+ *                  n_max_shift_inds=None, kernels_path=None, max_gpu_memory=None)
+ * (spikesorting_fullpursuit/overlap/binary_pursuit_parallel.py:175-183), i.e. the
+ * pyopencl program build, buffers and the pass loop of :296-1500 together with
+ * the eight OpenCL kernels of spikesorting_fullpursuit/kernels/binary_pursuit.cl.
+ * Chunk planning and output assembly stay in the Python drop-in.
+ *
+ * Conventions: plain pointers and sizes; every function returns an int status
+ * (FBP_OK or a negative FBP_E_*); fbp_last_error() gives the message.  One
+ * handle owns one HIP stream and its device buffers; a handle must not be used
+ * from two threads at once.  Voltage is [C][N] float32 with row stride `ld`
+ * elements; templates are [U][C][L] float32 (the reference's flattened
+ * templates_vector, binary_pursuit_parallel.py:239-246).
+ */
+#ifndef FBP_H
+#define FBP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FBP_OK 0
+#define FBP_E_INVALID -1     /* bad argument           -> ValueError   */
+#define FBP_E_NODEVICE -2    /* no usable GPU          -> RuntimeError("No GPU available"), binary_pursuit_parallel.py:292-293 */
+#define FBP_E_HIP -3         /* HIP runtime failure    -> RuntimeError */
+#define FBP_E_NOMEM -4       /* device allocation      -> MemoryError  */
+#define FBP_E_UNSUPPORTED -5 /* shape outside limits   -> ValueError   */
+#define FBP_E_STATE -6       /* call order             -> RuntimeError */
+
+/* fbp_run flags */
+#define FBP_FLAG_OVERLAP_RECHECK 1u  /* sort_info["do_overlap_recheck"], binary_pursuit_parallel.py:980-983 */
+#define FBP_FLAG_ADJUSTED_CLIPS 2u   /* sort_info["get_adjusted_clips"], binary_pursuit_parallel.py:1404-1486 */
+#define FBP_FLAG_RECHECK_LITERAL 4u  /* evaluate every overlap item literally (no screening) */
+#define FBP_FLAG_TIMING 8u           /* record per-kernel device time with HIP events */
+
+typedef struct fbp_handle fbp_handle;
+
+typedef struct fbp_stats {
+    int32_t passes;               /* iterations of the pass loop (binary_pursuit_parallel.py:907) */
+    int32_t reserved0;
+    int64_t spikes;               /* spikes found in the last run */
+    int64_t windows_listed;       /* sum over passes of listed windows */
+    int64_t windows_flagged;      /* sum over passes of overlap-recheck windows */
+    int64_t likelihood_positions; /* positions x templates recomputed */
+    double likelihood_flops;      /* 2 * sum over recomputed (x, t) of nz_channels(t) * L */
+    double ms_total;              /* device time of the run (events around the whole run) */
+    double ms_likelihood;         /* per kernel class, only with FBP_FLAG_TIMING */
+    double ms_window_max;
+    double ms_recheck;
+    double ms_pursuit;
+    double ms_residual;
+    double ms_other;
+    int64_t n_likelihood_launches;
+    int64_t n_recheck_launches;
+    int64_t recheck_items;        /* overlap items evaluated exactly */
+} fbp_stats;
+
+/* Create a handle on HIP device `device` (replaces the device search and
+ * program build of binary_pursuit_parallel.py:282-300). */
+int fbp_create(int device, fbp_handle **out);
+int fbp_destroy(fbp_handle *h);
+const char *fbp_last_error(const fbp_handle *h);
+
+/* Total device memory in bytes (GLOBAL_MEM_SIZE for the chunk plan,
+ * binary_pursuit_parallel.py:387-409). */
+int fbp_device_memory(fbp_handle *h, int64_t *total_bytes);
+
+/* Upload templates [U][C][L], bias[U] = float32(-0.5*template_SS) and
+ * thresholds[U] = float32(neuron_lower_thresholds); host pointers
+ * (binary_pursuit_parallel.py:239-264, :480-496). */
+int fbp_set_templates(fbp_handle *h, const float *templates, int32_t U, int32_t C, int32_t L,
+                      const float *bias, const float *thresholds);
+
+/* Run the pass loop on one chunk V[c][x], c < C, x < N, row stride ld.
+ * v_on_device != 0: V is a device pointer (it is copied, never modified).
+ * Replaces one iteration of the chunk loop, binary_pursuit_parallel.py:510-1500. */
+int fbp_run(fbp_handle *h, const float *V, int64_t ld, int64_t N, int32_t v_on_device,
+            uint32_t n_max_shift_inds, uint32_t flags);
+
+/* Results of the last fbp_run: spike onsets within the chunk and labels in
+ * pass order (ascending window within a pass). */
+int fbp_result_count(fbp_handle *h, int64_t *n);
+int fbp_copy_result(fbp_handle *h, uint32_t *indices, uint32_t *labels);
+/* Adjusted clips [n][C*L] of the last run (requires FBP_FLAG_ADJUSTED_CLIPS). */
+int fbp_copy_clips(fbp_handle *h, float *clips);
+/* Residual voltage [C][N] of the last run into a host buffer. */
+int fbp_copy_residual(fbp_handle *h, float *out);
+int fbp_get_stats(fbp_handle *h, fbp_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FBP_H */

@@ -1,0 +1,189 @@
+"""ctypes binding of libfbp_hip.so (the C ABI declared in include/fbp.h).
+
+The library is built in-tree by spikesorting_fullpursuit_amd.build.  There is
+no CPU fallback: if the library or a GPU is missing, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "libfbp_hip.so")
+
+FBP_OK = 0
+FBP_E_INVALID = -1
+FBP_E_NODEVICE = -2
+FBP_E_HIP = -3
+FBP_E_NOMEM = -4
+FBP_E_UNSUPPORTED = -5
+FBP_E_STATE = -6
+
+FLAG_OVERLAP_RECHECK = 1
+FLAG_ADJUSTED_CLIPS = 2
+FLAG_RECHECK_LITERAL = 4
+FLAG_TIMING = 8
+
+# Every symbol include/fbp.h declares.
+EXPORTS = (
+    "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
+    "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
+    "fbp_get_stats",
+)
+
+
+class FbpStats(ctypes.Structure):
+    _fields_ = [
+        ("passes", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32),
+        ("spikes", ctypes.c_int64),
+        ("windows_listed", ctypes.c_int64),
+        ("windows_flagged", ctypes.c_int64),
+        ("likelihood_positions", ctypes.c_int64),
+        ("likelihood_flops", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("ms_likelihood", ctypes.c_double),
+        ("ms_window_max", ctypes.c_double),
+        ("ms_recheck", ctypes.c_double),
+        ("ms_pursuit", ctypes.c_double),
+        ("ms_residual", ctypes.c_double),
+        ("ms_other", ctypes.c_double),
+        ("n_likelihood_launches", ctypes.c_int64),
+        ("n_recheck_launches", ctypes.c_int64),
+        ("recheck_items", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved0"}
+
+
+_lib = None
+
+
+def load():
+    """Load libfbp_hip.so and declare its signatures (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -m spikesorting_fullpursuit_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, u32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32
+    L.fbp_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+    L.fbp_destroy.argtypes = [vp]
+    L.fbp_last_error.argtypes = [vp]
+    L.fbp_last_error.restype = ctypes.c_char_p
+    L.fbp_device_memory.argtypes = [vp, ctypes.POINTER(i64)]
+    L.fbp_set_templates.argtypes = [vp, vp, i32, i32, i32, vp, vp]
+    L.fbp_run.argtypes = [vp, vp, i64, i64, i32, u32, u32]
+    L.fbp_result_count.argtypes = [vp, ctypes.POINTER(i64)]
+    L.fbp_copy_result.argtypes = [vp, vp, vp]
+    L.fbp_copy_clips.argtypes = [vp, vp]
+    L.fbp_copy_residual.argtypes = [vp, vp]
+    L.fbp_get_stats.argtypes = [vp, ctypes.POINTER(FbpStats)]
+    for name in EXPORTS:
+        if name != "fbp_last_error":
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _raise(code, msg):
+    if code == FBP_E_NODEVICE:
+        raise RuntimeError("No GPU available")
+    if code in (FBP_E_INVALID, FBP_E_UNSUPPORTED):
+        raise ValueError(msg)
+    if code == FBP_E_NOMEM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+class Handle:
+    """One device + stream + buffers (fbp_handle)."""
+
+    def __init__(self, device=0):
+        self._lib = load()
+        self._h = ctypes.c_void_p()
+        rc = self._lib.fbp_create(int(device), ctypes.byref(self._h))
+        if rc != FBP_OK:
+            _raise(rc, f"fbp_create failed ({rc})")
+        self.C = self.L = self.U = None
+
+    def close(self):
+        if self._h:
+            self._lib.fbp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != FBP_OK:
+            msg = self._lib.fbp_last_error(self._h)
+            _raise(rc, msg.decode() if msg else f"error {rc}")
+
+    def device_memory(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.fbp_device_memory(self._h, ctypes.byref(n)))
+        return int(n.value)
+
+    def set_templates(self, T, bias, thr):
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        if T.ndim != 3:
+            raise ValueError("templates must be [U][C][L]")
+        bias = np.ascontiguousarray(bias, dtype=np.float32)
+        thr = np.ascontiguousarray(thr, dtype=np.float32)
+        U, C, L = T.shape
+        if bias.shape != (U,) or thr.shape != (U,):
+            raise ValueError("bias and thresholds must have one entry per template")
+        self._check(self._lib.fbp_set_templates(self._h, T.ctypes.data, U, C, L, bias.ctypes.data,
+                                                thr.ctypes.data))
+        self.U, self.C, self.L = U, C, L
+
+    def run(self, V, n_max_shift, flags, ld=None, n_samples=None, device_ptr=None):
+        """Run one chunk.  V: host float32 [C][>=N] array, or pass device_ptr
+        (int address of a device float32 [C][ld] buffer) with ld and n_samples."""
+        if device_ptr is not None:
+            self._check(self._lib.fbp_run(self._h, ctypes.c_void_p(int(device_ptr)), int(ld), int(n_samples), 1,
+                                          int(n_max_shift), int(flags)))
+            return
+        V = np.asarray(V)
+        if V.dtype != np.float32 or V.strides[1] != 4:
+            V = np.ascontiguousarray(V, dtype=np.float32)
+        if V.shape[0] != self.C:
+            raise ValueError("voltage channels do not match templates")
+        ld = V.strides[0] // 4
+        self._check(self._lib.fbp_run(self._h, V.ctypes.data, int(ld), int(V.shape[1]), 0, int(n_max_shift),
+                                      int(flags)))
+
+    def result(self):
+        n = ctypes.c_int64(0)
+        self._check(self._lib.fbp_result_count(self._h, ctypes.byref(n)))
+        idx = np.zeros(n.value, np.uint32)
+        lab = np.zeros(n.value, np.uint32)
+        self._check(self._lib.fbp_copy_result(self._h, idx.ctypes.data, lab.ctypes.data))
+        return idx, lab
+
+    def clips(self, n):
+        out = np.zeros((n, self.C * self.L), np.float32)
+        self._check(self._lib.fbp_copy_clips(self._h, out.ctypes.data))
+        return out
+
+    def residual(self, n_samples):
+        out = np.zeros((self.C, n_samples), np.float32)
+        self._check(self._lib.fbp_copy_residual(self._h, out.ctypes.data))
+        return out
+
+    def stats(self):
+        s = FbpStats()
+        self._check(self._lib.fbp_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()

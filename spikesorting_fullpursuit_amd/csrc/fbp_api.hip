@@ -1,0 +1,539 @@
+// C-ABI and host pass loop of the MI355X binary pursuit.
+//
+// Replaces the pyopencl driver of SF/overlap/binary_pursuit_parallel.py:296-1500
+// (SF = spikesorting_fullpursuit/) for one chunk.  The pass loop keeps the
+// window lists on the device; the host reads two small counters per pass
+// (overlap windows, then spikes added and next list length).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fbp.h"
+#include "fbp_internal.h"
+
+namespace fbp {
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
+void launch_iota(uint32_t* p, int64_t n, hipStream_t s);
+void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
+bool likelihood_fits(const TemplateSet& ts);
+void launch_likelihood(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* W, uint32_t nW, float* LIK,
+                       hipStream_t s);
+void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
+void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
+void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
+                          uint32_t* blockbuf, hipStream_t s);
+void launch_next_windows(const PassState& st, uint32_t* blockbuf, hipStream_t s);
+uint32_t recheck_literal_chunks(const TemplateSet& ts, int64_t n_max);
+void launch_recheck_literal(const TemplateSet& ts, const PassState& st, int64_t n_max, uint32_t nF, float* part_v,
+                            uint32_t* part_k, hipStream_t s);
+void launch_reassign(const TemplateSet& ts, const PassState& st, uint32_t nF, hipStream_t s);
+void launch_pursuit(const TemplateSet& ts, const PassState& st, uint32_t nW, int use_giveup, hipStream_t s);
+void launch_residual(const TemplateSet& ts, float* V, int64_t N, const uint32_t* sidx, const uint32_t* slab, uint32_t nsp,
+                     int reach, hipStream_t s);
+void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* sidx, const uint32_t* slab,
+                           uint64_t nsp, float* clips, hipStream_t s);
+}  // namespace fbp
+
+using namespace fbp;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            if (e != hipSuccess) return e;
+            p = nullptr;
+            cap = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) cap = want;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Growable buffer that keeps its contents (stream-ordered copy).
+hipError_t grow_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s)
+{
+    if (bytes <= b.cap) return hipSuccess;
+    size_t want = std::max(bytes, b.cap * 2);
+    void* np = nullptr;
+    hipError_t e = hipMalloc(&np, want);
+    if (e != hipSuccess) return e;
+    if (used && b.p) {
+        e = hipMemcpyAsync(np, b.p, used, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
+    if (b.p) (void)hipFree(b.p);
+    b.p = np;
+    b.cap = want;
+    return hipSuccess;
+}
+
+}  // namespace
+
+struct fbp_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // templates
+    int U = 0, C = 0, L = 0, Lp = 0;
+    bool have_templates = false;
+    DevBuf T, rows, nch, off, chan, nz, bias, thr;
+    std::vector<int> h_nch;
+    // chunk state
+    int64_t N = 0, nwin = 0;
+    DevBuf V, LIK, best_idx, best_lab, best_lik, obest_idx, obest_lab;
+    DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf;
+    DevBuf part_v, part_k;
+    DevBuf sp_idx, sp_lab, clips;
+    uint32_t* h_counters = nullptr;  // pinned
+    int64_t n_spikes = 0;
+    bool have_result = false, have_clips = false;
+    fbp_stats stats{};
+};
+
+static int set_err(fbp_handle* h, int code, const std::string& msg)
+{
+    if (h) h->err = msg;
+    return code;
+}
+
+static int hip_err(fbp_handle* h, hipError_t e, const char* where)
+{
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    return set_err(h, e == hipErrorOutOfMemory ? FBP_E_NOMEM : FBP_E_HIP, m);
+}
+
+#define HIP_TRY(expr, where)                               \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hip_err(h, _e, where); \
+    } while (0)
+
+static TemplateSet template_view(const fbp_handle* h)
+{
+    TemplateSet ts;
+    ts.T = h->T.as<float>();
+    ts.rows = h->rows.as<float>();
+    ts.nch = h->nch.as<int>();
+    ts.off = h->off.as<int>();
+    ts.chan = h->chan.as<int>();
+    ts.nz = h->nz.as<uint8_t>();
+    ts.bias = h->bias.as<float>();
+    ts.thr = h->thr.as<float>();
+    ts.U = h->U;
+    ts.C = h->C;
+    ts.L = h->L;
+    ts.Lp = h->Lp;
+    return ts;
+}
+
+static PassState pass_view(const fbp_handle* h)
+{
+    PassState st;
+    st.V = h->V.as<float>();
+    st.LIK = h->LIK.as<float>();
+    st.best_idx = h->best_idx.as<uint32_t>();
+    st.best_lab = h->best_lab.as<uint32_t>();
+    st.best_lik = h->best_lik.as<float>();
+    st.obest_idx = h->obest_idx.as<uint32_t>();
+    st.obest_lab = h->obest_lab.as<uint32_t>();
+    st.check = h->check.as<uint8_t>();
+    st.check_snap = h->check_snap.as<uint8_t>();
+    st.orc = h->orc.as<uint8_t>();
+    st.giveup = h->giveup.as<uint8_t>();
+    st.has = h->has.as<uint8_t>();
+    st.W = h->W.as<uint32_t>();
+    st.F = h->F.as<uint32_t>();
+    st.cand_idx = h->cand_idx.as<uint32_t>();
+    st.cand_lab = h->cand_lab.as<uint32_t>();
+    st.counters = h->counters.as<uint32_t>();
+    st.N = h->N;
+    st.nwin = h->nwin;
+    return st;
+}
+
+// Event-pair timer for one kernel class.
+struct Timer {
+    hipStream_t s;
+    bool on;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];
+    Timer(hipStream_t st, bool enable) : s(st), on(enable) {}
+    void begin(int cls)
+    {
+        if (!on) return;
+        hipEvent_t a, b;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+        ev[cls].push_back({a, b});
+    }
+    void end(int cls)
+    {
+        if (!on) return;
+        hipEventRecord(ev[cls].back().second, s);
+    }
+    double total(int cls)
+    {
+        double t = 0;
+        for (auto& p : ev[cls]) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, p.first, p.second);
+            t += ms;
+        }
+        return t;
+    }
+    ~Timer()
+    {
+        for (auto& v : ev)
+            for (auto& p : v) {
+                hipEventDestroy(p.first);
+                hipEventDestroy(p.second);
+            }
+    }
+};
+enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER };
+
+extern "C" {
+
+int fbp_create(int device, fbp_handle** out)
+{
+    if (!out) return FBP_E_INVALID;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return FBP_E_NODEVICE;
+    if (device < 0 || device >= ndev) return FBP_E_INVALID;
+    fbp_handle* h = new fbp_handle();
+    h->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_counters, CNT_N * sizeof(uint32_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        delete h;
+        return FBP_E_HIP;
+    }
+    *out = h;
+    return FBP_OK;
+}
+
+int fbp_destroy(fbp_handle* h)
+{
+    if (!h) return FBP_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK,
+                      &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
+                      &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
+                      &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips};
+    for (DevBuf* b : bufs) b->release();
+    if (h->h_counters) (void)hipHostFree(h->h_counters);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FBP_OK;
+}
+
+const char* fbp_last_error(const fbp_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int fbp_device_memory(fbp_handle* h, int64_t* total_bytes)
+{
+    if (!h || !total_bytes) return FBP_E_INVALID;
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot), "hipMemGetInfo");
+    *total_bytes = (int64_t)tot;
+    return FBP_OK;
+}
+
+int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t C, int32_t L, const float* bias,
+                      const float* thresholds)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!templates || !bias || !thresholds || U <= 0 || C <= 0 || L <= 0)
+        return set_err(h, FBP_E_INVALID, "templates, bias and thresholds must be non-empty");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    const int Lp = (L + 3) & ~3;
+    std::vector<uint8_t> nz((size_t)U * C, 0);
+    std::vector<int> nch(U, 0), off(U, 0), chan;
+    std::vector<float> rows;
+    for (int u = 0; u < U; u++) {
+        off[u] = (int)chan.size();
+        for (int c = 0; c < C; c++) {
+            const float* r = templates + ((size_t)u * C + c) * L;
+            bool any = false;
+            for (int i = 0; i < L; i++)
+                if (r[i] != 0.0f) { any = true; break; }
+            if (!any) continue;
+            nz[(size_t)u * C + c] = 1;
+            nch[u]++;
+            chan.push_back(c);
+            size_t base = rows.size();
+            rows.resize(base + Lp, 0.0f);
+            std::memcpy(rows.data() + base, r, L * sizeof(float));
+        }
+    }
+    if (chan.empty()) {  // keep buffers non-empty
+        chan.push_back(0);
+        rows.resize(Lp, 0.0f);
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    HIP_TRY(h->T.ensure((size_t)U * C * L * 4), "alloc templates");
+    HIP_TRY(h->rows.ensure(rows.size() * 4), "alloc rows");
+    HIP_TRY(h->nch.ensure(U * 4), "alloc nch");
+    HIP_TRY(h->off.ensure(U * 4), "alloc off");
+    HIP_TRY(h->chan.ensure(chan.size() * 4), "alloc chan");
+    HIP_TRY(h->nz.ensure(nz.size()), "alloc nz");
+    HIP_TRY(h->bias.ensure(U * 4), "alloc bias");
+    HIP_TRY(h->thr.ensure(U * 4), "alloc thr");
+    HIP_TRY(hipMemcpy(h->T.p, templates, (size_t)U * C * L * 4, hipMemcpyHostToDevice), "copy templates");
+    HIP_TRY(hipMemcpy(h->rows.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice), "copy rows");
+    HIP_TRY(hipMemcpy(h->nch.p, nch.data(), U * 4, hipMemcpyHostToDevice), "copy nch");
+    HIP_TRY(hipMemcpy(h->off.p, off.data(), U * 4, hipMemcpyHostToDevice), "copy off");
+    HIP_TRY(hipMemcpy(h->chan.p, chan.data(), chan.size() * 4, hipMemcpyHostToDevice), "copy chan");
+    HIP_TRY(hipMemcpy(h->nz.p, nz.data(), nz.size(), hipMemcpyHostToDevice), "copy nz");
+    HIP_TRY(hipMemcpy(h->bias.p, bias, U * 4, hipMemcpyHostToDevice), "copy bias");
+    HIP_TRY(hipMemcpy(h->thr.p, thresholds, U * 4, hipMemcpyHostToDevice), "copy thr");
+    h->U = U;
+    h->C = C;
+    h->L = L;
+    h->Lp = Lp;
+    h->h_nch = nch;
+    h->have_templates = true;
+    h->have_result = false;
+    return FBP_OK;
+}
+
+int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_on_device, uint32_t n_max_shift,
+            uint32_t flags)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!h->have_templates) return set_err(h, FBP_E_STATE, "fbp_set_templates must be called first");
+    if (!Vsrc || N <= 0 || ld < N) return set_err(h, FBP_E_INVALID, "voltage must be non-empty with ld >= N");
+    const int64_t L = h->L, C = h->C, U = h->U;
+    if (N >= (int64_t)0xffffffffLL) return set_err(h, FBP_E_UNSUPPORTED, "chunk longer than 2^32-1 samples");
+    const uint64_t S = 2ull * n_max_shift + 1ull;
+    const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
+    if (recheck && S * S * (uint64_t)U >= 0xffffffffull)
+        return set_err(h, FBP_E_UNSUPPORTED, "(2*n_max_shift_inds+1)^2 * n_templates must be < 2^32-1");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    TemplateSet ts = template_view(h);
+    if (!likelihood_fits(ts))
+        return set_err(h, FBP_E_UNSUPPORTED, "channels x template length exceed the likelihood tile (64 KiB LDS)");
+
+    const int64_t nwin = (N + L - 1) / L;
+    h->N = N;
+    h->nwin = nwin;
+    h->have_result = false;
+    h->have_clips = false;
+    HIP_TRY(h->V.ensure((size_t)C * N * 4), "alloc voltage");
+    HIP_TRY(h->LIK.ensure((size_t)U * N * 4), "alloc likelihood");
+    HIP_TRY(h->best_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->best_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->best_lik.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->obest_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->obest_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->check.ensure(nwin), "alloc");
+    HIP_TRY(h->check_snap.ensure(nwin), "alloc");
+    HIP_TRY(h->orc.ensure(nwin), "alloc");
+    HIP_TRY(h->giveup.ensure(nwin), "alloc");
+    HIP_TRY(h->has.ensure(nwin), "alloc");
+    HIP_TRY(h->W.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->F.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->cand_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->cand_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->counters.ensure(CNT_N * 4), "alloc");
+    HIP_TRY(h->blockbuf.ensure(((nwin + kCompactTile - 1) / kCompactTile + 1) * 4), "alloc");
+    HIP_TRY(grow_keep(h->sp_idx, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
+    HIP_TRY(grow_keep(h->sp_lab, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
+
+    Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
+    hipEvent_t ev0, ev1;
+    HIP_TRY(hipEventCreate(&ev0), "event");
+    HIP_TRY(hipEventCreate(&ev1), "event");
+    HIP_TRY(hipEventRecord(ev0, s), "event");
+
+    PassState st = pass_view(h);
+    if (v_on_device) {
+        launch_copy_rows(Vsrc, ld, st.V, C, N, s);
+    } else {
+        HIP_TRY(hipMemcpy2DAsync(st.V, N * 4, Vsrc, ld * 4, N * 4, C, hipMemcpyHostToDevice, s), "copy voltage");
+    }
+    // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
+    // best_* zeros, flags zeros, window list 0..nwin-1).
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
+    launch_fill_u32(st.best_idx, 0u, nwin, s);
+    launch_fill_u32(st.best_lab, 0u, nwin, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.best_lik), 0u, nwin, s);
+    launch_fill_u32(st.obest_idx, 0u, nwin, s);
+    launch_fill_u32(st.obest_lab, 0u, nwin, s);
+    launch_fill_u8(st.check, 0, nwin, s);
+    launch_fill_u8(st.orc, 0, nwin, s);
+    launch_fill_u8(st.giveup, 0, nwin, s);
+    launch_fill_u32(st.counters, 0u, CNT_N, s);
+    launch_iota(st.W, nwin, s);
+
+    fbp_stats stats{};
+    int64_t nz_sum = 0;
+    for (int u = 0; u < h->U; u++) nz_sum += h->h_nch[u];
+    const int reach = (int)(3 + (2 * (int64_t)n_max_shift + L) / L);
+    const uint32_t nchunk = recheck ? recheck_literal_chunks(ts, n_max_shift) : 0;
+    uint32_t nW = (uint32_t)nwin;
+    int64_t total = 0;
+    int passes = 0;
+    for (;;) {
+        stats.windows_listed += nW;
+        stats.likelihood_positions += (int64_t)nW * L * U;
+        stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
+        tm.begin(TM_LIK);
+        launch_likelihood(ts, st.V, N, st.W, nW, st.LIK, s);
+        tm.end(TM_LIK);
+        stats.n_likelihood_launches++;
+        tm.begin(TM_MAX);
+        launch_window_max(ts, st, nW, s);
+        tm.end(TM_MAX);
+        uint32_t nF = 0;
+        if (recheck) {
+            tm.begin(TM_OTHER);
+            launch_flag_compact(st, nW, h->blockbuf.as<uint32_t>(), s);
+            tm.end(TM_OTHER);
+            HIP_TRY(hipMemcpyAsync(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost, s), "read counters");
+            HIP_TRY(hipStreamSynchronize(s), "sync");
+            nF = h->h_counters[CNT_F];
+            stats.windows_flagged += nF;
+            if (nF > 0) {
+                HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
+                HIP_TRY(h->part_k.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
+                tm.begin(TM_RECHECK);
+                launch_recheck_literal(ts, st, n_max_shift, nF, h->part_v.as<float>(), h->part_k.as<uint32_t>(), s);
+                tm.end(TM_RECHECK);
+                stats.n_recheck_launches++;
+                tm.begin(TM_OTHER);
+                launch_reassign(ts, st, nF, s);
+                tm.end(TM_OTHER);
+            }
+        }
+        tm.begin(TM_PURSUIT);
+        launch_pursuit(ts, st, nW, recheck && nF > 0 ? 1 : 0, s);
+        tm.end(TM_PURSUIT);
+        const size_t need = (size_t)(total + nW) * 4;
+        HIP_TRY(grow_keep(h->sp_idx, need, (size_t)total * 4, s), "grow spikes");
+        HIP_TRY(grow_keep(h->sp_lab, need, (size_t)total * 4, s), "grow spikes");
+        tm.begin(TM_OTHER);
+        launch_spike_compact(st, nW, h->sp_idx.as<uint32_t>(), h->sp_lab.as<uint32_t>(), (uint64_t)total,
+                             h->blockbuf.as<uint32_t>(), s);
+        launch_next_windows(st, h->blockbuf.as<uint32_t>(), s);
+        tm.end(TM_OTHER);
+        HIP_TRY(hipMemcpyAsync(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost, s), "read counters");
+        HIP_TRY(hipStreamSynchronize(s), "sync");
+        passes++;
+        const uint32_t nadd = h->h_counters[CNT_SPIKES];
+        if (nadd == 0) break;
+        tm.begin(TM_RESID);
+        launch_residual(ts, st.V, N, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, reach, s);
+        tm.end(TM_RESID);
+        total += nadd;
+        nW = h->h_counters[CNT_W];
+    }
+    if (flags & FBP_FLAG_ADJUSTED_CLIPS) {
+        HIP_TRY(h->clips.ensure((size_t)std::max<int64_t>(total, 1) * C * L * 4), "alloc clips");
+        launch_adjusted_clips(ts, st.V, N, h->sp_idx.as<uint32_t>(), h->sp_lab.as<uint32_t>(), (uint64_t)total,
+                              h->clips.as<float>(), s);
+        h->have_clips = true;
+    }
+    HIP_TRY(hipEventRecord(ev1, s), "event");
+    HIP_TRY(hipGetLastError(), "kernel launch");
+    HIP_TRY(hipEventSynchronize(ev1), "sync");
+    float ms = 0;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    hipEventDestroy(ev0);
+    hipEventDestroy(ev1);
+    stats.ms_total = ms;
+    stats.ms_likelihood = tm.total(TM_LIK);
+    stats.ms_window_max = tm.total(TM_MAX);
+    stats.ms_recheck = tm.total(TM_RECHECK);
+    stats.ms_pursuit = tm.total(TM_PURSUIT);
+    stats.ms_residual = tm.total(TM_RESID);
+    stats.ms_other = tm.total(TM_OTHER);
+    stats.passes = passes;
+    stats.spikes = total;
+    h->stats = stats;
+    h->n_spikes = total;
+    h->have_result = true;
+    return FBP_OK;
+}
+
+int fbp_result_count(fbp_handle* h, int64_t* n)
+{
+    if (!h || !n) return FBP_E_INVALID;
+    if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
+    *n = h->n_spikes;
+    return FBP_OK;
+}
+
+int fbp_copy_result(fbp_handle* h, uint32_t* indices, uint32_t* labels)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
+    if (h->n_spikes == 0) return FBP_OK;
+    if (!indices || !labels) return set_err(h, FBP_E_INVALID, "null output");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(indices, h->sp_idx.p, h->n_spikes * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+    HIP_TRY(hipMemcpyAsync(labels, h->sp_lab.p, h->n_spikes * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    return FBP_OK;
+}
+
+int fbp_copy_clips(fbp_handle* h, float* clips)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!h->have_result || !h->have_clips)
+        return set_err(h, FBP_E_STATE, "no clips: run with FBP_FLAG_ADJUSTED_CLIPS first");
+    if (h->n_spikes == 0) return FBP_OK;
+    if (!clips) return set_err(h, FBP_E_INVALID, "null output");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(clips, h->clips.p, (size_t)h->n_spikes * h->C * h->L * 4, hipMemcpyDeviceToHost, h->stream),
+            "copy");
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    return FBP_OK;
+}
+
+int fbp_copy_residual(fbp_handle* h, float* out)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
+    if (!out) return set_err(h, FBP_E_INVALID, "null output");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(out, h->V.p, (size_t)h->C * h->N * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    return FBP_OK;
+}
+
+int fbp_get_stats(fbp_handle* h, fbp_stats* out)
+{
+    if (!h || !out) return FBP_E_INVALID;
+    *out = h->stats;
+    return FBP_OK;
+}
+
+}  // extern "C"

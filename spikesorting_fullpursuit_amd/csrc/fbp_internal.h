@@ -1,0 +1,51 @@
+// Internal declarations shared by the HIP kernels and the C-ABI host loop.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fbp {
+
+constexpr int kBlock = 256;          // threads per block for the streaming kernels
+constexpr int kCompactTile = 1024;   // elements per block in ordered compaction
+constexpr uint32_t kNoItem = 0xffffffffu;
+
+// Device view of the template set.  Rows of the compact layout are the
+// non-zero channels of each template, ascending, zero padded to Lp floats.
+struct TemplateSet {
+    const float* T;        // [U][C][L] dense
+    const float* rows;     // [R][Lp] non-zero channel rows
+    const int* nch;        // [U] non-zero channels per template
+    const int* off;        // [U] first row of template u
+    const int* chan;       // [R] channel of row r
+    const uint8_t* nz;     // [U][C] 1 where the template row is not all zero
+    const float* bias;     // [U]
+    const float* thr;      // [U]
+    int U, C, L, Lp;
+};
+
+// Per-chunk pursuit state (all device pointers).
+struct PassState {
+    float* V;              // [C][N] residual, row stride N
+    float* LIK;            // [U][N]
+    uint32_t* best_idx;    // [nwin]
+    uint32_t* best_lab;
+    float* best_lik;
+    uint32_t* obest_idx;
+    uint32_t* obest_lab;
+    uint8_t* check;        // [nwin] check_window_on_next_pass
+    uint8_t* check_snap;   // [nwin]
+    uint8_t* orc;          // [nwin] overlap_recheck
+    uint8_t* giveup;       // [nwin] set by the reassignment step
+    uint8_t* has;          // [nwin] per listed window: spike added this pass
+    uint32_t* W;           // window list
+    uint32_t* F;           // flagged (overlap) list
+    uint32_t* cand_idx;    // [nwin] per listed window: candidate spike
+    uint32_t* cand_lab;
+    uint32_t* counters;    // [8] device counters (see Counter)
+    int64_t N;
+    int64_t nwin;
+};
+
+enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_N = 8 };
+
+}  // namespace fbp

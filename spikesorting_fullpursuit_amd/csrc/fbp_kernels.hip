@@ -1,0 +1,755 @@
+// HIP kernels of the full-binary-pursuit pass loop for gfx950 (CDNA4).
+//
+// Reference behaviour: SF/kernels/binary_pursuit.cl (SF = spikesorting_fullpursuit/),
+// restated with a phase-separated meaning where the reference races.  Every
+// floating-point reduction that feeds a decision uses the same operation order
+// as the CPU restatement (explicit fmaf, ascending channel then sample), so
+// decisions are bit-identical to it.  Built with -ffp-contract=off.
+#include "fbp_internal.h"
+
+namespace fbp {
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// ---------------------------------------------------------------------------
+// Block helpers (256 threads = 4 wave64)
+// ---------------------------------------------------------------------------
+
+// Exclusive prefix of `flag` over the block in thread order; `total` receives
+// the block count.  lds must hold 4 uint32.
+__device__ __forceinline__ uint32_t block_excl_scan_flag(bool flag, uint32_t* lds, uint32_t& total)
+{
+    const uint64_t m = __ballot(flag);
+    const int lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) lds[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / 64; i++) {
+        const uint32_t c = lds[i];
+        woff += (i < wv) ? c : 0u;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return woff + pre;
+}
+
+// (value, key) arg-max with ties to the smaller key.  key == kNoItem means "none".
+__device__ __forceinline__ void argmax_merge(float& v, uint32_t& k, float v2, uint32_t k2)
+{
+    if (k2 == kNoItem) return;
+    if (k == kNoItem || v2 > v || (v2 == v && k2 < k)) { v = v2; k = k2; }
+}
+
+__device__ __forceinline__ void wave_argmax(float& v, uint32_t& k)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float v2 = __shfl_xor(v, off, 64);
+        const uint32_t k2 = __shfl_xor(k, off, 64);
+        argmax_merge(v, k, v2, k2);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Likelihood: compute_full_likelihood + compute_maximum_likelihood,
+// binary_pursuit.cl:216-300.  One block handles `wb` listed windows; a work
+// item is (window, template, group of 8 consecutive positions) and slides a
+// register window over the voltage row staged in LDS.
+// ---------------------------------------------------------------------------
+
+constexpr int kPos = 8;  // positions per work item
+
+template <int LT>
+__device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, const float* vrow, float* acc)
+{
+    float r[LT + kPos - 1];
+#pragma unroll
+    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
+#pragma unroll
+    for (int i = 0; i < LT; i += 4) {
+        const float4 tv = *reinterpret_cast<const float4*>(trow + i);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.x, r[i + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.y, r[i + 1 + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.z, r[i + 2 + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.w, r[i + 3 + q], acc[q]);
+    }
+}
+
+__device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, const float* vrow, int L, float* acc)
+{
+    float r[kPos];
+#pragma unroll
+    for (int q = 0; q < kPos; q++) r[q] = vrow[q];
+    for (int i = 0; i < L; i++) {
+        const float tv = trow[i];
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
+        r[kPos - 1] = vrow[i + kPos];
+    }
+}
+
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
+                                                       const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
+                                                       float* __restrict__ LIK)
+{
+    extern __shared__ float sV[];  // [wb][C][sv]
+    const int C = ts.C, U = ts.U;
+    const int L = (LT > 0) ? LT : ts.L;
+    const int ngr = (L + kPos - 1) / kPos;
+    const int per_win = U * ngr;
+    const uint32_t nbatch = (nW + wb - 1) / wb;
+    for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+        const uint32_t w0 = bt * wb;
+        const int nb = (int)min((uint32_t)wb, nW - w0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < nb * C * sv; e += blockDim.x) {
+            const int b = e / (C * sv);
+            const int rem = e - b * C * sv;
+            const int c = rem / sv, j = rem - c * sv;
+            const int64_t x = (int64_t)W[w0 + b] * L + j;
+            sV[e] = (x < N) ? V[(size_t)c * N + x] : 0.0f;
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
+            const int b = it / per_win;
+            const int rem = it - b * per_win;
+            const int t = rem / ngr, g = rem - t * ngr;
+            const int p0 = g * kPos;
+            const int64_t x0 = (int64_t)W[w0 + b] * L + p0;
+            float acc[kPos];
+            const float bt0 = ts.bias[t];
+#pragma unroll
+            for (int q = 0; q < kPos; q++) acc[q] = bt0;
+            const int r0 = ts.off[t], nr = ts.nch[t];
+            for (int k = 0; k < nr; k++) {
+                const int c = ts.chan[r0 + k];
+                const float* trow = ts.rows + (size_t)(r0 + k) * ts.Lp;
+                const float* vrow = sV + (b * C + c) * sv + p0;
+                if (LT > 0) lik_row_fixed<(LT > 0 ? LT : 4)>(trow, vrow, acc);
+                else lik_row_generic(trow, vrow, L, acc);
+            }
+            float* out = LIK + (size_t)t * N;
+#pragma unroll
+            for (int q = 0; q < kPos; q++) {
+                const int64_t x = x0 + q;
+                if (p0 + q < L && x < N) out[x] = (x + L >= N) ? 0.0f : acc[q];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Window maximum: compute_template_maximum_likelihood launched for every
+// template in order, binary_pursuit.cl:350-441.  One wave per listed window;
+// lexicographic arg-max (value, then template, then position).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, const float* __restrict__ LIK,
+                                                       const uint32_t* __restrict__ W, uint32_t nW, PassState st)
+{
+    const uint32_t wpb = kBlock / 64;
+    for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < nW; w += gridDim.x * wpb) {
+        const int64_t id = W[w];
+        const int64_t sw = id * L, ew = (id + 1) * L;
+        if (ew >= N - L) continue;  // binary_pursuit.cl:380-383
+        const int64_t start = (L > sw) ? 0 : sw - L;
+        const int64_t stop = (ew + L > N) ? N : ew + L;
+        float best = 0.0f;
+        uint32_t key = kNoItem;  // t * span + (x - start)
+        const uint32_t span = (uint32_t)(stop - start);
+        bool cw = false;
+        for (int t = 0; t < U; t++) {
+            const float* lr = LIK + (size_t)t * N;
+            for (int64_t x = start + lane_id(); x < stop; x += 64) {
+                const float v = lr[x];
+                if (v > best) { best = v; key = (uint32_t)t * span + (uint32_t)(x - start); }
+                cw |= (v > 0.0f) && (x >= sw) && (x < ew);
+            }
+        }
+        wave_argmax(best, key);
+        const bool any_cw = __ballot(cw) != 0ull;
+        if (lane_id() == 0) {
+            uint32_t lab = st.best_lab[id], idx = st.best_idx[id];
+            float bl = 0.0f;
+            if (key != kNoItem) {
+                bl = best;
+                lab = key / span;
+                idx = (uint32_t)(start + (key - lab * span));
+            }
+            if (any_cw) st.check[id] = 1;
+            st.orc[id] = (bl > 0.0f && (int64_t)idx >= sw && (int64_t)idx < ew) ? 1 : 0;
+            st.best_lik[id] = bl;
+            st.best_lab[id] = lab;
+            st.best_idx[id] = idx;
+            st.obest_idx[id] = idx;
+            st.obest_lab[id] = lab;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered stream compaction (three kernels: count, scan, scatter).  Output
+// order equals input order, so lists stay sorted by window.
+// ---------------------------------------------------------------------------
+struct SelFlagged {  // F = [W[i] : orc[W[i]]]
+    const uint32_t* W; const uint8_t* orc; uint32_t* F;
+    __device__ bool pred(uint32_t e) const { return orc[W[e]] != 0; }
+    __device__ void emit(uint32_t e, uint32_t pos) const { F[pos] = W[e]; }
+    __device__ void always(uint32_t) const {}
+};
+struct SelSpikes {   // spikes of this pass, appended at `base`
+    const uint8_t* has; const uint32_t* ci; const uint32_t* cl; uint32_t* oi; uint32_t* ol; uint64_t base;
+    __device__ bool pred(uint32_t e) const { return has[e] != 0; }
+    __device__ void emit(uint32_t e, uint32_t pos) const { oi[base + pos] = ci[e]; ol[base + pos] = cl[e]; }
+    __device__ void always(uint32_t) const {}
+};
+struct SelNextWindows {  // W' = nonzero(check); resets check and overlap flags
+    uint8_t* check; uint8_t* orc; uint32_t* W;
+    __device__ bool pred(uint32_t e) const { return check[e] != 0; }
+    __device__ void emit(uint32_t e, uint32_t pos) const { W[pos] = e; }
+    __device__ void always(uint32_t e) const { check[e] = 0; orc[e] = 0; }
+};
+
+template <class Sel>
+__global__ __launch_bounds__(kBlock) void k_compact_count(Sel sel, uint32_t n, uint32_t* blockcnt)
+{
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint32_t base = blockIdx.x * kCompactTile;
+    uint32_t tot_all = 0;
+    for (int j = 0; j < kCompactTile / kBlock; j++) {
+        const uint32_t e = base + j * kBlock + threadIdx.x;
+        const bool f = e < n && sel.pred(e);
+        uint32_t tot;
+        block_excl_scan_flag(f, lds, tot);
+        tot_all += tot;
+    }
+    if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot_all;
+}
+
+// Exclusive scan of nb block counts in place; total into *out_total.
+__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* blockcnt, uint32_t nb, uint32_t* out_total)
+{
+    __shared__ uint32_t part[1024];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = (i < nb) ? blockcnt[i] : 0u;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t a = (threadIdx.x >= (uint32_t)off) ? part[threadIdx.x - off] : 0u;
+            __syncthreads();
+            part[threadIdx.x] += a;
+            __syncthreads();
+        }
+        if (i < nb) blockcnt[i] = carry + part[threadIdx.x] - v;
+        const uint32_t chunk_total = part[1023];
+        __syncthreads();
+        carry += chunk_total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+template <class Sel>
+__global__ __launch_bounds__(kBlock) void k_compact_scatter(Sel sel, uint32_t n, const uint32_t* blockoff)
+{
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint32_t base = blockIdx.x * kCompactTile;
+    uint32_t run = blockoff[blockIdx.x];
+    for (int j = 0; j < kCompactTile / kBlock; j++) {
+        const uint32_t e = base + j * kBlock + threadIdx.x;
+        const bool inb = e < n;
+        const bool f = inb && sel.pred(e);
+        uint32_t tot;
+        const uint32_t pos = block_excl_scan_flag(f, lds, tot);
+        if (f) sel.emit(e, run + pos);
+        if (inb) sel.always(e);
+        run += tot;
+    }
+}
+
+template <class Sel>
+static void compact(Sel sel, uint32_t n, uint32_t* blockbuf, uint32_t* out_total, hipStream_t s)
+{
+    const uint32_t nb = (n + kCompactTile - 1) / kCompactTile;
+    if (nb == 0) {
+        hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, s, blockbuf, 0u, out_total);
+        return;
+    }
+    hipLaunchKernelGGL(k_compact_count<Sel>, dim3(nb), dim3(kBlock), 0, s, sel, n, blockbuf);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, s, blockbuf, nb, out_total);
+    hipLaunchKernelGGL(k_compact_scatter<Sel>, dim3(nb), dim3(kBlock), 0, s, sel, n, blockbuf);
+}
+
+// ---------------------------------------------------------------------------
+// Overlap recheck, literal form: overlap_recheck_indices,
+// binary_pursuit.cl:444-654.  A block evaluates 256*kItemsPerThread items of
+// one flagged window and writes its arg-max (lowest item id on ties).
+// ---------------------------------------------------------------------------
+constexpr int kItemsPerThread = 4;
+
+__device__ __forceinline__ bool recheck_item_skipped(const float* __restrict__ LIK, int64_t N, int L, uint32_t lab,
+                                                     uint32_t u, int64_t b, int64_t f, int64_t s, int64_t n)
+{
+    if (f + b < n || f + b >= N - L + n) return true;
+    if (s + b < n || s + b >= N - L + n) return true;
+    if (f > s && f - s > L) return true;
+    if (s > f && s - f > L) return true;
+    const int64_t af = b + f - n, as = b + s - n;
+    return LIK[(size_t)lab * N + af] <= 0.0f && LIK[(size_t)u * N + as] <= 0.0f;
+}
+
+// Exact FP32 value of item (u, f, s), binary_pursuit.cl:578-632.  vw points at
+// the window's voltage rows: vw[c * vstride + j] = V[c, b + j].
+__device__ __forceinline__ float recheck_item_value(const TemplateSet& ts, const float* vw, int vstride,
+                                                    uint32_t lab, uint32_t u, int64_t f, int64_t s, int64_t n)
+{
+    const int C = ts.C, L = ts.L;
+    const int fa = (int)(n - f), sa = (int)(n - s);
+    const int ffj = fa < 0 ? -fa : 0, sfj = sa < 0 ? -sa : 0;
+    float lik = 0.0f;
+    for (int c = 0; c < C; c++) {
+        const bool nzf = ts.nz[(size_t)lab * C + c] != 0;
+        const bool nzs = ts.nz[(size_t)u * C + c] != 0;
+        if (!nzf && !nzs) continue;
+        const float* tf = ts.T + ((size_t)lab * C + c) * L;
+        const float* tsr = ts.T + ((size_t)u * C + c) * L;
+        const float* vr = vw + (size_t)c * vstride;
+        float ss = 0.0f;
+        for (int j = 0; j < L; j++) {
+            const float fv = (j >= ffj && fa + j < L) ? tf[fa + j] : 0.0f;
+            const float sv = (j >= sfj && sa + j < L) ? tsr[sa + j] : 0.0f;
+            const float sum = fv + sv;
+            ss = fmaf(sum, sum, ss);
+            lik = fmaf(sum, vr[j], lik);
+        }
+        if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
+    }
+    return lik;
+}
+
+__global__ __launch_bounds__(kBlock) void k_recheck_literal(TemplateSet ts, PassState st, int64_t n_max,
+                                                            uint32_t nchunk, int use_lds,
+                                                            float* __restrict__ part_v, uint32_t* __restrict__ part_k)
+{
+    extern __shared__ float sW[];  // [C][L] voltage of the window
+    __shared__ float red_v[kBlock / 64];
+    __shared__ uint32_t red_k[kBlock / 64];
+    const uint32_t nF = st.counters[CNT_F];
+    const int64_t N = st.N;
+    const int L = ts.L, C = ts.C;
+    const int64_t U = ts.U, S = 2 * n_max + 1;
+    const uint64_t items = (uint64_t)S * S * U;
+    const uint64_t total_blocks = (uint64_t)nF * nchunk;
+    for (uint64_t blk = blockIdx.x; blk < total_blocks; blk += gridDim.x) {
+        const uint32_t w = (uint32_t)(blk / nchunk);
+        const uint32_t ch = (uint32_t)(blk - (uint64_t)w * nchunk);
+        const uint32_t id = st.F[w];
+        const uint32_t lab = st.best_lab[id];
+        const int64_t b = st.best_idx[id];
+        const float lb = st.LIK[(size_t)lab * N + b];
+        const bool wskip = (lb < ts.thr[lab]) || (b >= N - L);
+        const float* vw;
+        int vstride;
+        __syncthreads();
+        if (use_lds) {
+            if (!wskip)
+                for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+                    const int c = e / L, j = e - c * L;
+                    sW[e] = st.V[(size_t)c * N + b + j];
+                }
+            __syncthreads();
+            vw = sW; vstride = L;
+        } else {
+            vw = st.V + b; vstride = (int)N;
+        }
+        float bv = 0.0f;
+        uint32_t bk = kNoItem;
+        if (!wskip) {
+            const uint64_t k0 = (uint64_t)ch * kBlock * kItemsPerThread;
+            for (int r = 0; r < kItemsPerThread; r++) {
+                const uint64_t k = k0 + (uint64_t)r * kBlock + threadIdx.x;
+                if (k >= items) break;
+                const uint32_t u = (uint32_t)(k % U);
+                const int64_t f = (int64_t)((k / U) % S), s = (int64_t)(k / (S * U));
+                if (recheck_item_skipped(st.LIK, N, L, lab, u, b, f, s, n_max)) continue;
+                const float v = recheck_item_value(ts, vw, vstride, lab, u, f, s, n_max);
+                argmax_merge(bv, bk, v, (uint32_t)k);
+            }
+        }
+        wave_argmax(bv, bk);
+        if (lane_id() == 0) { red_v[threadIdx.x >> 6] = bv; red_k[threadIdx.x >> 6] = bk; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = red_v[0];
+            uint32_t k = red_k[0];
+            for (int i = 1; i < kBlock / 64; i++) argmax_merge(v, k, red_v[i], red_k[i]);
+            part_v[blk] = v;
+            part_k[blk] = k;
+        }
+    }
+}
+
+// parse_overlap_recheck_indices, binary_pursuit.cl:657-815: one wave per
+// flagged window reduces its partial maxima and applies the accept cascade.
+__global__ __launch_bounds__(kBlock) void k_recheck_parse(TemplateSet ts, PassState st, int64_t n_max, uint32_t nchunk,
+                                                          const float* __restrict__ part_v,
+                                                          const uint32_t* __restrict__ part_k)
+{
+    const uint32_t nF = st.counters[CNT_F];
+    const uint32_t wpb = kBlock / 64;
+    const int64_t N = st.N;
+    const int64_t U = ts.U, S = 2 * n_max + 1;
+    for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < nF; w += gridDim.x * wpb) {
+        float v = 0.0f;
+        uint32_t k = kNoItem;
+        for (uint32_t c = lane_id(); c < nchunk; c += 64)
+            argmax_merge(v, k, part_v[(size_t)w * nchunk + c], part_k[(size_t)w * nchunk + c]);
+        wave_argmax(v, k);
+        if (lane_id() != 0) continue;
+        const uint32_t id = st.F[w];
+        const float bl = st.best_lik[id];
+        if (k == kNoItem || !(v > bl) || v <= 0.0f) { st.orc[id] = 0; continue; }
+        const uint32_t lab = st.best_lab[id];
+        const int64_t b = st.best_idx[id];
+        const uint32_t u = (uint32_t)(k % U);
+        const int64_t f = (int64_t)((k / U) % S), s = (int64_t)(k / (S * U));
+        const int64_t af = b + f - n_max, as = b + s - n_max;
+        const float A = st.LIK[(size_t)lab * N + af];
+        const float B = st.LIK[(size_t)u * N + as];
+        if (v < A || v < B || A <= 0.0f || B <= 0.0f) { st.orc[id] = 0; continue; }
+        if (A >= B && A > ts.thr[lab]) {
+            st.best_lik[id] = A; st.obest_lab[id] = lab; st.obest_idx[id] = (uint32_t)af;
+        } else if (B > A && B > ts.thr[u]) {
+            st.best_lik[id] = B; st.obest_lab[id] = u; st.obest_idx[id] = (uint32_t)as;
+        } else if (A > ts.thr[lab]) {
+            st.best_lik[id] = A; st.obest_lab[id] = lab; st.obest_idx[id] = (uint32_t)af;
+        } else {
+            st.obest_idx[id] = st.best_idx[id];
+            st.obest_lab[id] = st.best_lab[id];
+            st.orc[id] = 0;
+        }
+    }
+}
+
+// check_overlap_reassignments, binary_pursuit.cl:822-875.  orc is only read
+// here (the post-parse snapshot); giving up is recorded in `giveup`.
+__global__ __launch_bounds__(kBlock) void k_reassign(int L, PassState st)
+{
+    const uint32_t nF = st.counters[CNT_F];
+    const int64_t nwf = st.N / L;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nF; w += gridDim.x * blockDim.x) {
+        const int64_t id = st.F[w];
+        uint8_t gu = 0;
+        if (st.orc[id] != 0) {
+            const int64_t sw = id * L, ew = (id + 1) * L;
+            const uint32_t oi = st.obest_idx[id];
+            st.best_idx[id] = oi;
+            st.best_lab[id] = st.obest_lab[id];
+            if ((int64_t)oi < sw && id > 1) {
+                if (st.orc[id - 2] == 1) {
+                    gu = 1;
+                    st.best_idx[id] = 0;
+                    st.check[id - 1] = 2;
+                    st.check[id] = 2;
+                    st.check[id + 1] = 2;
+                } else {
+                    st.check[id - 2] = 2;
+                }
+            }
+            if ((int64_t)oi >= ew && id + 2 < nwf) st.check[id + 2] = 2;
+        }
+        st.giveup[id] = gu;
+    }
+}
+
+// binary_pursuit kernel phase A (binary_pursuit.cl:960-1004): add decisions
+// and the "=3" flags.  `use_giveup` is 0 when no recheck ran this pass.
+__global__ __launch_bounds__(kBlock) void k_pursuit_decide(int L, const float* __restrict__ thr, PassState st,
+                                                           uint32_t nW, int use_giveup)
+{
+    const int64_t N = st.N, nwf = N / L;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nW; w += gridDim.x * blockDim.x) {
+        const int64_t id = st.W[w];
+        const int64_t sw = id * L, ew = (id + 1) * L;
+        float ml = 0.0f;
+        uint32_t lab = 0, idx = 0;
+        if (ew < N - L) { ml = st.best_lik[id]; lab = st.best_lab[id]; idx = st.best_idx[id]; }
+        const bool orc = st.orc[id] == 1 && !(use_giveup && st.giveup[id]);
+        uint8_t h = 0;
+        if ((ml > thr[lab]) || orc) {
+            if (((int64_t)idx >= sw && (int64_t)idx < ew) || orc) {
+                h = 1;
+                st.check[id] = 3;
+                if (id > 0) st.check[id - 1] = 3;
+                if (id + 1 < nwf) st.check[id + 1] = 3;
+            }
+        }
+        st.has[w] = h;
+        st.cand_idx[w] = idx;
+        st.cand_lab[w] = lab;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_bytes(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// binary_pursuit kernel phase B (binary_pursuit.cl:1009-1045): pruning of
+// windows that added nothing, reading neighbour flags from the snapshot.
+__global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __restrict__ thr, PassState st, uint32_t nW)
+{
+    const int64_t N = st.N, nwf = N / L;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nW; w += gridDim.x * blockDim.x) {
+        const int64_t id = st.W[w];
+        const int64_t sw = id * L, ew = (id + 1) * L;
+        if (!(ew < N - L) || st.has[w]) continue;
+        const float ml = st.best_lik[id];
+        const uint32_t lab = st.best_lab[id];
+        const int64_t idx = st.best_idx[id];
+        uint8_t own = st.check_snap[id];
+        if (idx < sw && own == 1 && id > 0 && st.check_snap[id - 1] == 0) own = 0;
+        if (idx >= ew && own == 1 && id + 1 < nwf && st.check_snap[id + 1] == 0) own = 0;
+        if (ml <= thr[lab] && own < 3) own = 0;
+        st.check[id] = own;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Residual: compute_residual, binary_pursuit.cl:57-102, without atomics.
+// Every sample covered by several spikes of the pass is written once, by the
+// first covering spike in list order, which subtracts all covering templates
+// in list order.  Covering spikes lie within `reach` list entries.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __restrict__ V, int64_t N,
+                                                     const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ slab,
+                                                     uint32_t nsp, int reach)
+{
+    const int C = ts.C, L = ts.L;
+    for (uint32_t k = blockIdx.x; k < nsp; k += gridDim.x) {
+        const int64_t pk = sidx[k];
+        if (slab[k] >= (uint32_t)ts.U) continue;
+        const uint32_t jlo = (k > (uint32_t)reach) ? k - reach : 0u;
+        const uint32_t jhi = min(nsp - 1, k + (uint32_t)reach);
+        for (int i = threadIdx.x; i < L; i += blockDim.x) {
+            const int64_t x = pk + i;
+            if (x >= N) continue;
+            bool owner = true;
+            for (uint32_t j = jlo; j < k; j++) {
+                const int64_t pj = sidx[j];
+                if (slab[j] < (uint32_t)ts.U && x >= pj && x < pj + L) { owner = false; break; }
+            }
+            if (!owner) continue;
+            for (int c = 0; c < C; c++) {
+                float v = V[(size_t)c * N + x];
+                for (uint32_t j = k; j <= jhi; j++) {
+                    const int64_t pj = sidx[j];
+                    const uint32_t lj = slab[j];
+                    if (lj >= (uint32_t)ts.U || x < pj || x >= pj + L) continue;
+                    v = v - ts.T[((size_t)lj * C + c) * L + (x - pj)];
+                }
+                V[(size_t)c * N + x] = v;
+            }
+        }
+    }
+}
+
+// get_adjusted_clips, binary_pursuit.cl:1088-1130.
+__global__ __launch_bounds__(kBlock) void k_adjusted_clips(TemplateSet ts, const float* __restrict__ V, int64_t N,
+                                                           const uint32_t* __restrict__ sidx,
+                                                           const uint32_t* __restrict__ slab, uint64_t nsp,
+                                                           float* __restrict__ clips)
+{
+    const int C = ts.C, L = ts.L;
+    const uint64_t per = (uint64_t)C * L;
+    const uint64_t total = nsp * per;
+    for (uint64_t e = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; e < total; e += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = e / per;
+        const int rem = (int)(e - k * per);
+        const int c = rem / L, i = rem - c * L;
+        const int64_t x = (int64_t)sidx[k] + i;
+        float out = 0.0f;
+        if (slab[k] < (uint32_t)ts.U && x < N)
+            out = V[(size_t)c * N + x] + ts.T[((size_t)slab[k] * C + c) * L + i];
+        clips[e] = out;
+    }
+}
+
+// Zero fill (used instead of hipMemsetAsync so the sequence stays capturable).
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* p, uint32_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ __launch_bounds__(kBlock) void k_fill_u8(uint8_t* p, uint8_t v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ __launch_bounds__(kBlock) void k_iota(uint32_t* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
+}
+__global__ __launch_bounds__(kBlock) void k_copy_rows(const float* __restrict__ src, int64_t ld, float* __restrict__ dst,
+                                                      int64_t C, int64_t N)
+{
+    const int64_t total = C * N;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / N, x = e - c * N;
+        dst[e] = src[c * ld + x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int64_t per_block, unsigned cap = 65535u * 16u)
+{
+    int64_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, v, n);
+}
+void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, v, n);
+}
+void launch_iota(uint32_t* p, int64_t n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, n);
+}
+void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(C * N, kBlock * 8)), dim3(kBlock), 0, s, src, ld, dst, C, N);
+}
+
+// Windows per likelihood block and LDS row length.
+void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
+{
+    const int L = ts.L;
+    *sv = 2 * L + kPos + 1;
+    const int ngr = (L + kPos - 1) / kPos;
+    const int per_win = ts.U * ngr;
+    int w = (2 * kBlock + per_win - 1) / per_win;
+    if (w < 1) w = 1;
+    const int64_t lds_per = (int64_t)ts.C * (*sv) * 4;
+    while (w > 1 && w * lds_per > 48 * 1024) w--;
+    *wb = w;
+}
+
+bool likelihood_fits(const TemplateSet& ts)
+{
+    int wb, sv;
+    likelihood_plan(ts, &wb, &sv);
+    return (int64_t)wb * ts.C * sv * 4 <= 64 * 1024;
+}
+
+void launch_likelihood(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* W, uint32_t nW, float* LIK,
+                       hipStream_t s)
+{
+    if (nW == 0) return;
+    int wb, sv;
+    likelihood_plan(ts, &wb, &sv);
+    const size_t lds = (size_t)wb * ts.C * sv * sizeof(float);
+    const uint32_t nbatch = (nW + wb - 1) / wb;
+    const dim3 g(grid_for(nbatch, 1, 1u << 20)), b(kBlock);
+    if (ts.L == 48) hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
+    else hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
+}
+
+void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
+{
+    if (nW == 0) return;
+    hipLaunchKernelGGL(k_window_max, dim3(grid_for(nW, kBlock / 64, 1u << 20)), dim3(kBlock), 0, s, ts.U, ts.L, st.N,
+                       st.LIK, st.W, nW, st);
+}
+
+void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s)
+{
+    SelFlagged sel{st.W, st.orc, st.F};
+    compact(sel, nW, blockbuf, st.counters + CNT_F, s);
+}
+
+void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
+                          uint32_t* blockbuf, hipStream_t s)
+{
+    SelSpikes sel{st.has, st.cand_idx, st.cand_lab, out_idx, out_lab, base};
+    compact(sel, nW, blockbuf, st.counters + CNT_SPIKES, s);
+}
+
+void launch_next_windows(const PassState& st, uint32_t* blockbuf, hipStream_t s)
+{
+    SelNextWindows sel{st.check, st.orc, st.W};
+    compact(sel, (uint32_t)st.nwin, blockbuf, st.counters + CNT_W, s);
+}
+
+uint32_t recheck_literal_chunks(const TemplateSet& ts, int64_t n_max)
+{
+    const uint64_t S = 2 * (uint64_t)n_max + 1;
+    const uint64_t items = S * S * (uint64_t)ts.U;
+    return (uint32_t)((items + kBlock * kItemsPerThread - 1) / (kBlock * kItemsPerThread));
+}
+
+void launch_recheck_literal(const TemplateSet& ts, const PassState& st, int64_t n_max, uint32_t nF, float* part_v,
+                            uint32_t* part_k, hipStream_t s)
+{
+    if (nF == 0) return;
+    const uint32_t nchunk = recheck_literal_chunks(ts, n_max);
+    const size_t lds_need = (size_t)ts.C * ts.L * sizeof(float);
+    const int use_lds = lds_need <= 48 * 1024;
+    const uint64_t blocks = (uint64_t)nF * nchunk;
+    hipLaunchKernelGGL(k_recheck_literal, dim3(grid_for((int64_t)blocks, 1, 1u << 21)), dim3(kBlock),
+                       use_lds ? lds_need : 0, s, ts, st, n_max, nchunk, use_lds, part_v, part_k);
+    hipLaunchKernelGGL(k_recheck_parse, dim3(grid_for(nF, kBlock / 64, 1u << 20)), dim3(kBlock), 0, s, ts, st, n_max,
+                       nchunk, part_v, part_k);
+}
+
+void launch_reassign(const TemplateSet& ts, const PassState& st, uint32_t nF, hipStream_t s)
+{
+    if (nF == 0) return;
+    hipLaunchKernelGGL(k_reassign, dim3(grid_for(nF, kBlock, 1u << 20)), dim3(kBlock), 0, s, ts.L, st);
+}
+
+void launch_pursuit(const TemplateSet& ts, const PassState& st, uint32_t nW, int use_giveup, hipStream_t s)
+{
+    if (nW == 0) return;
+    const unsigned g = grid_for(nW, kBlock, 1u << 20);
+    hipLaunchKernelGGL(k_pursuit_decide, dim3(g), dim3(kBlock), 0, s, ts.L, ts.thr, st, nW, use_giveup);
+    hipLaunchKernelGGL(k_copy_bytes, dim3(grid_for(st.nwin, kBlock * 4)), dim3(kBlock), 0, s, st.check, st.check_snap,
+                       st.nwin);
+    hipLaunchKernelGGL(k_pursuit_prune, dim3(g), dim3(kBlock), 0, s, ts.L, ts.thr, st, nW);
+}
+
+void launch_residual(const TemplateSet& ts, float* V, int64_t N, const uint32_t* sidx, const uint32_t* slab, uint32_t nsp,
+                     int reach, hipStream_t s)
+{
+    if (nsp == 0) return;
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(nsp, 1, 1u << 20)), dim3(ts.L <= 64 ? 64 : kBlock), 0, s, ts, V, N,
+                       sidx, slab, nsp, reach);
+}
+
+void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* sidx, const uint32_t* slab,
+                           uint64_t nsp, float* clips, hipStream_t s)
+{
+    if (nsp == 0) return;
+    hipLaunchKernelGGL(k_adjusted_clips, dim3(grid_for((int64_t)(nsp * ts.C * ts.L), kBlock * 4)), dim3(kBlock), 0, s,
+                       ts, V, N, sidx, slab, nsp, clips);
+}
+
+}  // namespace fbp

@@ -1,0 +1,175 @@
+"""Drop-in replacement of the reference's binary_pursuit() on MI355X.
+
+Same call signature, inputs read and return arrays as
+SF/overlap/binary_pursuit_parallel.py:175-1532 (SF = spikesorting_fullpursuit/).
+The device work runs in libfbp_hip.so (HIP kernels for gfx950, C ABI in
+include/fbp.h); this module keeps the reference's host logic: argument
+handling, the GPU-memory chunk plan and the output assembly.
+"""
+import numpy as np
+
+from .. import _native
+from ..processing.conversions import time_window_to_samples
+from . import neuron_separability
+
+# Work-group size the reference's memory model divides the overlap items by
+# (CL_KERNEL_WORK_GROUP_SIZE of overlap_recheck_indices,
+# binary_pursuit_parallel.py:336-340); 256 is the AMD OpenCL value.
+OVERLAP_LOCAL_WORK_SIZE = 256
+DEVICE_ADDRESS_BITS = 64
+
+_handles = {}
+
+
+def _handle(device):
+    h = _handles.get(device)
+    if h is None:
+        h = _native.Handle(device)
+        _handles[device] = h
+    return h
+
+
+def release_device_buffers():
+    """Free the cached device handles and their buffers."""
+    for h in list(_handles.values()):
+        h.close()
+    _handles.clear()
+
+
+def chunk_plan(n_chans, n_samples, sampling_rate, n_templates, template_samples_per_chan,
+               n_max_shift_inds, gpu_total_memory, max_gpu_memory=None, template_bytes=None):
+    """Chunk length and onsets, binary_pursuit_parallel.py:387-507.
+
+    Returns (num_indices_per_chunk, chunk_onsets).  Raises ValueError when a
+    chunk would hold fewer than 4 template widths (:472-475).
+    """
+    gpu_memory_size = float(gpu_total_memory)
+    gpu_memory_size = max(gpu_memory_size * 0.5, gpu_memory_size - (1024 * 1024 * 1024))
+    if max_gpu_memory is not None:
+        gpu_memory_size = min(gpu_memory_size, max_gpu_memory)
+    assert gpu_memory_size > 0
+    L = int(template_samples_per_chan)
+    items_per_index = np.int64((2 * int(n_max_shift_inds) + 1) ** 2 * int(n_templates))
+    n_local_ID = np.int64(((items_per_index - 1) / OVERLAP_LOCAL_WORK_SIZE) + 1)
+    if template_bytes is None:
+        template_bytes = 4 * int(n_templates) * int(n_chans) * L
+    constant_memory_usage = template_bytes + 4 * int(n_templates)
+    memory_usage_per_second = n_chans * sampling_rate * 4.0
+    memory_usage_per_second += n_templates * sampling_rate * 4.0
+    memory_usage_per_second += (sampling_rate / L) * (
+        4 + 4 + 4 + 4 + 4 + (n_local_ID / 3) * 4 + (n_local_ID / 3) * 4)
+    num_seconds_per_chunk = (gpu_memory_size - constant_memory_usage) / memory_usage_per_second
+    max_addressable_seconds = ((1 << DEVICE_ADDRESS_BITS) - 1) / (4.0 * sampling_rate * n_chans)
+    num_seconds_per_chunk = min(num_seconds_per_chunk, np.floor(max_addressable_seconds))
+    num_indices_per_chunk = int(np.floor(num_seconds_per_chunk * sampling_rate))
+    if num_indices_per_chunk < 4 * L:
+        raise ValueError(
+            "Cannot fit enough data on GPU to run binary pursuit. Decrease template samples or get a new GPU.")
+    onsets = []
+    curr = 0
+    while curr < n_samples - L:
+        onsets.append(curr)
+        curr += num_indices_per_chunk - 3 * L
+    return num_indices_per_chunk, onsets
+
+
+def _is_device_tensor(x):
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
+                   kernels_path=None, max_gpu_memory=None, *, device=0, return_residual=False,
+                   recheck_literal=False, stats=None):
+    """Binary pursuit over every chunk of `voltage` ([C][N], any real dtype;
+    a float32 torch tensor already on the GPU is used in place).
+
+    Returns (event_indices int64, neuron_labels int64, binary_pursuit_spike_bool,
+    adjusted_clips) exactly as binary_pursuit_parallel.py:1506-1532, or four
+    empty lists when nothing is found.  `kernels_path` is accepted and ignored
+    (there is no OpenCL source).  Keyword-only extras: `return_residual` appends
+    the list of per-chunk residual voltages; `stats` (a list) receives the
+    device statistics of every chunk.
+    """
+    n_chans = int(voltage.shape[0])
+    n_samples = int(voltage.shape[1])
+    sampling_rate = sort_info["sampling_rate"]
+    chan_win, _ = time_window_to_samples(sort_info["clip_width"], sampling_rate)
+    template_samples_per_chan = int(sort_info["n_samples_per_chan"])
+
+    templates = separability_metrics["templates"]
+    if templates.ndim == 1:
+        templates = np.reshape(templates, (1, -1))
+    templates = np.float32(templates)
+    n_templates = templates.shape[0]
+    if templates.shape[1] != n_chans * template_samples_per_chan:
+        raise ValueError("templates must have n_channels * n_samples_per_chan columns")
+    clip_init_samples = int(np.abs(chan_win[0]))
+    template_sum_squared = np.float32(-0.5 * separability_metrics["template_SS"])
+    _ = np.float32(separability_metrics["template_SS_by_chan"].flatten(order="C"))
+    neuron_lower_thresholds = np.float32(separability_metrics["neuron_lower_thresholds"])
+    do_overlap_recheck = sort_info["do_overlap_recheck"]
+    get_adjusted_clips = sort_info["get_adjusted_clips"]
+
+    h = _handle(device)
+    if n_max_shift_inds is None or n_max_shift_inds < 0:
+        n_max_shift_inds = template_samples_per_chan // 2
+    n_max_shift_inds = np.uint32(n_max_shift_inds)
+
+    num_indices_per_chunk, chunk_onsets = chunk_plan(
+        n_chans, n_samples, sampling_rate, n_templates, template_samples_per_chan, n_max_shift_inds,
+        h.device_memory(), max_gpu_memory, template_bytes=templates.nbytes)
+
+    T3 = templates.reshape(n_templates, n_chans, template_samples_per_chan)
+    h.set_templates(T3, np.atleast_1d(template_sum_squared), np.atleast_1d(neuron_lower_thresholds))
+    flags = 0
+    if do_overlap_recheck:
+        flags |= _native.FLAG_OVERLAP_RECHECK
+    if get_adjusted_clips:
+        flags |= _native.FLAG_ADJUSTED_CLIPS
+    if recheck_literal:
+        flags |= _native.FLAG_RECHECK_LITERAL
+
+    on_device = _is_device_tensor(voltage)
+    if on_device:
+        import torch
+        if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
+            voltage = voltage.to(torch.float32).contiguous()
+        # the library works on its own stream: the producer's writes must be done
+        torch.cuda.current_stream(voltage.device).synchronize()
+
+    secret_spike_indices, secret_spike_labels, secret_spike_bool = [], [], []
+    adjusted_spike_clips, residuals = [], []
+    for start_index in chunk_onsets:
+        stop_index = min(n_samples, start_index + num_indices_per_chunk)
+        if on_device:
+            h.run(None, n_max_shift_inds, flags, ld=voltage.stride(0), n_samples=stop_index - start_index,
+                  device_ptr=voltage.data_ptr() + 4 * start_index)
+        else:
+            chunk_voltage = np.float32(voltage[:, start_index:stop_index])
+            h.run(chunk_voltage, n_max_shift_inds, flags)
+        idx, lab = h.result()
+        if stats is not None:
+            stats.append(h.stats())
+        if return_residual:
+            residuals.append(h.residual(stop_index - start_index))
+        if idx.shape[0] > 0:
+            clips = h.clips(idx.shape[0]) if get_adjusted_clips else []
+            secret_spike_indices.append(idx.astype(np.int64) + start_index)
+            secret_spike_labels.append(lab)
+            adjusted_spike_clips.append(clips)
+            secret_spike_bool.append(np.ones(idx.shape[0], dtype="bool"))
+
+    if len(secret_spike_indices) > 0:
+        event_indices = np.int64(np.hstack(secret_spike_indices))
+        neuron_labels = np.int64(np.hstack(secret_spike_labels))
+        adjusted_clips = (np.vstack(adjusted_spike_clips)).astype(v_dtype)
+        binary_pursuit_spike_bool = np.hstack(secret_spike_bool)
+        event_indices += clip_init_samples
+    else:
+        event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips = [], [], [], []
+
+    neuron_separability.add_n_spikes(separability_metrics, neuron_labels)
+    out = (event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips)
+    if return_residual:
+        out = out + (residuals,)
+    return out

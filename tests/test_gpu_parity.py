@@ -1,0 +1,213 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Decisions (spike indices, labels, order) must be identical and the residual
+bit-identical: both sides use the same FP32 operation order."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import fbp_oracle
+from spikesorting_fullpursuit_amd import _native, synthetic
+from spikesorting_fullpursuit_amd.overlap import binary_pursuit_parallel as bpp
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def handle():
+    h = _native.Handle(0)
+    yield h
+    h.close()
+
+
+def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal=True):
+    h.set_templates(T, bias, thr)
+    flags = (_native.FLAG_OVERLAP_RECHECK if recheck else 0) | (_native.FLAG_ADJUSTED_CLIPS if clips else 0)
+    if literal:
+        flags |= _native.FLAG_RECHECK_LITERAL
+    print(f"[gpu_run] C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n_max={n_max} "
+          f"recheck={recheck}", flush=True)
+    h.run(V, n_max, flags)
+    idx, lab = h.result()
+    R = h.residual(V.shape[1])
+    out = [idx, lab, R]
+    if clips:
+        out.append(h.clips(len(idx)))
+    return out
+
+
+def assert_same(gpu, ora):
+    gi, gl, gR = gpu[:3]
+    oi, ol, oR = ora[:3]
+    assert len(gi) == len(oi), (len(gi), len(oi))
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gl, ol)
+    np.testing.assert_array_equal(gR, oR)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_noisy_cases_match_oracle(handle, seed):
+    V, T, bias, thr, on, lab = cases.noisy_case(seed)
+    n = T.shape[2] - 1
+    assert_same(gpu_run(handle, V, T, bias, thr, n), fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 5, 9])
+def test_correlated_pairs_match_oracle(handle, seed):
+    U, C, L = 4, 3, 16
+    T = cases.dog_templates(U, C, L, seed).astype(np.float32)
+    V, truth = cases.planted_pairs(T, 2000, 100 + seed, min_d=3)
+    V = V + np.random.default_rng(seed).normal(size=V.shape).astype(np.float32) * 0.2
+    bias, thr = cases.bias_thr(T)
+    assert_same(gpu_run(handle, V, T, bias, thr, L - 1),
+                fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
+
+
+@pytest.mark.parametrize("recheck", [True, False])
+def test_small_workload_3s_matches_oracle(handle, recheck):
+    d = synthetic.generate("small", duration_s=3.0)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = d["voltage"]
+    assert_same(gpu_run(handle, V, T, bias, thr, n, recheck=recheck),
+                fbp_oracle.pursuit(V, T, bias, thr, n, do_recheck=recheck, return_residual=True))
+
+
+def test_adjusted_clips_match_oracle(handle):
+    V, T, bias, thr, on, lab = cases.noisy_case(3)
+    n = T.shape[2] - 1
+    g = gpu_run(handle, V, T, bias, thr, n, clips=True)
+    oi, ol, oR = fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True)
+    np.testing.assert_array_equal(g[3], fbp_oracle.adjusted_clips(oR, T, oi, ol))
+
+
+@pytest.mark.parametrize("n_max", [0, 1, 3, 20])
+def test_shift_range_variants(handle, n_max):
+    V, T, bias, thr, on, lab = cases.noisy_case(20 + n_max, U=3, L=8)
+    assert_same(gpu_run(handle, V, T, bias, thr, n_max),
+                fbp_oracle.pursuit(V, T, bias, thr, n_max, return_residual=True))
+
+
+def test_zero_channel_templates_and_odd_length(handle):
+    rng = np.random.default_rng(8)
+    U, C, L, N = 4, 5, 11, 900
+    T = (rng.normal(size=(U, C, L)) * 4).astype(np.float32)
+    T[0, 1:] = 0
+    T[1, [0, 4]] = 0
+    T[3, 2] = 0
+    V = (rng.normal(size=(C, N)) * 0.3).astype(np.float32)
+    for p in range(20, N - 2 * L, 37):
+        V[:, p:p + L] += T[(p // 37) % U]
+    bias, thr = cases.bias_thr(T.astype(np.float64), 0.05)
+    assert_same(gpu_run(handle, V, T, bias, thr, L - 1),
+                fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
+
+
+def test_short_chunk_edges(handle):
+    # N just above 4 template widths, a spike at the very end and one at 0
+    U, C, L = 2, 2, 8
+    T = cases.random_templates(U, C, L, 4).astype(np.float32)
+    N = 4 * L + 3
+    V = np.zeros((C, N), np.float32)
+    V[:, 0:L] += T[0]
+    V[:, N - L:] += T[1]
+    V[:, 13:13 + L] += T[1]
+    bias, thr = cases.bias_thr(T.astype(np.float64))
+    assert_same(gpu_run(handle, V, T, bias, thr, L - 1),
+                fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
+
+
+def test_empty_result(handle):
+    T = cases.random_templates(3, 2, 8, 0).astype(np.float32)
+    bias, thr = cases.bias_thr(T.astype(np.float64))
+    g = gpu_run(handle, np.zeros((2, 500), np.float32), T, bias, thr, 7)
+    assert len(g[0]) == 0
+
+
+def test_probe16_slice_matches_oracle(handle):
+    d = synthetic.generate("probe16", duration_s=0.5)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = d["voltage"]
+    assert_same(gpu_run(handle, V, T, bias, thr, n),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+def _ref_output(d, chunks=None):
+    """Oracle through the reference's output assembly (single chunk)."""
+    T, bias, thr = synthetic.kernel_inputs(d)
+    idx, lab = fbp_oracle.pursuit(d["voltage"], T, bias, thr, d["n_max_shift_inds"])
+    return idx.astype(np.int64) + d["clip_init_samples"], lab.astype(np.int64)
+
+
+def test_dropin_binary_pursuit_matches_oracle():
+    d = synthetic.generate("small", duration_s=5.0)
+    m = dict(d["separability_metrics"])
+    ev, lab, bpbool, clips = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m,
+                                                n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+    rev, rlab = _ref_output(d)
+    assert ev.dtype == np.int64 and lab.dtype == np.int64 and bpbool.dtype == bool
+    np.testing.assert_array_equal(ev, rev)
+    np.testing.assert_array_equal(lab, rlab)
+    assert bpbool.all()
+    assert clips.shape == (1, 0)  # np.vstack([[]]) as in binary_pursuit_parallel.py:1509
+    np.testing.assert_array_equal(m["n_spikes"], np.bincount(rlab, minlength=m["templates"].shape[0]))
+
+
+def test_dropin_chunked_matches_oracle_per_chunk():
+    d = synthetic.generate("small", duration_s=2.0)
+    si = dict(d["sort_info"])
+    m = dict(d["separability_metrics"])
+    cap = 4 * 2**20
+    ev, lab, _, _ = bpp.binary_pursuit(d["voltage"], np.float32, si, m, n_max_shift_inds=d["n_max_shift_inds"],
+                                       max_gpu_memory=cap, recheck_literal=True)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    V = d["voltage"]
+    L = T.shape[2]
+    n_per, onsets = bpp.chunk_plan(V.shape[0], V.shape[1], si["sampling_rate"], T.shape[0], L,
+                                   d["n_max_shift_inds"], 288 * 2**30, max_gpu_memory=cap)
+    assert len(onsets) > 1
+    rev, rlab = [], []
+    for s0 in onsets:
+        s1 = min(V.shape[1], s0 + n_per)
+        i, l = fbp_oracle.pursuit(V[:, s0:s1], T, bias, thr, d["n_max_shift_inds"])
+        rev.append(i.astype(np.int64) + s0)
+        rlab.append(l.astype(np.int64))
+    np.testing.assert_array_equal(ev, np.concatenate(rev) + d["clip_init_samples"])
+    np.testing.assert_array_equal(lab, np.concatenate(rlab))
+
+
+def test_dropin_adjusted_clips_dtype_and_values():
+    d = synthetic.generate("small", duration_s=1.0)
+    si = dict(d["sort_info"], get_adjusted_clips=True)
+    m = dict(d["separability_metrics"])
+    ev, lab, _, clips = bpp.binary_pursuit(d["voltage"], np.float64, si, m, n_max_shift_inds=d["n_max_shift_inds"],
+                                           recheck_literal=True)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    i, l, R = fbp_oracle.pursuit(d["voltage"], T, bias, thr, d["n_max_shift_inds"], return_residual=True)
+    assert clips.dtype == np.float64
+    np.testing.assert_array_equal(clips, fbp_oracle.adjusted_clips(R, T, i, l).astype(np.float64))
+
+
+def test_dropin_device_resident_voltage_matches_host():
+    import torch
+    d = synthetic.generate("small", duration_s=1.0)
+    m1, m2 = dict(d["separability_metrics"]), dict(d["separability_metrics"])
+    ev1, lab1, _, _ = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m1,
+                                         n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+    vt = torch.from_numpy(d["voltage"]).cuda()
+    ev2, lab2, _, _ = bpp.binary_pursuit(vt, np.float32, d["sort_info"], m2,
+                                         n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+    np.testing.assert_array_equal(ev1, ev2)
+    np.testing.assert_array_equal(lab1, lab2)
+
+
+def test_dropin_no_spikes_returns_empty_lists():
+    d = synthetic.generate("small", duration_s=0.5)
+    m = dict(d["separability_metrics"])
+    out = bpp.binary_pursuit(np.zeros_like(d["voltage"]), np.float32, d["sort_info"], m,
+                             n_max_shift_inds=d["n_max_shift_inds"])
+    assert out == ([], [], [], [])
+    assert np.all(m["n_spikes"] == 0)

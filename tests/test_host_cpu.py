@@ -1,0 +1,121 @@
+"""Host logic of the drop-in and the C ABI surface, without a GPU."""
+import re
+import os
+
+import numpy as np
+import pytest
+
+from spikesorting_fullpursuit_amd import _native
+from spikesorting_fullpursuit_amd.overlap import binary_pursuit_parallel as bpp
+from spikesorting_fullpursuit_amd.overlap import neuron_separability
+from spikesorting_fullpursuit_amd.processing.conversions import time_window_to_samples
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _has_gpu():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+def test_time_window_to_samples_closed_form():
+    # SF/processing/conversions.py:32-37
+    w, tw = time_window_to_samples([0.8e-3, 0.775e-3], 40000)
+    assert w == [-32, 32] and tw[0] == -0.8e-3
+    w, _ = time_window_to_samples([-0.5e-3, 0.0], 30000)
+    assert w == [-15, 2]  # second element at least 1, then +1
+    w, _ = time_window_to_samples([0.0, 1e-3], 30000)
+    assert w == [0, 31]
+
+
+def test_add_n_spikes_counts_per_label():
+    m = {"templates": np.zeros((4, 10))}
+    neuron_separability.add_n_spikes(m, np.array([0, 2, 2, 3, 3, 3], np.int64))
+    assert m["n_spikes"].dtype == np.float64
+    np.testing.assert_array_equal(m["n_spikes"], [1, 0, 2, 3])
+    neuron_separability.add_n_spikes(m, [])
+    np.testing.assert_array_equal(m["n_spikes"], [0, 0, 0, 0])
+
+
+def test_set_bp_threshold_takes_max_of_thr_zero_and_ci():
+    m = {"neuron_lower_thresholds": np.array([-1.0, 2.0, 3.0]), "neuron_lower_CI": np.array([0.5, -np.inf, 5.0])}
+    neuron_separability.set_bp_threshold(m)
+    np.testing.assert_array_equal(m["neuron_lower_thresholds"], [0.5, 2.0, 5.0])
+
+
+def test_template_thresholds_formula():
+    rng = np.random.default_rng(0)
+    U, C, L = 3, 2, 5
+    T = rng.normal(size=(U, C * L))
+    covs = [np.eye(L) * (c + 1) for c in range(C)]
+    SS, thr, ci = neuron_separability.template_thresholds(T, covs, L, 3.719, 2.0)
+    var = np.array([sum((c + 1) * np.sum(T[u, c * L:(c + 1) * L] ** 2) for c in range(C)) for u in range(U)])
+    np.testing.assert_allclose(SS, np.sum(T ** 2, 1))
+    np.testing.assert_allclose(thr, -0.5 * SS + 3.719 * np.sqrt(var))
+    np.testing.assert_allclose(ci, 0.5 * SS - 2.0 * np.sqrt(var))
+
+
+def test_chunk_plan_single_chunk_on_large_memory():
+    n, onsets = bpp.chunk_plan(64, 18_000_000, 30000.0, 120, 48, 47, 288 * 2**30)
+    assert onsets == [0] and n >= 18_000_000
+
+
+def test_chunk_plan_formula_and_overlap():
+    # binary_pursuit_parallel.py:398-507, by hand for a small memory cap
+    C, fs, U, L, nmax = 4, 40000.0, 5, 64, 63
+    cap = 64 * 2**20
+    n, onsets = bpp.chunk_plan(C, 1_200_000, fs, U, L, nmax, 288 * 2**30, max_gpu_memory=cap)
+    items = (2 * nmax + 1) ** 2 * U
+    nloc = np.int64((items - 1) / 256 + 1)
+    per_s = C * fs * 4 + U * fs * 4 + (fs / L) * (20 + (nloc / 3) * 8)
+    expect = int(np.floor((cap - (4 * U * C * L + 4 * U)) / per_s * fs))
+    assert n == expect
+    assert onsets[0] == 0 and all(b - a == n - 3 * L for a, b in zip(onsets, onsets[1:]))
+    assert onsets[-1] < 1_200_000 - L <= onsets[-1] + n - 3 * L
+
+
+def test_chunk_plan_too_small_raises():
+    with pytest.raises(ValueError):
+        bpp.chunk_plan(4, 100000, 40000.0, 5, 64, 63, 288 * 2**30, max_gpu_memory=4096)
+
+
+def test_chunk_plan_short_voltage_has_no_chunks():
+    _, onsets = bpp.chunk_plan(4, 64, 40000.0, 5, 64, 63, 2**34)
+    assert onsets == []
+
+
+def test_library_exports_every_declared_symbol():
+    with open(os.path.join(ROOT, "include", "fbp.h")) as fh:
+        header = fh.read()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(fbp_\w+)\s*\(", header, re.M))
+    assert declared == set(_native.EXPORTS)
+    lib = _native.load()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the no-GPU error path")
+def test_no_gpu_raises_like_reference():
+    # binary_pursuit_parallel.py:292-293
+    with pytest.raises(RuntimeError, match="No GPU available"):
+        _native.Handle(0)
+    V = np.zeros((2, 1000), np.float32)
+    m = {"templates": np.ones((1, 16)), "template_SS": np.array([16.0]),
+         "template_SS_by_chan": np.ones((1, 2)), "neuron_lower_thresholds": np.array([1.0])}
+    si = {"sampling_rate": 40000.0, "clip_width": [-4 / 40000, 3 / 40000], "n_samples_per_chan": 8,
+          "do_overlap_recheck": True, "get_adjusted_clips": False}
+    with pytest.raises(RuntimeError, match="No GPU available"):
+        bpp.binary_pursuit(V, np.float32, si, m)
+
+
+def test_missing_sort_info_key_raises_keyerror():
+    V = np.zeros((2, 1000), np.float32)
+    m = {"templates": np.ones((1, 16)), "template_SS": np.array([16.0]),
+         "template_SS_by_chan": np.ones((1, 2)), "neuron_lower_thresholds": np.array([1.0])}
+    si = {"sampling_rate": 40000.0, "clip_width": [-4 / 40000, 3 / 40000], "n_samples_per_chan": 8,
+          "get_adjusted_clips": False}
+    with pytest.raises(KeyError):
+        bpp.binary_pursuit(V, np.float32, si, m)

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -37,7 +38,17 @@ void launch_residual(const TemplateSet& ts, float* V, int64_t N, const uint32_t*
                      int reach, hipStream_t s);
 void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* sidx, const uint32_t* slab,
                            uint64_t nsp, float* clips, hipStream_t s);
+void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s);
+size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
+size_t screen_select_lds(const TemplateSet& ts, int S);
+void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
+                   float* Bbuf, hipStream_t s);
 }  // namespace fbp
+
+// Limits of the screened recheck (beyond them the literal recheck runs).
+static const size_t kMaxGramBytes = (size_t)8 << 30;
+static const size_t kMaxBbufBytes = (size_t)2 << 30;
+static const size_t kMaxLds = 64 * 1024;
 
 using namespace fbp;
 
@@ -101,6 +112,14 @@ struct fbp_handle {
     bool have_templates = false;
     DevBuf T, rows, nch, off, chan, nz, bias, thr;
     std::vector<int> h_nch;
+    std::vector<uint8_t> h_nz;
+    std::vector<float> h_rowL1, h_SS;
+    int Cmax = 0;
+    // screened-recheck tables (valid for tab_n)
+    int64_t tab_n = -1;
+    int64_t npairs = 0;
+    bool screen_ok = false;
+    DevBuf pid, pair_l, pair_u, G, minG, Ntr, rowL1, SS, Bbuf;
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, LIK, best_idx, best_lab, best_lik, obest_idx, obest_lab;
@@ -215,6 +234,78 @@ struct Timer {
 };
 enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER };
 
+static ScreenTables tables_view(const fbp_handle* h)
+{
+    ScreenTables tb;
+    tb.pid = h->pid.as<int>();
+    tb.pair_l = h->pair_l.as<int>();
+    tb.pair_u = h->pair_u.as<int>();
+    tb.G = h->G.as<float>();
+    tb.minG = h->minG.as<float>();
+    tb.Ntr = h->Ntr.as<float>();
+    tb.rowL1 = h->rowL1.as<float>();
+    tb.SS = h->SS.as<float>();
+    tb.n = (int)h->tab_n;
+    tb.S = (int)(2 * h->tab_n + 1);
+    // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u)) bounds |E-hat - E_fp32|:
+    // both are sums of at most K rounded terms (gamma_K each), doubled for safety.
+    const double K = 2.0 * h->Cmax * (h->L + 1) + 2.0 * h->L + 16.0;
+    const double u = std::ldexp(1.0, -24);
+    tb.kappa = (float)(4.0 * K * u / (1.0 - K * u));
+    return tb;
+}
+
+// Builds the voltage-independent tables of the screened recheck for shift
+// range n.  Returns false (literal recheck) when they do not fit the limits.
+static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
+{
+    *ok = false;
+    h->tab_n = n;
+    h->screen_ok = false;
+    const int U = h->U, C = h->C, L = h->L;
+    const int64_t S = 2 * n + 1;
+    TemplateSet ts = template_view(h);
+    int rp;
+    if (screen_corr_lds(ts, (int)S, &rp) > kMaxLds || screen_select_lds(ts, (int)S) > kMaxLds) return FBP_OK;
+    if ((size_t)U * S * 4 > kMaxBbufBytes) return FBP_OK;
+    std::vector<int> pid((size_t)U * U, -1), pl, pu;
+    for (int l = 0; l < U; l++)
+        for (int u = 0; u < U; u++) {
+            bool sh = false;
+            for (int c = 0; c < C && !sh; c++) sh = h->h_nz[(size_t)l * C + c] && h->h_nz[(size_t)u * C + c];
+            if (sh) {
+                pid[(size_t)l * U + u] = (int)pl.size();
+                pl.push_back(l);
+                pu.push_back(u);
+            }
+        }
+    const int64_t np = (int64_t)pl.size();
+    if ((size_t)np * S * S * 4 > kMaxGramBytes) return FBP_OK;
+    if (pl.empty()) { pl.push_back(0); pu.push_back(0); }
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    HIP_TRY(h->pid.ensure(pid.size() * 4), "alloc pid");
+    HIP_TRY(h->pair_l.ensure(pl.size() * 4), "alloc pairs");
+    HIP_TRY(h->pair_u.ensure(pu.size() * 4), "alloc pairs");
+    HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * S * 4), "alloc gram table");
+    HIP_TRY(h->minG.ensure((size_t)std::max<int64_t>(np, 1) * S * 4), "alloc gram minima");
+    HIP_TRY(h->Ntr.ensure((size_t)U * S * 4), "alloc energies");
+    HIP_TRY(h->rowL1.ensure(std::max<size_t>(h->h_rowL1.size(), 1) * 4), "alloc norms");
+    HIP_TRY(h->SS.ensure((size_t)U * 4), "alloc norms");
+    HIP_TRY(hipMemcpy(h->pid.p, pid.data(), pid.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->pair_l.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->pair_u.p, pu.data(), pu.size() * 4, hipMemcpyHostToDevice), "copy");
+    if (!h->h_rowL1.empty())
+        HIP_TRY(hipMemcpy(h->rowL1.p, h->h_rowL1.data(), h->h_rowL1.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->SS.p, h->h_SS.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    h->npairs = np;
+    launch_screen_tables(ts, tables_view(h), np, h->stream);
+    HIP_TRY(hipGetLastError(), "screen tables");
+    HIP_TRY(hipStreamSynchronize(h->stream), "screen tables");
+    h->screen_ok = true;
+    *ok = true;
+    return FBP_OK;
+}
+
 extern "C" {
 
 int fbp_create(int device, fbp_handle** out)
@@ -245,7 +336,8 @@ int fbp_destroy(fbp_handle* h)
     DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK,
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
-                      &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips};
+                      &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
+                      &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf};
     for (DevBuf* b : bufs) b->release();
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -318,6 +410,26 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
     h->L = L;
     h->Lp = Lp;
     h->h_nch = nch;
+    h->h_nz = nz;
+    h->Cmax = 0;
+    for (int u = 0; u < U; u++) h->Cmax = std::max(h->Cmax, nch[u]);
+    h->h_rowL1.assign(chan.size(), 0.0f);
+    h->h_SS.assign(U, 0.0f);
+    for (int u = 0; u < U; u++) {
+        double ss = 0.0;
+        for (int k = 0; k < nch[u]; k++) {
+            const int r = off[u] + k;
+            double l1 = 0.0;
+            for (int i = 0; i < L; i++) {
+                const double t = rows[(size_t)r * Lp + i];
+                l1 += std::fabs(t);
+                ss += t * t;
+            }
+            h->h_rowL1[r] = (float)(l1 * (1.0 + 1e-6));
+        }
+        h->h_SS[u] = (float)(ss * (1.0 + 1e-6));
+    }
+    h->tab_n = -1;
     h->have_templates = true;
     h->have_result = false;
     return FBP_OK;
@@ -398,6 +510,17 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     for (int u = 0; u < h->U; u++) nz_sum += h->h_nch[u];
     const int reach = (int)(3 + (2 * (int64_t)n_max_shift + L) / L);
     const uint32_t nchunk = recheck ? recheck_literal_chunks(ts, n_max_shift) : 0;
+    bool use_screen = false;
+    if (recheck && !(flags & FBP_FLAG_RECHECK_LITERAL)) {
+        if (h->tab_n != (int64_t)n_max_shift) {
+            int rc = build_screen_tables(h, n_max_shift, &use_screen);
+            if (rc != FBP_OK) return rc;
+        }
+        use_screen = h->screen_ok;
+    }
+    const uint32_t screen_batch =
+        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
+    if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -422,10 +545,16 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             nF = h->h_counters[CNT_F];
             stats.windows_flagged += nF;
             if (nF > 0) {
-                HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
-                HIP_TRY(h->part_k.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
                 tm.begin(TM_RECHECK);
-                launch_recheck_literal(ts, st, n_max_shift, nF, h->part_v.as<float>(), h->part_k.as<uint32_t>(), s);
+                if (use_screen) {
+                    const ScreenTables tb = tables_view(h);
+                    for (uint32_t f0 = 0; f0 < nF; f0 += screen_batch)
+                        launch_screen(ts, st, tb, f0, std::min(screen_batch, nF - f0), h->Bbuf.as<float>(), s);
+                } else {
+                    HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
+                    HIP_TRY(h->part_k.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
+                    launch_recheck_literal(ts, st, n_max_shift, nF, h->part_v.as<float>(), h->part_k.as<uint32_t>(), s);
+                }
                 tm.end(TM_RECHECK);
                 stats.n_recheck_launches++;
                 tm.begin(TM_OTHER);
@@ -475,6 +604,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.ms_pursuit = tm.total(TM_PURSUIT);
     stats.ms_residual = tm.total(TM_RESID);
     stats.ms_other = tm.total(TM_OTHER);
+    HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
+    stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.passes = passes;
     stats.spikes = total;
     h->stats = stats;

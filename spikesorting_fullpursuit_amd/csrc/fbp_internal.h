@@ -46,6 +46,21 @@ struct PassState {
     int64_t nwin;
 };
 
+// Voltage-independent tables of the screened overlap recheck.
+struct ScreenTables {
+    const int* pid;       // [U*U] pair index of (l, u) sharing a channel, or -1
+    const int* pair_l;    // [npairs]
+    const int* pair_u;    // [npairs]
+    float* G;             // [npairs][S][S]  G[p][s][f]: window-truncated cross product
+    float* minG;          // [npairs][S]     min over f
+    float* Ntr;           // [U][S]          energy of the window-truncated template
+    const float* rowL1;   // [R] L1 norm of each compact template row (rounded up)
+    const float* SS;      // [U] sum of squares of each template (rounded up)
+    int S;
+    int n;
+    float kappa;          // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u))
+};
+
 enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_N = 8 };
 
 }  // namespace fbp

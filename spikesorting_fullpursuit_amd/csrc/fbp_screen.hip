@@ -1,0 +1,393 @@
+// Screened overlap recheck: the same decisions as overlap_recheck_indices +
+// parse_overlap_recheck_indices (SF/kernels/binary_pursuit.cl:444-815, SF =
+// spikesorting_fullpursuit/) at a fraction of the work.
+//
+// In exact arithmetic the value of item (u, f, s) of a window whose best spike
+// is (l, b) splits into
+//     E(f, u, s) = A(f) + B(u, s) - G(f, u, s)
+// with B(u, s) = <b_s, v> - |b_s|^2 / 2 for the window-truncated template b_s
+// of unit u shifted by s - n (A(f) = B(l, f)) and G the window-truncated
+// cross product of the two templates, which is voltage independent and zero
+// when the units share no channel.  The screen evaluates E-hat (the FP32
+// decomposed value) with a per-window bound delta >= |E-hat - E_fp32|,
+// keeps every item whose E-hat could still be the maximum, and rescores those
+// candidates with the exact FP32 operation order of the literal kernel.  The
+// winner (value, then lowest item id) is therefore the literal kernel's.
+#include <math.h>
+
+#include "fbp_device.h"
+
+namespace fbp {
+
+// Template value of unit u on channel c at window sample j when the template
+// onset sits `sh` samples after the window start (zero outside the template).
+__device__ __forceinline__ float shifted(const TemplateSet& ts, int u, int c, int j, int sh)
+{
+    const int i = j - sh;
+    return (i >= 0 && i < ts.L) ? ts.T[((size_t)u * ts.C + c) * ts.L + i] : 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tab_gram(TemplateSet ts, ScreenTables tb, int64_t npairs)
+{
+    const int S = tb.S, n = tb.n, L = ts.L, C = ts.C;
+    const int64_t total = npairs * S * S;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = e / ((int64_t)S * S);
+        const int rem = (int)(e - p * S * S);
+        const int s = rem / S, f = rem - s * S;
+        const int l = tb.pair_l[p], u = tb.pair_u[p];
+        const int fsh = f - n, ssh = s - n;
+        const int jlo = max(0, max(fsh, ssh)), jhi = min(L, min(L + fsh, L + ssh));
+        float g = 0.0f;
+        for (int c = 0; c < C; c++) {
+            if (!ts.nz[(size_t)l * C + c] || !ts.nz[(size_t)u * C + c]) continue;
+            for (int j = jlo; j < jhi; j++) g = fmaf(shifted(ts, l, c, j, fsh), shifted(ts, u, c, j, ssh), g);
+        }
+        tb.G[e] = g;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tab_min_ntr(TemplateSet ts, ScreenTables tb, int64_t npairs)
+{
+    const int S = tb.S, n = tb.n, L = ts.L, C = ts.C;
+    const int64_t total_min = npairs * S;
+    const int64_t total = total_min + (int64_t)ts.U * S;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        if (e < total_min) {
+            const float* row = tb.G + e * S;
+            float m = row[0];
+            for (int f = 1; f < S; f++) m = fminf(m, row[f]);
+            tb.minG[e] = m;
+        } else {
+            const int64_t r = e - total_min;
+            const int u = (int)(r / S), s = (int)(r - (int64_t)u * S);
+            const int ssh = s - n;
+            const int jlo = max(0, ssh), jhi = min(L, L + ssh);
+            float a = 0.0f;
+            for (int c = 0; c < C; c++) {
+                if (!ts.nz[(size_t)u * C + c]) continue;
+                for (int j = jlo; j < jhi; j++) {
+                    const float t = shifted(ts, u, c, j, ssh);
+                    a = fmaf(t, t, a);
+                }
+            }
+            tb.Ntr[r] = a;
+        }
+    }
+}
+
+// Taps [ilo, ihi) of a sliding correlation (4-tap chunks outside are skipped;
+// they would only add products with the zero padding).
+template <int LT>
+__device__ __forceinline__ void corr_row_range(const float* __restrict__ trow, const float* vrow, float* acc, int ilo,
+                                               int ihi)
+{
+    float r[LT + kPos - 1];
+#pragma unroll
+    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
+#pragma unroll
+    for (int i = 0; i < LT; i += 4) {
+        if (i + 3 < ilo || i >= ihi) continue;
+        const float4 tv = *reinterpret_cast<const float4*>(trow + i);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.x, r[i + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.y, r[i + 1 + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.z, r[i + 2 + q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.w, r[i + 3 + q], acc[q]);
+    }
+}
+
+__device__ __forceinline__ bool window_skipped(const TemplateSet& ts, const PassState& st, uint32_t id)
+{
+    const uint32_t lab = st.best_lab[id];
+    const int64_t b = st.best_idx[id];
+    const float lb = st.LIK[(size_t)lab * st.N + b];
+    return (lb < ts.thr[lab]) || (b >= st.N - ts.L);  // binary_pursuit.cl:518-527
+}
+
+// B(u, s) for every unit and shift of the flagged windows F[f0 .. f0+nbatch).
+// One block per window; the window's voltage (zero outside [b, b+L)) is staged
+// in LDS as rows of rp samples, sample p <-> v[p - n].
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
+                                                        uint32_t nbatch, int rp, float* __restrict__ Bbuf)
+{
+    extern __shared__ float sP[];
+    const int C = ts.C, U = ts.U;
+    const int L = (LT > 0) ? LT : ts.L;
+    const int S = tb.S, n = tb.n;
+    const int ngr = (S + kPos - 1) / kPos;
+    const int64_t N = st.N;
+    const uint32_t nF = st.counters[CNT_F];
+    for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
+        const uint32_t w = f0 + wi;
+        if (w >= nF) break;
+        const uint32_t id = st.F[w];
+        const bool skip = window_skipped(ts, st, id);
+        const int64_t b = st.best_idx[id];
+        __syncthreads();
+        if (!skip)
+            for (int e = threadIdx.x; e < C * rp; e += blockDim.x) {
+                const int c = e / rp, p = e - c * rp;
+                const int j = p - n;
+                sP[e] = (j >= 0 && j < L) ? st.V[(size_t)c * N + b + j] : 0.0f;
+            }
+        __syncthreads();
+        if (skip) continue;
+        float* Bw = Bbuf + (size_t)wi * U * S;
+        for (int it = threadIdx.x; it < U * ngr; it += blockDim.x) {
+            const int g = it / U, u = it - g * U;
+            const int s0 = g * kPos;
+            float acc[kPos];
+#pragma unroll
+            for (int q = 0; q < kPos; q++) acc[q] = 0.0f;
+            const int ilo = n - s0 - (kPos - 1), ihi = n + L - s0;
+            const int r0 = ts.off[u], nr = ts.nch[u];
+            for (int k = 0; k < nr; k++) {
+                const int c = ts.chan[r0 + k];
+                const float* trow = ts.rows + (size_t)(r0 + k) * ts.Lp;
+                const float* vrow = sP + c * rp + s0;
+                if (LT > 0) corr_row_range<(LT > 0 ? LT : 4)>(trow, vrow, acc, ilo, ihi);
+                else lik_row_generic(trow, vrow, L, acc);
+            }
+#pragma unroll
+            for (int q = 0; q < kPos; q++) {
+                const int s = s0 + q;
+                if (s < S) Bw[(size_t)u * S + s] = fmaf(-0.5f, tb.Ntr[(size_t)u * S + s], acc[q]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ float block_max(float v, float* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = red[0];
+#pragma unroll
+    for (int i = 1; i < kBlock / 64; i++) m = fmaxf(m, red[i]);
+    return m;
+}
+
+// Selection, exact rescoring and the parse cascade for one window per block.
+__global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
+                                                          uint32_t nbatch, const float* __restrict__ Bbuf)
+{
+    extern __shared__ float sm[];
+    __shared__ float red[kBlock / 64];
+    __shared__ uint32_t redk[kBlock / 64];
+    __shared__ uint32_t ncand_s;
+    const int C = ts.C, U = ts.U, L = ts.L;
+    const int S = tb.S, n = tb.n;
+    const int64_t N = st.N;
+    const int nwords = (U * S + 31) / 32;
+    float* vW = sm;                       // [C][L]
+    float* vmax = vW + C * L;             // [C]
+    float* A = vmax + C;                  // [S]
+    float* RAall = A + S;                 // [S]
+    float* RApos = RAall + S;             // [S]
+    float* Q = RApos + S;                 // [U]
+    uint32_t* posu = reinterpret_cast<uint32_t*>(Q + U);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
+    uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
+    uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
+    const uint32_t nF = st.counters[CNT_F];
+    for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
+        const uint32_t w = f0 + wi;
+        if (w >= nF) break;
+        const uint32_t id = st.F[w];
+        const uint32_t lab = st.best_lab[id];
+        const int64_t b = st.best_idx[id];
+        const float bl = st.best_lik[id];
+        __syncthreads();
+        if (window_skipped(ts, st, id)) {
+            if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
+            continue;
+        }
+        const float* Bw = Bbuf + (size_t)wi * U * S;
+        for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+            const int c = e / L, j = e - c * L;
+            vW[e] = st.V[(size_t)c * N + b + j];
+        }
+        for (int f = threadIdx.x; f < S; f += blockDim.x) {
+            const int64_t af = b + f - n;
+            const bool in = af >= 0 && af < N - L;
+            fin[f] = in;
+            posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
+            A[f] = Bw[(size_t)lab * S + f];
+        }
+        for (int wd = threadIdx.x; wd < nwords; wd += blockDim.x) {
+            uint32_t bits = 0;
+            for (int k = 0; k < 32; k++) {
+                const int e = wd * 32 + k;
+                if (e >= U * S) break;
+                const int u = e / S, s = e - u * S;
+                const int64_t as = b + s - n;
+                if (as >= 0 && as < N - L && st.LIK[(size_t)u * N + as] > 0.0f) bits |= 1u << k;
+            }
+            posu[wd] = bits;
+        }
+        if (threadIdx.x == 0) ncand_s = 0;
+        __syncthreads();
+        for (int c = threadIdx.x; c < C; c += blockDim.x) {
+            float m = 0.0f;
+            for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * L + j]));
+            vmax[c] = m;
+        }
+        // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
+        for (int s = threadIdx.x; s < S; s += blockDim.x) {
+            float ma = -INFINITY, mp = -INFINITY;
+            const int fl = max(0, s - L), fh = min(S - 1, s + L);
+            for (int f = fl; f <= fh; f++) {
+                if (fin[f]) ma = fmaxf(ma, A[f]);
+                if (posl[f]) mp = fmaxf(mp, A[f]);
+            }
+            RAall[s] = ma;
+            RApos[s] = mp;
+        }
+        __syncthreads();
+        float dloc = 0.0f;
+        for (int u = threadIdx.x; u < U; u += blockDim.x) {
+            float q = 0.0f;
+            const int r0 = ts.off[u], nr = ts.nch[u];
+            for (int k = 0; k < nr; k++) q += tb.rowL1[r0 + k] * vmax[ts.chan[r0 + k]];
+            Q[u] = q;
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += blockDim.x)
+            dloc = fmaxf(dloc, tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])));
+        float mA = -INFINITY;
+        for (int f = threadIdx.x; f < S; f += blockDim.x)
+            if (fin[f]) mA = fmaxf(mA, A[f]);
+        const float delta = block_max(dloc, red);
+        const float maxA = block_max(mA, red);
+        // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable
+        const int* pidl = tb.pid + (size_t)lab * U;
+        float mns = -INFINITY;
+        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+            const int u = e / S, s = e - u * S;
+            if (pidl[u] >= 0) continue;
+            const int64_t as = b + s - n;
+            if (as < 0 || as >= N - L) continue;
+            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+            const float ra = pu ? RAall[s] : RApos[s];
+            if (ra == -INFINITY) continue;
+            mns = fmaxf(mns, ra + Bw[e]);
+        }
+        const float Mns = block_max(mns, red);
+        // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G
+        const float tp = fmaxf(Mns - 2.0f * delta, bl - delta);
+        float msh = -INFINITY;
+        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+            const int u = e / S, s = e - u * S;
+            const int p = pidl[u];
+            if (p < 0) continue;
+            const int64_t as = b + s - n;
+            if (as < 0 || as >= N - L) continue;
+            const float Bv = Bw[e];
+            if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < tp) continue;
+            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+            const float* Gr = tb.G + ((size_t)p * S + s) * S;
+            for (int f = 0; f < S; f++) {
+                if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                msh = fmaxf(msh, (A[f] + Bv) - Gr[f]);
+            }
+        }
+        const float M = fmaxf(Mns, block_max(msh, red));
+        const float theta = fmaxf(M - 2.0f * delta, bl - delta);
+        // phase 3: every item whose E-hat reaches theta is rescored exactly
+        float bv = 0.0f;
+        uint32_t bk = kNoItem;
+        uint32_t ncand = 0;
+        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+            const int u = e / S, s = e - u * S;
+            const int64_t as = b + s - n;
+            if (as < 0 || as >= N - L) continue;
+            const float Bv = Bw[e];
+            const int p = pidl[u];
+            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+            if (p < 0) {
+                const float ra = pu ? RAall[s] : RApos[s];
+                if (ra == -INFINITY || ra + Bv < theta) continue;
+            } else if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < theta) {
+                continue;
+            }
+            const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+            for (int f = 0; f < S; f++) {
+                if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
+                if (eh < theta) continue;
+                const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
+                const uint32_t k = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
+                argmax_merge(bv, bk, v, k);
+                ncand++;
+            }
+        }
+        wave_argmax(bv, bk);
+        __syncthreads();
+        if (lane_id() == 0) { red[threadIdx.x >> 6] = bv; redk[threadIdx.x >> 6] = bk; }
+        if (ncand) atomicAdd(&ncand_s, ncand);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = red[0];
+            uint32_t k = redk[0];
+            for (int i = 1; i < kBlock / 64; i++) argmax_merge(v, k, red[i], redk[i]);
+            apply_parse(ts, st, n, id, v, k);
+            atomicAdd(st.counters + CNT_EVAL, ncand_s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned sgrid(int64_t n, int64_t per, unsigned cap)
+{
+    int64_t g = (n + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s)
+{
+    if (npairs > 0)
+        hipLaunchKernelGGL(k_tab_gram, dim3(sgrid(npairs * tb.S * tb.S, kBlock, 1u << 20)), dim3(kBlock), 0, s, ts, tb,
+                           npairs);
+    hipLaunchKernelGGL(k_tab_min_ntr, dim3(sgrid(npairs * tb.S + (int64_t)ts.U * tb.S, kBlock, 1u << 20)),
+                       dim3(kBlock), 0, s, ts, tb, npairs);
+}
+
+size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
+{
+    const int ngr = (S + kPos - 1) / kPos;
+    *rp = ngr * kPos + ts.L + kPos + 1;
+    return (size_t)ts.C * (*rp) * sizeof(float);
+}
+
+size_t screen_select_lds(const TemplateSet& ts, int S)
+{
+    const int nwords = (ts.U * S + 31) / 32;
+    return (size_t)(ts.C * ts.L + ts.C + 3 * S + ts.U) * 4 + (size_t)nwords * 4 + 2 * (size_t)S + 16;
+}
+
+void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
+                   float* Bbuf, hipStream_t s)
+{
+    if (nbatch == 0) return;
+    int rp;
+    const size_t lds1 = screen_corr_lds(ts, tb.S, &rp);
+    const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
+    if (ts.L == 48) hipLaunchKernelGGL(k_screen_corr<48>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    hipLaunchKernelGGL(k_screen_select, g, b, screen_select_lds(ts, tb.S), s, ts, st, tb, f0, nbatch, Bbuf);
+}
+
+}  // namespace fbp

@@ -47,31 +47,34 @@ def assert_same(gpu, ora):
     np.testing.assert_array_equal(gR, oR)
 
 
+@pytest.mark.parametrize("literal", [True, False])
 @pytest.mark.parametrize("seed", range(10))
-def test_noisy_cases_match_oracle(handle, seed):
+def test_noisy_cases_match_oracle(handle, seed, literal):
     V, T, bias, thr, on, lab = cases.noisy_case(seed)
     n = T.shape[2] - 1
-    assert_same(gpu_run(handle, V, T, bias, thr, n), fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=literal),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
+@pytest.mark.parametrize("literal", [True, False])
 @pytest.mark.parametrize("seed", [0, 1, 5, 9])
-def test_correlated_pairs_match_oracle(handle, seed):
+def test_correlated_pairs_match_oracle(handle, seed, literal):
     U, C, L = 4, 3, 16
     T = cases.dog_templates(U, C, L, seed).astype(np.float32)
     V, truth = cases.planted_pairs(T, 2000, 100 + seed, min_d=3)
     V = V + np.random.default_rng(seed).normal(size=V.shape).astype(np.float32) * 0.2
     bias, thr = cases.bias_thr(T)
-    assert_same(gpu_run(handle, V, T, bias, thr, L - 1),
+    assert_same(gpu_run(handle, V, T, bias, thr, L - 1, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
 
 
-@pytest.mark.parametrize("recheck", [True, False])
-def test_small_workload_3s_matches_oracle(handle, recheck):
+@pytest.mark.parametrize("recheck,literal", [(True, True), (True, False), (False, False)])
+def test_small_workload_3s_matches_oracle(handle, recheck, literal):
     d = synthetic.generate("small", duration_s=3.0)
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(gpu_run(handle, V, T, bias, thr, n, recheck=recheck),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, recheck=recheck, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, n, do_recheck=recheck, return_residual=True))
 
 
@@ -83,14 +86,16 @@ def test_adjusted_clips_match_oracle(handle):
     np.testing.assert_array_equal(g[3], fbp_oracle.adjusted_clips(oR, T, oi, ol))
 
 
+@pytest.mark.parametrize("literal", [True, False])
 @pytest.mark.parametrize("n_max", [0, 1, 3, 20])
-def test_shift_range_variants(handle, n_max):
+def test_shift_range_variants(handle, n_max, literal):
     V, T, bias, thr, on, lab = cases.noisy_case(20 + n_max, U=3, L=8)
-    assert_same(gpu_run(handle, V, T, bias, thr, n_max),
+    assert_same(gpu_run(handle, V, T, bias, thr, n_max, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, n_max, return_residual=True))
 
 
-def test_zero_channel_templates_and_odd_length(handle):
+@pytest.mark.parametrize("literal", [True, False])
+def test_zero_channel_templates_and_odd_length(handle, literal):
     rng = np.random.default_rng(8)
     U, C, L, N = 4, 5, 11, 900
     T = (rng.normal(size=(U, C, L)) * 4).astype(np.float32)
@@ -101,7 +106,7 @@ def test_zero_channel_templates_and_odd_length(handle):
     for p in range(20, N - 2 * L, 37):
         V[:, p:p + L] += T[(p // 37) % U]
     bias, thr = cases.bias_thr(T.astype(np.float64), 0.05)
-    assert_same(gpu_run(handle, V, T, bias, thr, L - 1),
+    assert_same(gpu_run(handle, V, T, bias, thr, L - 1, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
 
 
@@ -126,13 +131,36 @@ def test_empty_result(handle):
     assert len(g[0]) == 0
 
 
-def test_probe16_slice_matches_oracle(handle):
+@pytest.mark.parametrize("literal", [True, False])
+def test_probe16_slice_matches_oracle(handle, literal):
     d = synthetic.generate("probe16", duration_s=0.5)
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(gpu_run(handle, V, T, bias, thr, n),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+def test_probe64_slice_screened_matches_oracle(handle):
+    d = synthetic.generate("probe64", duration_s=0.1)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = d["voltage"]
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=False),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+@pytest.mark.parametrize("literal", [True, False])
+def test_exact_ties_identical_templates(handle, literal):
+    # two identical units: every overlap value ties between the labels, the
+    # winner must be the lowest item id in both paths
+    T1 = cases.random_templates(2, 3, 12, 21)
+    T = np.concatenate([T1, T1[:1]]).astype(np.float32)
+    V, truth = cases.planted_pairs(T, 1800, 21)
+    V = V + np.random.default_rng(21).normal(size=V.shape).astype(np.float32) * 0.05
+    bias, thr = cases.bias_thr(T.astype(np.float64))
+    assert_same(gpu_run(handle, V, T, bias, thr, 11, literal=literal),
+                fbp_oracle.pursuit(V, T, bias, thr, 11, return_residual=True))
 
 
 def _ref_output(d, chunks=None):
@@ -146,7 +174,7 @@ def test_dropin_binary_pursuit_matches_oracle():
     d = synthetic.generate("small", duration_s=5.0)
     m = dict(d["separability_metrics"])
     ev, lab, bpbool, clips = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m,
-                                                n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+                                                n_max_shift_inds=d["n_max_shift_inds"])
     rev, rlab = _ref_output(d)
     assert ev.dtype == np.int64 and lab.dtype == np.int64 and bpbool.dtype == bool
     np.testing.assert_array_equal(ev, rev)
@@ -160,7 +188,7 @@ def test_dropin_chunked_matches_oracle_per_chunk():
     d = synthetic.generate("small", duration_s=2.0)
     si = dict(d["sort_info"])
     m = dict(d["separability_metrics"])
-    cap = 4 * 2**20
+    cap = 1 * 2**20
     ev, lab, _, _ = bpp.binary_pursuit(d["voltage"], np.float32, si, m, n_max_shift_inds=d["n_max_shift_inds"],
                                        max_gpu_memory=cap, recheck_literal=True)
     T, bias, thr = synthetic.kernel_inputs(d)

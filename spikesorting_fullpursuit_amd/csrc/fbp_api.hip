@@ -40,7 +40,7 @@ void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, con
                            uint64_t nsp, float* clips, hipStream_t s);
 void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s);
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
-size_t screen_select_lds(const TemplateSet& ts, int S);
+size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
                    float* Bbuf, hipStream_t s);
 }  // namespace fbp
@@ -247,6 +247,7 @@ static ScreenTables tables_view(const fbp_handle* h)
     tb.SS = h->SS.as<float>();
     tb.n = (int)h->tab_n;
     tb.S = (int)(2 * h->tab_n + 1);
+    tb.cmax = std::max(h->Cmax, 1);
     // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u)) bounds |E-hat - E_fp32|:
     // both are sums of at most K rounded terms (gamma_K each), doubled for safety.
     const double K = 2.0 * h->Cmax * (h->L + 1) + 2.0 * h->L + 16.0;
@@ -266,7 +267,8 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     const int64_t S = 2 * n + 1;
     TemplateSet ts = template_view(h);
     int rp;
-    if (screen_corr_lds(ts, (int)S, &rp) > kMaxLds || screen_select_lds(ts, (int)S) > kMaxLds) return FBP_OK;
+    if (screen_corr_lds(ts, (int)S, &rp) > kMaxLds || screen_select_lds(ts, (int)S, std::max(h->Cmax, 1)) > kMaxLds - 8192)
+        return FBP_OK;
     if ((size_t)U * S * 4 > kMaxBbufBytes) return FBP_OK;
     std::vector<int> pid((size_t)U * U, -1), pl, pu;
     for (int l = 0; l < U; l++)

@@ -58,6 +58,7 @@ struct ScreenTables {
     const float* SS;      // [U] sum of squares of each template (rounded up)
     int S;
     int n;
+    int cmax;             // most non-zero channels of any template
     float kappa;          // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u))
 };
 

@@ -138,8 +138,12 @@ __global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassStat
         __syncthreads();
         if (skip) continue;
         float* Bw = Bbuf + (size_t)wi * U * S;
-        for (int it = threadIdx.x; it < U * ngr; it += blockDim.x) {
-            const int g = it / U, u = it - g * U;
+        // item = (shift group g, unit u) with units padded to a multiple of 64,
+        // so every wave works on one shift group and skips the same taps
+        const int Up = (U + 63) & ~63;
+        for (int it = threadIdx.x; it < Up * ngr; it += blockDim.x) {
+            const int g = it / Up, u = it - g * Up;
+            if (u >= U) continue;
             const int s0 = g * kPos;
             float acc[kPos];
 #pragma unroll
@@ -175,6 +179,53 @@ __device__ __forceinline__ float block_max(float v, float* red)
     return m;
 }
 
+// Exact FP32 value of item (u, f, s) with both templates' non-zero channel rows
+// in LDS (rows of L floats, channels ascending).  Same operation order as
+// recheck_item_value: channels of either template ascending, then samples.
+__device__ inline float exact_item_rows(const float* vW, int L, const float* Tl, const int* cl, int nl, const float* Tu,
+                                        const int* cu, int nu, int f, int s, int n)
+{
+    const int fa = n - f, sa = n - s;
+    const int ffj = fa < 0 ? -fa : 0, sfj = sa < 0 ? -sa : 0;
+    float lik = 0.0f;
+    int il = 0, iu = 0;
+    while (il < nl || iu < nu) {
+        const int c1 = il < nl ? cl[il] : 0x7fffffff;
+        const int c2 = iu < nu ? cu[iu] : 0x7fffffff;
+        const int c = min(c1, c2);
+        const float* tf = (c1 == c) ? Tl + (il++) * L : nullptr;
+        const float* tsr = (c2 == c) ? Tu + (iu++) * L : nullptr;
+        const float* vr = vW + c * L;
+        float ss = 0.0f;
+        for (int j = 0; j < L; j++) {
+            const float fv = (tf && j >= ffj && fa + j < L) ? tf[fa + j] : 0.0f;
+            const float sv = (tsr && j >= sfj && sa + j < L) ? tsr[sa + j] : 0.0f;
+            const float sum = fv + sv;
+            ss = fmaf(sum, sum, ss);
+            lik = fmaf(sum, vr[j], lik);
+        }
+        if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
+    }
+    return lik;
+}
+
+// Copies the non-zero rows of template u (L floats each) and their channels
+// into LDS with the lanes of one wave; returns the row count.
+__device__ __forceinline__ int stage_rows_wave(const TemplateSet& ts, int u, float* dst, int* dch)
+{
+    const int r0 = ts.off[u], nr = ts.nch[u], L = ts.L;
+    for (int e = lane_id(); e < nr * L; e += 64) {
+        const int k = e / L, j = e - k * L;
+        dst[e] = ts.rows[(size_t)(r0 + k) * ts.Lp + j];
+    }
+    for (int k = lane_id(); k < nr; k += 64) dch[k] = ts.chan[r0 + k];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    return nr;
+}
+
+constexpr int kCandCap = 512;
+
 // Selection, exact rescoring and the parse cascade for one window per block.
 __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
                                                           uint32_t nbatch, const float* __restrict__ Bbuf)
@@ -182,9 +233,13 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     extern __shared__ float sm[];
     __shared__ float red[kBlock / 64];
     __shared__ uint32_t redk[kBlock / 64];
+    __shared__ float cand_v[kCandCap];
+    __shared__ uint32_t cand_k[kCandCap];
     __shared__ uint32_t ncand_s;
+    __shared__ float best1_v;
+    __shared__ uint32_t best1_k;
     const int C = ts.C, U = ts.U, L = ts.L;
-    const int S = tb.S, n = tb.n;
+    const int S = tb.S, n = tb.n, CM = tb.cmax;
     const int64_t N = st.N;
     const int nwords = (U * S + 31) / 32;
     float* vW = sm;                       // [C][L]
@@ -193,9 +248,15 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     float* RAall = A + S;                 // [S]
     float* RApos = RAall + S;             // [S]
     float* Q = RApos + S;                 // [U]
-    uint32_t* posu = reinterpret_cast<uint32_t*>(Q + U);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
+    float* D = Q + U;                     // [U] per-unit bound delta
+    float* Tl = D + U;                    // [CM][L] rows of the window's label
+    float* Tw = Tl + CM * L;              // [4][CM][L] rows staged by each wave
+    int* cl = reinterpret_cast<int*>(Tw + 4 * CM * L);  // [CM]
+    int* cw = cl + CM;                    // [4][CM]
+    uint32_t* posu = reinterpret_cast<uint32_t*>(cw + 4 * CM);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
     uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
+    const int wv = threadIdx.x >> 6;
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
         const uint32_t w = f0 + wi;
@@ -214,6 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const int c = e / L, j = e - c * L;
             vW[e] = st.V[(size_t)c * N + b + j];
         }
+        const int nl = ts.nch[lab];
+        for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
+            const int k = e / L, j = e - k * L;
+            Tl[e] = ts.rows[(size_t)(ts.off[lab] + k) * ts.Lp + j];
+        }
+        for (int k = threadIdx.x; k < nl; k += blockDim.x) cl[k] = ts.chan[ts.off[lab] + k];
         for (int f = threadIdx.x; f < S; f += blockDim.x) {
             const int64_t af = b + f - n;
             const bool in = af >= 0 && af < N - L;
@@ -251,7 +318,6 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             RApos[s] = mp;
         }
         __syncthreads();
-        float dloc = 0.0f;
         for (int u = threadIdx.x; u < U; u += blockDim.x) {
             float q = 0.0f;
             const int r0 = ts.off[u], nr = ts.nch[u];
@@ -260,13 +326,13 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         }
         __syncthreads();
         for (int u = threadIdx.x; u < U; u += blockDim.x)
-            dloc = fmaxf(dloc, tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])));
+            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u]));
         float mA = -INFINITY;
         for (int f = threadIdx.x; f < S; f += blockDim.x)
             if (fin[f]) mA = fmaxf(mA, A[f]);
-        const float delta = block_max(dloc, red);
         const float maxA = block_max(mA, red);
-        // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable
+        // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable.
+        // Tracks M' = max(E-hat - delta_u), a lower bound of the exact maximum.
         const int* pidl = tb.pid + (size_t)lab * U;
         float mns = -INFINITY;
         for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
@@ -277,11 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
             const float ra = pu ? RAall[s] : RApos[s];
             if (ra == -INFINITY) continue;
-            mns = fmaxf(mns, ra + Bw[e]);
+            mns = fmaxf(mns, (ra + Bw[e]) - D[u]);
         }
         const float Mns = block_max(mns, red);
         // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G
-        const float tp = fmaxf(Mns - 2.0f * delta, bl - delta);
         float msh = -INFINITY;
         for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
             const int u = e / S, s = e - u * S;
@@ -290,20 +355,18 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const int64_t as = b + s - n;
             if (as < 0 || as >= N - L) continue;
             const float Bv = Bw[e];
+            const float tp = fmaxf(Mns - D[u], bl - D[u]);
             if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < tp) continue;
             const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
             const float* Gr = tb.G + ((size_t)p * S + s) * S;
             for (int f = 0; f < S; f++) {
                 if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                msh = fmaxf(msh, (A[f] + Bv) - Gr[f]);
+                msh = fmaxf(msh, ((A[f] + Bv) - Gr[f]) - D[u]);
             }
         }
-        const float M = fmaxf(Mns, block_max(msh, red));
-        const float theta = fmaxf(M - 2.0f * delta, bl - delta);
-        // phase 3: every item whose E-hat reaches theta is rescored exactly
-        float bv = 0.0f;
-        uint32_t bk = kNoItem;
-        uint32_t ncand = 0;
+        const float Mp = fmaxf(Mns, block_max(msh, red));
+        // phase 3: collect every item whose E-hat reaches theta_u = max(M' - delta_u, bl - delta_u)
+        bool overflow = false;
         for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
             const int u = e / S, s = e - u * S;
             const int64_t as = b + s - n;
@@ -311,6 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const float Bv = Bw[e];
             const int p = pidl[u];
             const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+            const float theta = fmaxf(Mp - D[u], bl - D[u]);
             if (p < 0) {
                 const float ra = pu ? RAall[s] : RApos[s];
                 if (ra == -INFINITY || ra + Bv < theta) continue;
@@ -322,16 +386,78 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                 const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
                 if (eh < theta) continue;
-                const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
-                const uint32_t k = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
-                argmax_merge(bv, bk, v, k);
-                ncand++;
+                const uint32_t slot = atomicAdd(&ncand_s, 1u);
+                if (slot < kCandCap) {
+                    cand_v[slot] = eh + D[u];  // upper bound of the exact value
+                    cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
+                } else {
+                    overflow = true;
+                }
             }
+        }
+        __syncthreads();
+        const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
+        float bv = 0.0f;
+        uint32_t bk = kNoItem;
+        if (__syncthreads_or(overflow)) {
+            // more candidates than the list holds: rescore them where they are found
+            for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+                const int u = e / S, s = e - u * S;
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const float Bv = Bw[e];
+                const int p = pidl[u];
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                for (int f = 0; f < S; f++) {
+                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                    const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
+                    if (eh < theta) continue;
+                    const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
+                    argmax_merge(bv, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
+                }
+            }
+        } else if (nc > 0) {
+            // round 1: the candidate with the largest upper bound, rescored exactly
+            float tv = -INFINITY;
+            uint32_t tk = kNoItem;
+            for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) argmax_merge(tv, tk, cand_v[i], cand_k[i]);
+            wave_argmax(tv, tk);
+            if (lane_id() == 0) { red[wv] = tv; redk[wv] = tk; }
+            __syncthreads();
+            if (wv == 0) {
+                float v0 = red[0];
+                uint32_t k0 = redk[0];
+                for (int i = 1; i < kBlock / 64; i++) argmax_merge(v0, k0, red[i], redk[i]);
+                const int u = (int)(k0 % (uint32_t)U);
+                const int f = (int)((k0 / (uint32_t)U) % (uint32_t)S), s = (int)(k0 / ((uint32_t)U * S));
+                const int nu = stage_rows_wave(ts, u, Tw, cw);
+                if (lane_id() == 0) {
+                    best1_v = exact_item_rows(vW, L, Tl, cl, nl, Tw, cw, nu, f, s, n);
+                    best1_k = k0;
+                }
+            }
+            __syncthreads();
+            bv = best1_v;
+            bk = best1_k;
+            // round 2: every other candidate whose upper bound reaches the exact value found
+            float* Tmy = Tw + wv * CM * L;
+            int* cmy = cw + wv * CM;
+            for (uint32_t i = wv; i < nc; i += kBlock / 64) {
+                const uint32_t k = cand_k[i];
+                if (k == best1_k || cand_v[i] < best1_v) continue;
+                const int u = (int)(k % (uint32_t)U);
+                const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
+                const int nu = stage_rows_wave(ts, u, Tmy, cmy);
+                if (lane_id() == 0) argmax_merge(bv, bk, exact_item_rows(vW, L, Tl, cl, nl, Tmy, cmy, nu, f, s, n), k);
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane_id() != 0) { bv = 0.0f; bk = kNoItem; }
         }
         wave_argmax(bv, bk);
         __syncthreads();
-        if (lane_id() == 0) { red[threadIdx.x >> 6] = bv; redk[threadIdx.x >> 6] = bk; }
-        if (ncand) atomicAdd(&ncand_s, ncand);
+        if (lane_id() == 0) { red[wv] = bv; redk[wv] = bk; }
         __syncthreads();
         if (threadIdx.x == 0) {
             float v = red[0];
@@ -370,10 +496,11 @@ size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
     return (size_t)ts.C * (*rp) * sizeof(float);
 }
 
-size_t screen_select_lds(const TemplateSet& ts, int S)
+size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
     const int nwords = (ts.U * S + 31) / 32;
-    return (size_t)(ts.C * ts.L + ts.C + 3 * S + ts.U) * 4 + (size_t)nwords * 4 + 2 * (size_t)S + 16;
+    return (size_t)(ts.C * ts.L + ts.C + 3 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
+           2 * (size_t)S + 16;
 }
 
 void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -387,7 +514,7 @@ void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTable
     else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    hipLaunchKernelGGL(k_screen_select, g, b, screen_select_lds(ts, tb.S), s, ts, st, tb, f0, nbatch, Bbuf);
+    hipLaunchKernelGGL(k_screen_select, g, b, screen_select_lds(ts, tb.S, tb.cmax), s, ts, st, tb, f0, nbatch, Bbuf);
 }
 
 }  // namespace fbp

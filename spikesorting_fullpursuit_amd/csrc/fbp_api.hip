@@ -22,8 +22,7 @@ void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
 void launch_iota(uint32_t* p, int64_t n, hipStream_t s);
 void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
 bool likelihood_fits(const TemplateSet& ts);
-void launch_likelihood(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* W, uint32_t nW, float* LIK,
-                       hipStream_t s);
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
 void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
@@ -122,7 +121,7 @@ struct fbp_handle {
     DevBuf pid, pair_l, pair_u, G, minG, Ntr, rowL1, SS, Bbuf;
     // chunk state
     int64_t N = 0, nwin = 0;
-    DevBuf V, LIK, best_idx, best_lab, best_lik, obest_idx, obest_lab;
+    DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
     DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf;
     DevBuf part_v, part_k;
     DevBuf sp_idx, sp_lab, clips;
@@ -173,6 +172,8 @@ static PassState pass_view(const fbp_handle* h)
     PassState st;
     st.V = h->V.as<float>();
     st.LIK = h->LIK.as<float>();
+    st.wmax_v = h->wmax_v.as<float>();
+    st.wmax_p = h->wmax_p.as<uint16_t>();
     st.best_idx = h->best_idx.as<uint32_t>();
     st.best_lab = h->best_lab.as<uint32_t>();
     st.best_lik = h->best_lik.as<float>();
@@ -335,7 +336,7 @@ int fbp_destroy(fbp_handle* h)
     if (!h) return FBP_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK,
+    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK, &h->wmax_v, &h->wmax_p,
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
@@ -445,6 +446,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     if (!Vsrc || N <= 0 || ld < N) return set_err(h, FBP_E_INVALID, "voltage must be non-empty with ld >= N");
     const int64_t L = h->L, C = h->C, U = h->U;
     if (N >= (int64_t)0xffffffffLL) return set_err(h, FBP_E_UNSUPPORTED, "chunk longer than 2^32-1 samples");
+    if (L > 65535) return set_err(h, FBP_E_UNSUPPORTED, "template length above 65535 samples");
     const uint64_t S = 2ull * n_max_shift + 1ull;
     const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
     if (recheck && S * S * (uint64_t)U >= 0xffffffffull)
@@ -462,6 +464,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     h->have_clips = false;
     HIP_TRY(h->V.ensure((size_t)C * N * 4), "alloc voltage");
     HIP_TRY(h->LIK.ensure((size_t)U * N * 4), "alloc likelihood");
+    HIP_TRY(h->wmax_v.ensure((size_t)U * nwin * 4), "alloc window maxima");
+    HIP_TRY(h->wmax_p.ensure((size_t)U * nwin * 2), "alloc window maxima");
     HIP_TRY(h->best_idx.ensure(nwin * 4), "alloc");
     HIP_TRY(h->best_lab.ensure(nwin * 4), "alloc");
     HIP_TRY(h->best_lik.ensure(nwin * 4), "alloc");
@@ -531,7 +535,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         stats.likelihood_positions += (int64_t)nW * L * U;
         stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
         tm.begin(TM_LIK);
-        launch_likelihood(ts, st.V, N, st.W, nW, st.LIK, s);
+        launch_likelihood(ts, st, nW, s);
         tm.end(TM_LIK);
         stats.n_likelihood_launches++;
         tm.begin(TM_MAX);

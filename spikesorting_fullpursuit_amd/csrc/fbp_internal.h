@@ -27,6 +27,8 @@ struct TemplateSet {
 struct PassState {
     float* V;              // [C][N] residual, row stride N
     float* LIK;            // [U][N]
+    float* wmax_v;         // [nwin][U] maximum of LIK over each window
+    uint16_t* wmax_p;      // [nwin][U] first position of that maximum in the window
     uint32_t* best_idx;    // [nwin]
     uint32_t* best_lab;
     float* best_lik;

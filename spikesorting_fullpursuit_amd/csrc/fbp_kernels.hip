@@ -19,14 +19,17 @@ namespace fbp {
 template <int LT>
 __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
                                                        const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
-                                                       float* __restrict__ LIK)
+                                                       float* __restrict__ LIK, float* __restrict__ wmax_v,
+                                                       uint16_t* __restrict__ wmax_p)
 {
-    extern __shared__ float sV[];  // [wb][C][sv]
+    extern __shared__ float sV[];  // [wb][C][sv], then per-item maxima [wb][U][ngr] (value, position)
     const int C = ts.C, U = ts.U;
     const int L = (LT > 0) ? LT : ts.L;
     const int ngr = (L + kPos - 1) / kPos;
     const int per_win = U * ngr;
     const uint32_t nbatch = (nW + wb - 1) / wb;
+    float* pv = sV + wb * C * sv;
+    uint32_t* pp = reinterpret_cast<uint32_t*>(pv + wb * per_win);
     for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
         const uint32_t w0 = bt * wb;
         const int nb = (int)min((uint32_t)wb, nW - w0);
@@ -58,11 +61,33 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
                 else lik_row_generic(trow, vrow, L, acc);
             }
             float* out = LIK + (size_t)t * N;
+            float mv = -INFINITY;
+            uint32_t mp = 0xffffu;
 #pragma unroll
             for (int q = 0; q < kPos; q++) {
                 const int64_t x = x0 + q;
-                if (p0 + q < L && x < N) out[x] = (x + L >= N) ? 0.0f : acc[q];
+                if (p0 + q < L && x < N) {
+                    const float v = (x + L >= N) ? 0.0f : acc[q];
+                    out[x] = v;
+                    if (v > mv) { mv = v; mp = (uint32_t)(p0 + q); }
+                }
             }
+            pv[it] = mv;
+            pp[it] = mp;
+        }
+        __syncthreads();
+        // per (window, template): first position of the window maximum
+        for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
+            const int b = pr / U, t = pr - b * U;
+            float mv = -INFINITY;
+            uint32_t mp = 0xffffu;
+            for (int g = 0; g < ngr; g++) {
+                const int it = b * per_win + t * ngr + g;
+                if (pv[it] > mv) { mv = pv[it]; mp = pp[it]; }
+            }
+            const size_t o = (size_t)W[w0 + b] * U + t;
+            wmax_v[o] = mv;
+            wmax_p[o] = (uint16_t)mp;
         }
     }
 }
@@ -72,7 +97,8 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
 // template in order, binary_pursuit.cl:350-441.  One wave per listed window;
 // lexicographic arg-max (value, then template, then position).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, const float* __restrict__ LIK,
+__global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, const float* __restrict__ wmax_v,
+                                                       const uint16_t* __restrict__ wmax_p,
                                                        const uint32_t* __restrict__ W, uint32_t nW, PassState st)
 {
     const uint32_t wpb = kBlock / 64;
@@ -80,18 +106,22 @@ __global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, 
         const int64_t id = W[w];
         const int64_t sw = id * L, ew = (id + 1) * L;
         if (ew >= N - L) continue;  // binary_pursuit.cl:380-383
-        const int64_t start = (L > sw) ? 0 : sw - L;
-        const int64_t stop = (ew + L > N) ? N : ew + L;
+        // the scanned range [max(0, sw-L), min(N, ew+L)) is windows id-1 .. id+1
+        const int64_t wlo = id > 0 ? id - 1 : 0;
+        const int64_t whi = min(st.nwin - 1, id + 1);
+        const int64_t start = wlo * L;
+        const uint32_t span = (uint32_t)((whi + 1) * L - start);
         float best = 0.0f;
         uint32_t key = kNoItem;  // t * span + (x - start)
-        const uint32_t span = (uint32_t)(stop - start);
         bool cw = false;
-        for (int t = 0; t < U; t++) {
-            const float* lr = LIK + (size_t)t * N;
-            for (int64_t x = start + lane_id(); x < stop; x += 64) {
-                const float v = lr[x];
-                if (v > best) { best = v; key = (uint32_t)t * span + (uint32_t)(x - start); }
-                cw |= (v > 0.0f) && (x >= sw) && (x < ew);
+        for (int t = lane_id(); t < U; t += 64) {
+            for (int64_t wd = wlo; wd <= whi; wd++) {
+                const float v = wmax_v[(size_t)wd * U + t];
+                if (v > best) {
+                    best = v;
+                    key = (uint32_t)t * span + (uint32_t)(wd * L - start) + wmax_p[(size_t)wd * U + t];
+                }
+                cw |= (wd == id) && (v > 0.0f);
             }
         }
         wave_argmax(best, key);
@@ -496,6 +526,12 @@ void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64
 }
 
 // Windows per likelihood block and LDS row length.
+static int64_t likelihood_lds_per_window(const TemplateSet& ts, int sv)
+{
+    const int ngr = (ts.L + kPos - 1) / kPos;
+    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 8;
+}
+
 void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
 {
     const int L = ts.L;
@@ -504,7 +540,7 @@ void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
     const int per_win = ts.U * ngr;
     int w = (2 * kBlock + per_win - 1) / per_win;
     if (w < 1) w = 1;
-    const int64_t lds_per = (int64_t)ts.C * (*sv) * 4;
+    const int64_t lds_per = likelihood_lds_per_window(ts, *sv);
     while (w > 1 && w * lds_per > 48 * 1024) w--;
     *wb = w;
 }
@@ -513,29 +549,34 @@ bool likelihood_fits(const TemplateSet& ts)
 {
     int wb, sv;
     likelihood_plan(ts, &wb, &sv);
-    return (int64_t)wb * ts.C * sv * 4 <= 64 * 1024;
+    return wb * likelihood_lds_per_window(ts, sv) <= 64 * 1024;
 }
 
-void launch_likelihood(const TemplateSet& ts, const float* V, int64_t N, const uint32_t* W, uint32_t nW, float* LIK,
-                       hipStream_t s)
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
 {
     if (nW == 0) return;
     int wb, sv;
     likelihood_plan(ts, &wb, &sv);
-    const size_t lds = (size_t)wb * ts.C * sv * sizeof(float);
+    const size_t lds = (size_t)wb * likelihood_lds_per_window(ts, sv);
     const uint32_t nbatch = (nW + wb - 1) / wb;
     const dim3 g(grid_for(nbatch, 1, 1u << 20)), b(kBlock);
-    if (ts.L == 48) hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
-    else if (ts.L == 64) hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
-    else if (ts.L == 32) hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
-    else hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, LIK);
+    const float* V = st.V;
+    const int64_t N = st.N;
+    if (ts.L == 48)
+        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+    else if (ts.L == 64)
+        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+    else if (ts.L == 32)
+        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+    else
+        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
 }
 
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
 {
     if (nW == 0) return;
     hipLaunchKernelGGL(k_window_max, dim3(grid_for(nW, kBlock / 64, 1u << 20)), dim3(kBlock), 0, s, ts.U, ts.L, st.N,
-                       st.LIK, st.W, nW, st);
+                       st.wmax_v, st.wmax_p, st.W, nW, st);
 }
 
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s)

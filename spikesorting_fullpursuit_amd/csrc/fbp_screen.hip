@@ -182,9 +182,11 @@ __device__ __forceinline__ float block_max(float v, float* red)
 // Exact FP32 value of item (u, f, s) with both templates' non-zero channel rows
 // in LDS (rows of L floats, channels ascending).  Same operation order as
 // recheck_item_value: channels of either template ascending, then samples.
-__device__ inline float exact_item_rows(const float* vW, int L, const float* Tl, const int* cl, int nl, const float* Tu,
+template <int LT>
+__device__ inline float exact_item_rows(const float* vW, int Lrt, const float* Tl, const int* cl, int nl, const float* Tu,
                                         const int* cu, int nu, int f, int s, int n)
 {
+    const int L = LT > 0 ? LT : Lrt;
     const int fa = n - f, sa = n - s;
     const int ffj = fa < 0 ? -fa : 0, sfj = sa < 0 ? -sa : 0;
     float lik = 0.0f;
@@ -193,13 +195,18 @@ __device__ inline float exact_item_rows(const float* vW, int L, const float* Tl,
         const int c1 = il < nl ? cl[il] : 0x7fffffff;
         const int c2 = iu < nu ? cu[iu] : 0x7fffffff;
         const int c = min(c1, c2);
-        const float* tf = (c1 == c) ? Tl + (il++) * L : nullptr;
-        const float* tsr = (c2 == c) ? Tu + (iu++) * L : nullptr;
+        const bool hf = c1 == c, hs = c2 == c;
+        const float* tf = Tl + (hf ? il : 0) * L;
+        const float* tsr = Tu + (hs ? iu : 0) * L;
+        il += hf;
+        iu += hs;
         const float* vr = vW + c * L;
         float ss = 0.0f;
+#pragma unroll 8
         for (int j = 0; j < L; j++) {
-            const float fv = (tf && j >= ffj && fa + j < L) ? tf[fa + j] : 0.0f;
-            const float sv = (tsr && j >= sfj && sa + j < L) ? tsr[sa + j] : 0.0f;
+            const int jf = fa + j, js = sa + j;
+            const float fv = (hf && j >= ffj && jf < L) ? tf[jf] : 0.0f;
+            const float sv = (hs && j >= sfj && js < L) ? tsr[js] : 0.0f;
             const float sum = fv + sv;
             ss = fmaf(sum, sum, ss);
             lik = fmaf(sum, vr[j], lik);
@@ -219,14 +226,16 @@ __device__ __forceinline__ int stage_rows_wave(const TemplateSet& ts, int u, flo
         dst[e] = ts.rows[(size_t)(r0 + k) * ts.Lp + j];
     }
     for (int k = lane_id(); k < nr; k += 64) dch[k] = ts.chan[r0 + k];
-    __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
     return nr;
 }
 
 constexpr int kCandCap = 512;
+constexpr int kListChunk = 2048;  // (unit, shift) pairs examined per work-list round
 
 // Selection, exact rescoring and the parse cascade for one window per block.
+template <int LT>
 __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
                                                           uint32_t nbatch, const float* __restrict__ Bbuf)
 {
@@ -235,13 +244,17 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     __shared__ uint32_t redk[kBlock / 64];
     __shared__ float cand_v[kCandCap];
     __shared__ uint32_t cand_k[kCandCap];
+    __shared__ uint32_t wl[kListChunk];
+    __shared__ uint32_t nwl;
     __shared__ uint32_t ncand_s;
     __shared__ float best1_v;
     __shared__ uint32_t best1_k;
-    const int C = ts.C, U = ts.U, L = ts.L;
+    const int C = ts.C, U = ts.U;
+    const int L = LT > 0 ? LT : ts.L;
     const int S = tb.S, n = tb.n, CM = tb.cmax;
+    const int US = U * S;
     const int64_t N = st.N;
-    const int nwords = (U * S + 31) / 32;
+    const int nwords = (US + 63) / 64 * 2;
     float* vW = sm;                       // [C][L]
     float* vmax = vW + C * L;             // [C]
     float* A = vmax + C;                  // [S]
@@ -256,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     uint32_t* posu = reinterpret_cast<uint32_t*>(cw + 4 * CM);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
     uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
-    const int wv = threadIdx.x >> 6;
+    const int wv = threadIdx.x >> 6, lane = lane_id();
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
         const uint32_t w = f0 + wi;
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
             continue;
         }
-        const float* Bw = Bbuf + (size_t)wi * U * S;
+        const float* Bw = Bbuf + (size_t)wi * US;
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
             const int c = e / L, j = e - c * L;
             vW[e] = st.V[(size_t)c * N + b + j];
@@ -288,16 +301,20 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
             A[f] = Bw[(size_t)lab * S + f];
         }
-        for (int wd = threadIdx.x; wd < nwords; wd += blockDim.x) {
-            uint32_t bits = 0;
-            for (int k = 0; k < 32; k++) {
-                const int e = wd * 32 + k;
-                if (e >= U * S) break;
+        // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs
+        for (int base = wv * 64; base < US; base += kBlock) {
+            const int e = base + lane;
+            bool bit = false;
+            if (e < US) {
                 const int u = e / S, s = e - u * S;
                 const int64_t as = b + s - n;
-                if (as >= 0 && as < N - L && st.LIK[(size_t)u * N + as] > 0.0f) bits |= 1u << k;
+                bit = as >= 0 && as < N - L && st.LIK[(size_t)u * N + as] > 0.0f;
             }
-            posu[wd] = bits;
+            const uint64_t m = __ballot(bit);
+            if (lane == 0) {
+                posu[base >> 5] = (uint32_t)m;
+                posu[(base >> 5) + 1] = (uint32_t)(m >> 32);
+            }
         }
         if (threadIdx.x == 0) ncand_s = 0;
         __syncthreads();
@@ -335,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         // Tracks M' = max(E-hat - delta_u), a lower bound of the exact maximum.
         const int* pidl = tb.pid + (size_t)lab * U;
         float mns = -INFINITY;
-        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+        for (int e = threadIdx.x; e < US; e += blockDim.x) {
             const int u = e / S, s = e - u * S;
             if (pidl[u] >= 0) continue;
             const int64_t as = b + s - n;
@@ -346,52 +363,78 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             mns = fmaxf(mns, (ra + Bw[e]) - D[u]);
         }
         const float Mns = block_max(mns, red);
-        // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G
+        // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G;
+        // survivors go to a work list, each enumerated by one wave over f
         float msh = -INFINITY;
-        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
-            const int u = e / S, s = e - u * S;
-            const int p = pidl[u];
-            if (p < 0) continue;
-            const int64_t as = b + s - n;
-            if (as < 0 || as >= N - L) continue;
-            const float Bv = Bw[e];
-            const float tp = fmaxf(Mns - D[u], bl - D[u]);
-            if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < tp) continue;
-            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-            const float* Gr = tb.G + ((size_t)p * S + s) * S;
-            for (int f = 0; f < S; f++) {
-                if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                msh = fmaxf(msh, ((A[f] + Bv) - Gr[f]) - D[u]);
+        for (int e0 = 0; e0 < US; e0 += kListChunk) {
+            if (threadIdx.x == 0) nwl = 0;
+            __syncthreads();
+            for (int e = e0 + threadIdx.x; e < min(US, e0 + kListChunk); e += blockDim.x) {
+                const int u = e / S, s = e - u * S;
+                const int p = pidl[u];
+                if (p < 0) continue;
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const float tp = fmaxf(Mns - D[u], bl - D[u]);
+                if ((maxA + Bw[e]) - tb.minG[(size_t)p * S + s] < tp) continue;
+                wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
+            }
+            __syncthreads();
+            for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
+                const int e = (int)wl[i];
+                const int u = e / S, s = e - u * S;
+                const float Bv = Bw[e];
+                const float Du = D[u];
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                const float* Gr = tb.G + ((size_t)pidl[u] * S + s) * S;
+                for (int f = lane; f < S; f += 64) {
+                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                    msh = fmaxf(msh, ((A[f] + Bv) - Gr[f]) - Du);
+                }
             }
         }
         const float Mp = fmaxf(Mns, block_max(msh, red));
-        // phase 3: collect every item whose E-hat reaches theta_u = max(M' - delta_u, bl - delta_u)
+        // phase 3: every item whose E-hat reaches theta_u = max(M' - delta_u, bl - delta_u)
         bool overflow = false;
-        for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
-            const int u = e / S, s = e - u * S;
-            const int64_t as = b + s - n;
-            if (as < 0 || as >= N - L) continue;
-            const float Bv = Bw[e];
-            const int p = pidl[u];
-            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-            const float theta = fmaxf(Mp - D[u], bl - D[u]);
-            if (p < 0) {
-                const float ra = pu ? RAall[s] : RApos[s];
-                if (ra == -INFINITY || ra + Bv < theta) continue;
-            } else if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < theta) {
-                continue;
+        for (int e0 = 0; e0 < US; e0 += kListChunk) {
+            if (threadIdx.x == 0) nwl = 0;
+            __syncthreads();
+            for (int e = e0 + threadIdx.x; e < min(US, e0 + kListChunk); e += blockDim.x) {
+                const int u = e / S, s = e - u * S;
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const float Bv = Bw[e];
+                const int p = pidl[u];
+                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                if (p < 0) {
+                    const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                    const float ra = pu ? RAall[s] : RApos[s];
+                    if (ra == -INFINITY || ra + Bv < theta) continue;
+                } else if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < theta) {
+                    continue;
+                }
+                wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
             }
-            const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
-            for (int f = 0; f < S; f++) {
-                if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
-                if (eh < theta) continue;
-                const uint32_t slot = atomicAdd(&ncand_s, 1u);
-                if (slot < kCandCap) {
-                    cand_v[slot] = eh + D[u];  // upper bound of the exact value
-                    cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
-                } else {
-                    overflow = true;
+            __syncthreads();
+            for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
+                const int e = (int)wl[i];
+                const int u = e / S, s = e - u * S;
+                const float Bv = Bw[e];
+                const int p = pidl[u];
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                for (int f = lane; f < S; f += 64) {
+                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                    const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
+                    if (eh < theta) continue;
+                    const uint32_t slot = atomicAdd(&ncand_s, 1u);
+                    if (slot < kCandCap) {
+                        cand_v[slot] = eh + D[u];  // upper bound of the exact value
+                        cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
+                    } else {
+                        overflow = true;
+                    }
                 }
             }
         }
@@ -401,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         uint32_t bk = kNoItem;
         if (__syncthreads_or(overflow)) {
             // more candidates than the list holds: rescore them where they are found
-            for (int e = threadIdx.x; e < U * S; e += blockDim.x) {
+            for (int e = threadIdx.x; e < US; e += blockDim.x) {
                 const int u = e / S, s = e - u * S;
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
@@ -424,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             uint32_t tk = kNoItem;
             for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) argmax_merge(tv, tk, cand_v[i], cand_k[i]);
             wave_argmax(tv, tk);
-            if (lane_id() == 0) { red[wv] = tv; redk[wv] = tk; }
+            if (lane == 0) { red[wv] = tv; redk[wv] = tk; }
             __syncthreads();
             if (wv == 0) {
                 float v0 = red[0];
@@ -433,8 +476,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int u = (int)(k0 % (uint32_t)U);
                 const int f = (int)((k0 / (uint32_t)U) % (uint32_t)S), s = (int)(k0 / ((uint32_t)U * S));
                 const int nu = stage_rows_wave(ts, u, Tw, cw);
-                if (lane_id() == 0) {
-                    best1_v = exact_item_rows(vW, L, Tl, cl, nl, Tw, cw, nu, f, s, n);
+                if (lane == 0) {
+                    best1_v = exact_item_rows<LT>(vW, L, Tl, cl, nl, Tw, cw, nu, f, s, n);
                     best1_k = k0;
                 }
             }
@@ -450,14 +493,14 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int u = (int)(k % (uint32_t)U);
                 const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
                 const int nu = stage_rows_wave(ts, u, Tmy, cmy);
-                if (lane_id() == 0) argmax_merge(bv, bk, exact_item_rows(vW, L, Tl, cl, nl, Tmy, cmy, nu, f, s, n), k);
+                if (lane == 0) argmax_merge(bv, bk, exact_item_rows<LT>(vW, L, Tl, cl, nl, Tmy, cmy, nu, f, s, n), k);
                 __builtin_amdgcn_wave_barrier();
             }
-            if (lane_id() != 0) { bv = 0.0f; bk = kNoItem; }
+            if (lane != 0) { bv = 0.0f; bk = kNoItem; }
         }
         wave_argmax(bv, bk);
         __syncthreads();
-        if (lane_id() == 0) { red[wv] = bv; redk[wv] = bk; }
+        if (lane == 0) { red[wv] = bv; redk[wv] = bk; }
         __syncthreads();
         if (threadIdx.x == 0) {
             float v = red[0];
@@ -498,7 +541,7 @@ size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
 
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
-    const int nwords = (ts.U * S + 31) / 32;
+    const int nwords = (ts.U * S + 63) / 64 * 2;
     return (size_t)(ts.C * ts.L + ts.C + 3 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
            2 * (size_t)S + 16;
 }
@@ -514,7 +557,11 @@ void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTable
     else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    hipLaunchKernelGGL(k_screen_select, g, b, screen_select_lds(ts, tb.S, tb.cmax), s, ts, st, tb, f0, nbatch, Bbuf);
+    const size_t lds2 = screen_select_lds(ts, tb.S, tb.cmax);
+    if (ts.L == 48) hipLaunchKernelGGL(k_screen_select<48>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_select<64>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_select<32>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    else hipLaunchKernelGGL(k_screen_select<0>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
 }
 
 }  // namespace fbp

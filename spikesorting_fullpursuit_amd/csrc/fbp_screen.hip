@@ -260,7 +260,11 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     float* A = vmax + C;                  // [S]
     float* RAall = A + S;                 // [S]
     float* RApos = RAall + S;             // [S]
-    float* Q = RApos + S;                 // [U]
+    float* PMa = RApos + S;               // [S] prefix / suffix maxima of A
+    float* PMp = PMa + S;
+    float* SMa = PMp + S;
+    float* SMp = SMa + S;
+    float* Q = SMp + S;                   // [U]
     float* D = Q + U;                     // [U] per-unit bound delta
     float* Tl = D + U;                    // [CM][L] rows of the window's label
     float* Tw = Tl + CM * L;              // [4][CM][L] rows staged by each wave
@@ -301,19 +305,28 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
             A[f] = Bw[(size_t)lab * S + f];
         }
-        // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs
-        for (int base = wv * 64; base < US; base += kBlock) {
-            const int e = base + lane;
-            bool bit = false;
-            if (e < US) {
-                const int u = e / S, s = e - u * S;
-                const int64_t as = b + s - n;
-                bit = as >= 0 && as < N - L && st.LIK[(size_t)u * N + as] > 0.0f;
+        // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs;
+        // each wave keeps 4 loads in flight
+        for (int base0 = wv * 64; base0 < US; base0 += 4 * kBlock) {
+            float lv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int e = base0 + r * kBlock + lane;
+                lv[r] = 0.0f;
+                if (e < US) {
+                    const int u = e / S, s = e - u * S;
+                    const int64_t as = b + s - n;
+                    if (as >= 0 && as < N - L) lv[r] = st.LIK[(size_t)u * N + as];
+                }
             }
-            const uint64_t m = __ballot(bit);
-            if (lane == 0) {
-                posu[base >> 5] = (uint32_t)m;
-                posu[(base >> 5) + 1] = (uint32_t)(m >> 32);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int base = base0 + r * kBlock;
+                const uint64_t m = __ballot(lv[r] > 0.0f);
+                if (lane == 0 && base < US) {
+                    posu[base >> 5] = (uint32_t)m;
+                    posu[(base >> 5) + 1] = (uint32_t)(m >> 32);
+                }
             }
         }
         if (threadIdx.x == 0) ncand_s = 0;
@@ -323,16 +336,52 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * L + j]));
             vmax[c] = m;
         }
-        // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
-        for (int s = threadIdx.x; s < S; s += blockDim.x) {
-            float ma = -INFINITY, mp = -INFINITY;
-            const int fl = max(0, s - L), fh = min(S - 1, s + L);
-            for (int f = fl; f <= fh; f++) {
-                if (fin[f]) ma = fmaxf(ma, A[f]);
-                if (posl[f]) mp = fmaxf(mp, A[f]);
+        // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0.
+        // With n <= L every range starts at 0 or ends at S-1, so prefix maxima
+        // (wave 0) and suffix maxima (wave 1), each a wave-wide scan, answer them.
+        if (n <= L) {
+            if (wv < 2) {
+                const bool suffix = wv == 1;
+                float* Xa = suffix ? SMa : PMa;
+                float* Xp = suffix ? SMp : PMp;
+                float ca = -INFINITY, cp = -INFINITY;
+                for (int c0 = 0; c0 < S; c0 += 64) {
+                    const int f = suffix ? S - 1 - (c0 + lane) : c0 + lane;
+                    const bool ok = c0 + lane < S;
+                    float va = -INFINITY, vp = -INFINITY;
+                    if (ok) {
+                        if (fin[f]) va = A[f];
+                        if (posl[f]) vp = A[f];
+                    }
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const float ta = __shfl_up(va, off, 64), tq = __shfl_up(vp, off, 64);
+                        if (lane >= off) { va = fmaxf(va, ta); vp = fmaxf(vp, tq); }
+                    }
+                    va = fmaxf(va, ca);
+                    vp = fmaxf(vp, cp);
+                    if (ok) { Xa[f] = va; Xp[f] = vp; }
+                    ca = __shfl(va, 63, 64);
+                    cp = __shfl(vp, 63, 64);
+                }
             }
-            RAall[s] = ma;
-            RApos[s] = mp;
+            __syncthreads();
+            for (int s = threadIdx.x; s < S; s += blockDim.x) {
+                const int fl = max(0, s - L), fh = min(S - 1, s + L);
+                RAall[s] = (fl == 0) ? PMa[fh] : SMa[fl];
+                RApos[s] = (fl == 0) ? PMp[fh] : SMp[fl];
+            }
+        } else {
+            for (int s = threadIdx.x; s < S; s += blockDim.x) {
+                float ma = -INFINITY, mp = -INFINITY;
+                const int fl = max(0, s - L), fh = min(S - 1, s + L);
+                for (int f = fl; f <= fh; f++) {
+                    if (fin[f]) ma = fmaxf(ma, A[f]);
+                    if (posl[f]) mp = fmaxf(mp, A[f]);
+                }
+                RAall[s] = ma;
+                RApos[s] = mp;
+            }
         }
         __syncthreads();
         for (int u = threadIdx.x; u < U; u += blockDim.x) {
@@ -352,15 +401,26 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         // Tracks M' = max(E-hat - delta_u), a lower bound of the exact maximum.
         const int* pidl = tb.pid + (size_t)lab * U;
         float mns = -INFINITY;
-        for (int e = threadIdx.x; e < US; e += blockDim.x) {
-            const int u = e / S, s = e - u * S;
-            if (pidl[u] >= 0) continue;
-            const int64_t as = b + s - n;
-            if (as < 0 || as >= N - L) continue;
-            const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-            const float ra = pu ? RAall[s] : RApos[s];
-            if (ra == -INFINITY) continue;
-            mns = fmaxf(mns, (ra + Bw[e]) - D[u]);
+        for (int e0 = threadIdx.x; e0 < US; e0 += 4 * kBlock) {
+            float bv4[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int e = e0 + r * kBlock;
+                bv4[r] = e < US ? Bw[e] : 0.0f;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int e = e0 + r * kBlock;
+                if (e >= US) continue;
+                const int u = e / S, s = e - u * S;
+                if (pidl[u] >= 0) continue;
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                const float ra = pu ? RAall[s] : RApos[s];
+                if (ra == -INFINITY) continue;
+                mns = fmaxf(mns, (ra + bv4[r]) - D[u]);
+            }
         }
         const float Mns = block_max(mns, red);
         // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G;
@@ -542,7 +602,7 @@ size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
     const int nwords = (ts.U * S + 63) / 64 * 2;
-    return (size_t)(ts.C * ts.L + ts.C + 3 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
            2 * (size_t)S + 16;
 }
 

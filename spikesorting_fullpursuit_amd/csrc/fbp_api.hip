@@ -48,6 +48,7 @@ void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTable
 static const size_t kMaxGramBytes = (size_t)8 << 30;
 static const size_t kMaxBbufBytes = (size_t)2 << 30;
 static const size_t kMaxLds = 64 * 1024;
+static const size_t kMaxLdsSelect = 160 * 1024;
 
 using namespace fbp;
 
@@ -268,7 +269,7 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     const int64_t S = 2 * n + 1;
     TemplateSet ts = template_view(h);
     int rp;
-    if (screen_corr_lds(ts, (int)S, &rp) > kMaxLds || screen_select_lds(ts, (int)S, std::max(h->Cmax, 1)) > kMaxLds - 8192)
+    if (screen_corr_lds(ts, (int)S, &rp) > kMaxLds || screen_select_lds(ts, (int)S, std::max(h->Cmax, 1)) > kMaxLdsSelect - 16384)
         return FBP_OK;
     if ((size_t)U * S * 4 > kMaxBbufBytes) return FBP_OK;
     std::vector<int> pid((size_t)U * U, -1), pl, pu;

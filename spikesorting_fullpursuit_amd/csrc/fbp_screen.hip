@@ -273,6 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     uint32_t* posu = reinterpret_cast<uint32_t*>(cw + 4 * CM);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
     uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
+    float* Bl = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(posl + S) + 15) & ~(uintptr_t)15);  // [U*S] B(u, s)
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
@@ -287,7 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
             continue;
         }
-        const float* Bw = Bbuf + (size_t)wi * US;
+        const float* Bg = Bbuf + (size_t)wi * US;
+        for (int e = threadIdx.x; e < US; e += blockDim.x) Bl[e] = Bg[e];
+        const float* Bw = Bl;
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
             const int c = e / L, j = e - c * L;
             vW[e] = st.V[(size_t)c * N + b + j];
@@ -303,7 +306,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const bool in = af >= 0 && af < N - L;
             fin[f] = in;
             posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
-            A[f] = Bw[(size_t)lab * S + f];
+            A[f] = Bg[(size_t)lab * S + f];
         }
         // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs;
         // each wave keeps 4 loads in flight
@@ -603,7 +606,16 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
     const int nwords = (ts.U * S + 63) / 64 * 2;
     return (size_t)(ts.C * ts.L + ts.C + 7 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
-           2 * (size_t)S + 16;
+           2 * (size_t)S + 16 + (size_t)ts.U * S * 4 + 16;
+}
+
+// Dynamic LDS above 64 KiB needs the per-kernel attribute (gfx950 allows 160 KiB).
+template <class K>
+static void allow_lds(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes);
 }
 
 void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -618,10 +630,19 @@ void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTable
     else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     const size_t lds2 = screen_select_lds(ts, tb.S, tb.cmax);
-    if (ts.L == 48) hipLaunchKernelGGL(k_screen_select<48>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
-    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_select<64>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
-    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_select<32>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
-    else hipLaunchKernelGGL(k_screen_select<0>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    if (ts.L == 48) {
+        allow_lds(k_screen_select<48>, lds2);
+        hipLaunchKernelGGL(k_screen_select<48>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    } else if (ts.L == 64) {
+        allow_lds(k_screen_select<64>, lds2);
+        hipLaunchKernelGGL(k_screen_select<64>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    } else if (ts.L == 32) {
+        allow_lds(k_screen_select<32>, lds2);
+        hipLaunchKernelGGL(k_screen_select<32>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    } else {
+        allow_lds(k_screen_select<0>, lds2);
+        hipLaunchKernelGGL(k_screen_select<0>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
+    }
 }
 
 }  // namespace fbp

@@ -38,6 +38,7 @@ extern "C" {
 #define FBP_FLAG_ADJUSTED_CLIPS 2u   /* sort_info["get_adjusted_clips"], binary_pursuit_parallel.py:1404-1486 */
 #define FBP_FLAG_RECHECK_LITERAL 4u  /* evaluate every overlap item literally (no screening) */
 #define FBP_FLAG_TIMING 8u           /* record per-kernel device time with HIP events */
+#define FBP_FLAG_SCREEN_FP32 16u     /* overlap screen correlations on FP32 vector units instead of split-fp16 GEMMs */
 
 typedef struct fbp_handle fbp_handle;
 

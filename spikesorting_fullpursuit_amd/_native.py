@@ -22,6 +22,7 @@ FLAG_OVERLAP_RECHECK = 1
 FLAG_ADJUSTED_CLIPS = 2
 FLAG_RECHECK_LITERAL = 4
 FLAG_TIMING = 8
+FLAG_SCREEN_FP32 = 16
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (

@@ -5,6 +5,7 @@
 // window lists on the device; the host reads two small counters per pass
 // (overlap windows, then spikes added and next list length).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -40,8 +41,14 @@ void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, con
 void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s);
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
-void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
-                   float* Bbuf, hipStream_t s);
+void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
+                        float* Bbuf, hipStream_t s);
+void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                          uint32_t nbatch, const float* Bbuf, hipStream_t s);
+void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int K, int S, int n, float tscale,
+                   _Float16* Ah, _Float16* Al, hipStream_t s);
+void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
+                      _Float16* Vl, float* vscale, hipStream_t s);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -120,6 +127,17 @@ struct fbp_handle {
     int64_t npairs = 0;
     bool screen_ok = false;
     DevBuf pid, pair_l, pair_u, G, minG, Ntr, rowL1, SS, Bbuf;
+    // split-fp16 GEMM form of the screen correlations
+    struct Group {
+        int c_lo, K, M, row_off;
+        size_t a_off;
+        int units_off, nunits;
+    };
+    std::vector<Group> groups;
+    bool gemm_ok = false;
+    float tscale = 1.0f;
+    rocblas_handle rb = nullptr;
+    DevBuf Ah, Al, brow, gunits, Vh, Vl, vscale;
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
@@ -255,7 +273,133 @@ static ScreenTables tables_view(const fbp_handle* h)
     const double K = 2.0 * h->Cmax * (h->L + 1) + 2.0 * h->L + 16.0;
     const double u = std::ldexp(1.0, -24);
     tb.kappa = (float)(4.0 * K * u / (1.0 - K * u));
+    // split-fp16 GEMM: 3 GEMMs of at most Cmax*L non-zero products each
+    // (gamma_{3K2} with the unit roundoff doubled for an unknown accumulation
+    // order), the hi/lo split residue (3 * 2^-22) and the scaling, doubled.
+    const double K2 = 3.0 * h->Cmax * h->L + 8.0;
+    const double u2 = 2.0 * u;
+    tb.kappa2 = (float)(2.0 * (K2 * u2 / (1.0 - K2 * u2) + 4.0 * std::ldexp(1.0, -22)));
+    tb.gemm = 0;
+    tb.brow = h->brow.as<int>();
+    tb.vscale = h->vscale.as<float>();
+    tb.inv_tscale = 1.0f / h->tscale;
     return tb;
+}
+
+// Groups units whose non-zero channels fit in a window of at most 16 channels
+// (or one unit's own span when wider) and builds the split-fp16 matrices.
+static int build_gemm_tables(fbp_handle* h, int64_t n)
+{
+    h->gemm_ok = false;
+    const int U = h->U, C = h->C, L = h->L;
+    const int S = (int)(2 * n + 1);
+    std::vector<int> first(U, 0), last(U, 0), order(U);
+    float tmax = 0.0f;
+    for (int u = 0; u < U; u++) {
+        int f = -1, l = -1;
+        for (int c = 0; c < C; c++)
+            if (h->h_nz[(size_t)u * C + c]) {
+                if (f < 0) f = c;
+                l = c;
+            }
+        first[u] = f < 0 ? 0 : f;
+        last[u] = l < 0 ? 0 : l;
+        order[u] = u;
+    }
+    std::vector<float> hT((size_t)U * C * L);
+    HIP_TRY(hipMemcpy(hT.data(), h->T.p, hT.size() * 4, hipMemcpyDeviceToHost), "copy templates");
+    for (float v : hT) tmax = std::max(tmax, std::fabs(v));
+    int ex = 0;
+    std::frexp(tmax > 0.0f ? tmax : 1.0f, &ex);
+    h->tscale = std::ldexp(1.0f, 14 - ex);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return first[a] != first[b] ? first[a] < first[b] : last[a] < last[b];
+    });
+    const int Wmax = 16;
+    h->groups.clear();
+    std::vector<int> gunits;
+    std::vector<int> brow(U, 0);
+    int row = 0;
+    size_t aoff = 0;
+    size_t i = 0;
+    while (i < order.size()) {
+        const int lo = first[order[i]];
+        int hi = last[order[i]];
+        size_t j = i + 1;
+        while (j < order.size() && std::max(hi, last[order[j]]) - lo + 1 <= std::max(Wmax, last[order[i]] - lo + 1)) {
+            hi = std::max(hi, last[order[j]]);
+            j++;
+        }
+        fbp_handle::Group g;
+        g.c_lo = lo;
+        g.K = (hi - lo + 1) * L;
+        g.nunits = (int)(j - i);
+        g.M = g.nunits * S;
+        g.row_off = row;
+        g.a_off = aoff;
+        g.units_off = (int)gunits.size();
+        for (size_t k = i; k < j; k++) {
+            brow[order[k]] = row + (int)(k - i) * S;
+            gunits.push_back(order[k]);
+        }
+        row += g.M;
+        aoff += (size_t)g.M * g.K;
+        h->groups.push_back(g);
+        i = j;
+    }
+    if (aoff * 2 * sizeof(_Float16) > kMaxGramBytes) return FBP_OK;
+    if (!h->rb) {
+        if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
+            h->rb = nullptr;
+            return FBP_OK;
+        }
+    }
+    if (rocblas_set_stream(h->rb, h->stream) != rocblas_status_success) return FBP_OK;
+    HIP_TRY(h->Ah.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
+    HIP_TRY(h->Al.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
+    HIP_TRY(h->brow.ensure((size_t)U * 4), "alloc gemm rows");
+    HIP_TRY(h->gunits.ensure(gunits.size() * 4), "alloc gemm rows");
+    HIP_TRY(hipMemcpy(h->brow.p, brow.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->gunits.p, gunits.data(), gunits.size() * 4, hipMemcpyHostToDevice), "copy");
+    TemplateSet ts = template_view(h);
+    for (const auto& g : h->groups)
+        launch_gemm_A(ts, h->gunits.as<int>() + g.units_off, g.nunits, g.c_lo, g.K, S, (int)n, h->tscale,
+                      h->Ah.as<_Float16>() + g.a_off, h->Al.as<_Float16>() + g.a_off, h->stream);
+    HIP_TRY(hipGetLastError(), "gemm tables");
+    HIP_TRY(hipStreamSynchronize(h->stream), "gemm tables");
+    h->gemm_ok = true;
+    return FBP_OK;
+}
+
+// B for the flagged windows F[f0 .. f0+nb) into Bbuf[w][row] by three
+// split-fp16 GEMMs per unit group.
+static int run_screen_gemm(fbp_handle* h, const PassState& st, uint32_t f0, uint32_t nb)
+{
+    TemplateSet ts = template_view(h);
+    launch_gemm_pack(ts, st, f0, nb, h->Vh.as<_Float16>(), h->Vl.as<_Float16>(), h->vscale.as<float>(), h->stream);
+    const int S = (int)(2 * h->tab_n + 1);
+    const int US = h->U * S;
+    const int CL = h->C * h->L;
+    const float one = 1.0f, zero = 0.0f;
+    const _Float16* Vh = h->Vh.as<_Float16>();
+    const _Float16* Vl = h->Vl.as<_Float16>();
+    for (const auto& g : h->groups) {
+        const _Float16* Ah = h->Ah.as<_Float16>() + g.a_off;
+        const _Float16* Al = h->Al.as<_Float16>() + g.a_off;
+        float* Cg = h->Bbuf.as<float>() + g.row_off;
+        const _Float16* Bs[3] = {Vh, Vl, Vh};
+        const _Float16* As[3] = {Ah, Ah, Al};
+        for (int k = 0; k < 3; k++) {
+            rocblas_status rs = rocblas_gemm_ex(
+                h->rb, rocblas_operation_transpose, rocblas_operation_none, g.M, (int)nb, g.K, &one, As[k],
+                rocblas_datatype_f16_r, g.K, Bs[k] + (size_t)g.c_lo * h->L, rocblas_datatype_f16_r, CL,
+                k == 0 ? &zero : &one, Cg, rocblas_datatype_f32_r, US, Cg, rocblas_datatype_f32_r, US,
+                rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+            if (rs != rocblas_status_success)
+                return set_err(h, FBP_E_HIP, std::string("rocblas_gemm_ex failed: ") + rocblas_status_to_string(rs));
+        }
+    }
+    return FBP_OK;
 }
 
 // Builds the voltage-independent tables of the screened recheck for shift
@@ -305,6 +449,10 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     launch_screen_tables(ts, tables_view(h), np, h->stream);
     HIP_TRY(hipGetLastError(), "screen tables");
     HIP_TRY(hipStreamSynchronize(h->stream), "screen tables");
+    {
+        int rc = build_gemm_tables(h, n);
+        if (rc != FBP_OK) return rc;
+    }
     h->screen_ok = true;
     *ok = true;
     return FBP_OK;
@@ -341,8 +489,10 @@ int fbp_destroy(fbp_handle* h)
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
-                      &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf};
+                      &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
+                      &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale};
     for (DevBuf* b : bufs) b->release();
+    if (h->rb) (void)rocblas_destroy_handle(h->rb);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -527,7 +677,13 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     }
     const uint32_t screen_batch =
         use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
+    const bool use_gemm = use_screen && h->gemm_ok && !(flags & FBP_FLAG_SCREEN_FP32);
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
+    if (use_gemm) {
+        HIP_TRY(h->Vh.ensure((size_t)screen_batch * C * L * sizeof(_Float16)), "alloc screen pack");
+        HIP_TRY(h->Vl.ensure((size_t)screen_batch * C * L * sizeof(_Float16)), "alloc screen pack");
+        HIP_TRY(h->vscale.ensure((size_t)screen_batch * 4), "alloc screen pack");
+    }
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -554,9 +710,19 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             if (nF > 0) {
                 tm.begin(TM_RECHECK);
                 if (use_screen) {
-                    const ScreenTables tb = tables_view(h);
-                    for (uint32_t f0 = 0; f0 < nF; f0 += screen_batch)
-                        launch_screen(ts, st, tb, f0, std::min(screen_batch, nF - f0), h->Bbuf.as<float>(), s);
+                    ScreenTables tb = tables_view(h);
+                    if (use_gemm) tb.gemm = 1;
+                    else tb.kappa2 = 0.0f;
+                    for (uint32_t f0 = 0; f0 < nF; f0 += screen_batch) {
+                        const uint32_t nb = std::min(screen_batch, nF - f0);
+                        if (use_gemm) {
+                            int rc = run_screen_gemm(h, st, f0, nb);
+                            if (rc != FBP_OK) return rc;
+                        } else {
+                            launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
+                        }
+                        launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
+                    }
                 } else {
                     HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
                     HIP_TRY(h->part_k.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");

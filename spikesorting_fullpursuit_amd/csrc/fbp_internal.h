@@ -61,7 +61,14 @@ struct ScreenTables {
     int S;
     int n;
     int cmax;             // most non-zero channels of any template
-    float kappa;          // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u))
+    float kappa;          // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u)) + kappa2 * (Q_l + Q_u)
+    float kappa2;         // 0 when B is the FP32 VALU correlation; > 0 for the split-fp16 GEMM
+    // split-fp16 GEMM layout of B (gemm != 0): row of (u, s) is brow[u] + s,
+    // column scale of window w is 1 / (vscale[w] * tscale)
+    int gemm;
+    const int* brow;      // [U]
+    const float* vscale;  // [batch]
+    float inv_tscale;
 };
 
 enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_N = 8 };

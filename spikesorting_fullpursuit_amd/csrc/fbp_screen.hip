@@ -289,7 +289,16 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             continue;
         }
         const float* Bg = Bbuf + (size_t)wi * US;
-        for (int e = threadIdx.x; e < US; e += blockDim.x) Bl[e] = Bg[e];
+        if (tb.gemm) {
+            // B(u, s) = column of the split-fp16 GEMM, unscaled, minus half the kept energy
+            const float sc = tb.inv_tscale / tb.vscale[wi];
+            for (int e = threadIdx.x; e < US; e += blockDim.x) {
+                const int u = e / S, s = e - u * S;
+                Bl[e] = fmaf(-0.5f, tb.Ntr[e], Bg[tb.brow[u] + s] * sc);
+            }
+        } else {
+            for (int e = threadIdx.x; e < US; e += blockDim.x) Bl[e] = Bg[e];
+        }
         const float* Bw = Bl;
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
             const int c = e / L, j = e - c * L;
@@ -306,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const bool in = af >= 0 && af < N - L;
             fin[f] = in;
             posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
-            A[f] = Bg[(size_t)lab * S + f];
+            A[f] = 0.0f;
         }
         // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs;
         // each wave keeps 4 loads in flight
@@ -333,6 +342,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             }
         }
         if (threadIdx.x == 0) ncand_s = 0;
+        __syncthreads();
+        for (int f = threadIdx.x; f < S; f += blockDim.x) A[f] = Bl[(size_t)lab * S + f];
         __syncthreads();
         for (int c = threadIdx.x; c < C; c += blockDim.x) {
             float m = 0.0f;
@@ -395,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         }
         __syncthreads();
         for (int u = threadIdx.x; u < U; u += blockDim.x)
-            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u]));
+            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])) + tb.kappa2 * (Q[lab] + Q[u]);
         float mA = -INFINITY;
         for (int f = threadIdx.x; f < S; f += blockDim.x)
             if (fin[f]) mA = fmaxf(mA, A[f]);
@@ -618,17 +629,101 @@ static void allow_lds(K kernel, size_t bytes)
                                   (int)bytes);
 }
 
-void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
-                   float* Bbuf, hipStream_t s)
+// ---------------------------------------------------------------------------
+// Split-fp16 GEMM form of B: B(u, s) = sum_{c, j} Tsh[(u, s)][(c, j)] * v[c][j]
+// with Tsh the window-truncated template of u shifted by s - n.  For a group
+// of units whose channels lie in [c_lo, c_lo + K/L), the rows (u, s) of Tsh
+// form a constant matrix A_g [M_g][K]; the flagged windows' voltage forms the
+// other operand [K][windows].  Both are split into fp16 hi + lo parts with
+// power-of-two scaling; three GEMMs (hi*hi + hi*lo + lo*hi) with fp32
+// accumulation give B to about 2^-22 relative plus the accumulation error,
+// which the select kernel's bound covers (kappa2).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gemm_A(TemplateSet ts, const int* __restrict__ units, int nunits, int c_lo,
+                                                   int K, int S, int n, float tscale, _Float16* __restrict__ Ah,
+                                                   _Float16* __restrict__ Al)
+{
+    const int L = ts.L, C = ts.C;
+    const int64_t total = (int64_t)nunits * S * K;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / K;
+        const int k = (int)(e - r * K);
+        const int u = units[r / S], s = (int)(r % S);
+        const int c = c_lo + k / L, j = k % L;
+        const int i = j - (s - n);
+        float v = 0.0f;
+        if (c < C && i >= 0 && i < L) v = ts.T[((size_t)u * C + c) * L + i] * tscale;
+        const _Float16 hi = (_Float16)v;
+        Ah[e] = hi;
+        Al[e] = (_Float16)(v - (float)hi);
+    }
+}
+
+// Window voltage of the flagged windows F[f0 + w] as split fp16 [w][C][L],
+// scaled by a power of two per window so that |v| * scale <= 2^14.
+__global__ __launch_bounds__(kBlock) void k_gemm_pack(TemplateSet ts, PassState st, uint32_t f0, uint32_t nbatch,
+                                                      _Float16* __restrict__ Vh, _Float16* __restrict__ Vl,
+                                                      float* __restrict__ vscale)
+{
+    __shared__ float red[kBlock / 64];
+    const int C = ts.C, L = ts.L;
+    const int64_t N = st.N;
+    const uint32_t nF = st.counters[CNT_F];
+    for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
+        const uint32_t w = f0 + wi;
+        _Float16* vh = Vh + (size_t)wi * C * L;
+        _Float16* vl = Vl + (size_t)wi * C * L;
+        const bool live = w < nF && !window_skipped(ts, st, st.F[w]);
+        const int64_t b = live ? (int64_t)st.best_idx[st.F[w]] : 0;
+        float m = 0.0f;
+        if (live)
+            for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+                const int c = e / L, j = e - c * L;
+                m = fmaxf(m, fabsf(st.V[(size_t)c * N + b + j]));
+            }
+        m = block_max(m, red);
+        float sc = 1.0f;
+        if (m > 0.0f) {
+            int ex;
+            frexpf(m, &ex);             // m < 2^ex
+            sc = ldexpf(1.0f, 14 - ex);  // m * sc < 2^14
+        }
+        for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+            float v = 0.0f;
+            if (live) {
+                const int c = e / L, j = e - c * L;
+                v = st.V[(size_t)c * N + b + j] * sc;
+            }
+            const _Float16 hi = (_Float16)v;
+            vh[e] = hi;
+            vl[e] = (_Float16)(v - (float)hi);
+        }
+        if (threadIdx.x == 0) vscale[wi] = sc;
+        __syncthreads();
+    }
+}
+
+void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int K, int S, int n, float tscale,
+                   _Float16* Ah, _Float16* Al, hipStream_t s)
+{
+    const int64_t total = (int64_t)nunits * S * K;
+    hipLaunchKernelGGL(k_gemm_A, dim3(sgrid(total, kBlock, 1u << 20)), dim3(kBlock), 0, s, ts, units, nunits, c_lo, K, S,
+                       n, tscale, Ah, Al);
+}
+
+void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
+                      _Float16* Vl, float* vscale, hipStream_t s)
 {
     if (nbatch == 0) return;
-    int rp;
-    const size_t lds1 = screen_corr_lds(ts, tb.S, &rp);
+    hipLaunchKernelGGL(k_gemm_pack, dim3(sgrid(nbatch, 1, 1u << 20)), dim3(kBlock), 0, s, ts, st, f0, nbatch, Vh, Vl,
+                       vscale);
+}
+
+void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                          uint32_t nbatch, const float* Bbuf, hipStream_t s)
+{
+    if (nbatch == 0) return;
     const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
-    if (ts.L == 48) hipLaunchKernelGGL(k_screen_corr<48>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     const size_t lds2 = screen_select_lds(ts, tb.S, tb.cmax);
     if (ts.L == 48) {
         allow_lds(k_screen_select<48>, lds2);
@@ -643,6 +738,19 @@ void launch_screen(const TemplateSet& ts, const PassState& st, const ScreenTable
         allow_lds(k_screen_select<0>, lds2);
         hipLaunchKernelGGL(k_screen_select<0>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
     }
+}
+
+void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
+                        float* Bbuf, hipStream_t s)
+{
+    if (nbatch == 0) return;
+    int rp;
+    const size_t lds1 = screen_corr_lds(ts, tb.S, &rp);
+    const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
+    if (ts.L == 48) hipLaunchKernelGGL(k_screen_corr<48>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
 }
 
 }  // namespace fbp

@@ -22,11 +22,20 @@ def handle():
     h.close()
 
 
-def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal=True):
+MODES = ["literal", "gemm", "valu"]
+
+
+def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="literal"):
+    """literal: "literal" (every overlap item), "gemm" (screen with split-fp16
+    GEMM correlations, the default path) or "valu" (screen with FP32 vector
+    correlations); booleans map to literal / gemm."""
     h.set_templates(T, bias, thr)
     flags = (_native.FLAG_OVERLAP_RECHECK if recheck else 0) | (_native.FLAG_ADJUSTED_CLIPS if clips else 0)
-    if literal:
+    mode = {True: "literal", False: "gemm"}.get(literal, literal)
+    if mode == "literal":
         flags |= _native.FLAG_RECHECK_LITERAL
+    elif mode == "valu":
+        flags |= _native.FLAG_SCREEN_FP32
     print(f"[gpu_run] C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n_max={n_max} "
           f"recheck={recheck}", flush=True)
     h.run(V, n_max, flags)
@@ -47,7 +56,7 @@ def assert_same(gpu, ora):
     np.testing.assert_array_equal(gR, oR)
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 @pytest.mark.parametrize("seed", range(10))
 def test_noisy_cases_match_oracle(handle, seed, literal):
     V, T, bias, thr, on, lab = cases.noisy_case(seed)
@@ -56,7 +65,7 @@ def test_noisy_cases_match_oracle(handle, seed, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 @pytest.mark.parametrize("seed", [0, 1, 5, 9])
 def test_correlated_pairs_match_oracle(handle, seed, literal):
     U, C, L = 4, 3, 16
@@ -68,7 +77,7 @@ def test_correlated_pairs_match_oracle(handle, seed, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
 
 
-@pytest.mark.parametrize("recheck,literal", [(True, True), (True, False), (False, False)])
+@pytest.mark.parametrize("recheck,literal", [(True, "literal"), (True, "gemm"), (True, "valu"), (False, "gemm")])
 def test_small_workload_3s_matches_oracle(handle, recheck, literal):
     d = synthetic.generate("small", duration_s=3.0)
     T, bias, thr = synthetic.kernel_inputs(d)
@@ -86,7 +95,7 @@ def test_adjusted_clips_match_oracle(handle):
     np.testing.assert_array_equal(g[3], fbp_oracle.adjusted_clips(oR, T, oi, ol))
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 @pytest.mark.parametrize("n_max", [0, 1, 3, 20])
 def test_shift_range_variants(handle, n_max, literal):
     V, T, bias, thr, on, lab = cases.noisy_case(20 + n_max, U=3, L=8)
@@ -94,7 +103,7 @@ def test_shift_range_variants(handle, n_max, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, n_max, return_residual=True))
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 def test_zero_channel_templates_and_odd_length(handle, literal):
     rng = np.random.default_rng(8)
     U, C, L, N = 4, 5, 11, 900
@@ -131,7 +140,7 @@ def test_empty_result(handle):
     assert len(g[0]) == 0
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 def test_probe16_slice_matches_oracle(handle, literal):
     d = synthetic.generate("probe16", duration_s=0.5)
     T, bias, thr = synthetic.kernel_inputs(d)
@@ -141,16 +150,17 @@ def test_probe16_slice_matches_oracle(handle, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
-def test_probe64_slice_screened_matches_oracle(handle):
+@pytest.mark.parametrize("literal", ["gemm", "valu"])
+def test_probe64_slice_screened_matches_oracle(handle, literal):
     d = synthetic.generate("probe64", duration_s=0.1)
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=False),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
-@pytest.mark.parametrize("literal", [True, False])
+@pytest.mark.parametrize("literal", MODES)
 def test_exact_ties_identical_templates(handle, literal):
     # two identical units: every overlap value ties between the labels, the
     # winner must be the lowest item id in both paths
@@ -190,7 +200,7 @@ def test_dropin_chunked_matches_oracle_per_chunk():
     m = dict(d["separability_metrics"])
     cap = 1 * 2**20
     ev, lab, _, _ = bpp.binary_pursuit(d["voltage"], np.float32, si, m, n_max_shift_inds=d["n_max_shift_inds"],
-                                       max_gpu_memory=cap, recheck_literal=True)
+                                       max_gpu_memory=cap)
     T, bias, thr = synthetic.kernel_inputs(d)
     V = d["voltage"]
     L = T.shape[2]
@@ -211,8 +221,7 @@ def test_dropin_adjusted_clips_dtype_and_values():
     d = synthetic.generate("small", duration_s=1.0)
     si = dict(d["sort_info"], get_adjusted_clips=True)
     m = dict(d["separability_metrics"])
-    ev, lab, _, clips = bpp.binary_pursuit(d["voltage"], np.float64, si, m, n_max_shift_inds=d["n_max_shift_inds"],
-                                           recheck_literal=True)
+    ev, lab, _, clips = bpp.binary_pursuit(d["voltage"], np.float64, si, m, n_max_shift_inds=d["n_max_shift_inds"])
     T, bias, thr = synthetic.kernel_inputs(d)
     i, l, R = fbp_oracle.pursuit(d["voltage"], T, bias, thr, d["n_max_shift_inds"], return_residual=True)
     assert clips.dtype == np.float64
@@ -224,10 +233,10 @@ def test_dropin_device_resident_voltage_matches_host():
     d = synthetic.generate("small", duration_s=1.0)
     m1, m2 = dict(d["separability_metrics"]), dict(d["separability_metrics"])
     ev1, lab1, _, _ = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m1,
-                                         n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+                                         n_max_shift_inds=d["n_max_shift_inds"])
     vt = torch.from_numpy(d["voltage"]).cuda()
     ev2, lab2, _, _ = bpp.binary_pursuit(vt, np.float32, d["sort_info"], m2,
-                                         n_max_shift_inds=d["n_max_shift_inds"], recheck_literal=True)
+                                         n_max_shift_inds=d["n_max_shift_inds"])
     np.testing.assert_array_equal(ev1, ev2)
     np.testing.assert_array_equal(lab1, lab2)
 

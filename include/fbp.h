@@ -59,7 +59,8 @@ typedef struct fbp_stats {
     double ms_other;
     int64_t n_likelihood_launches;
     int64_t n_recheck_launches;
-    int64_t recheck_items;        /* overlap items evaluated exactly */
+    int64_t recheck_items;        /* overlap items kept by the screen as candidates */
+    int64_t recheck_overflow;     /* windows whose candidates exceeded the on-chip list (slow path) */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

@@ -51,6 +51,7 @@ class FbpStats(ctypes.Structure):
         ("n_likelihood_launches", ctypes.c_int64),
         ("n_recheck_launches", ctypes.c_int64),
         ("recheck_items", ctypes.c_int64),
+        ("recheck_overflow", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -779,6 +779,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.ms_other = tm.total(TM_OTHER);
     HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
     stats.recheck_items = h->h_counters[CNT_EVAL];
+    stats.recheck_overflow = h->h_counters[CNT_CAND];
     stats.passes = passes;
     stats.spikes = total;
     h->stats = stats;

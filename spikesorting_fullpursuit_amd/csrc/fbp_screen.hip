@@ -179,12 +179,13 @@ __device__ __forceinline__ float block_max(float v, float* red)
     return m;
 }
 
-// Exact FP32 value of item (u, f, s) with both templates' non-zero channel rows
-// in LDS (rows of L floats, channels ascending).  Same operation order as
-// recheck_item_value: channels of either template ascending, then samples.
+// Exact FP32 value of item (u, f, s): rows of the window's label l in LDS
+// (stride L), rows of unit u in the compact template set (stride Lp).  Same
+// operation order as recheck_item_value: channels of either template
+// ascending, then samples.
 template <int LT>
-__device__ inline float exact_item_rows(const float* vW, int Lrt, const float* Tl, const int* cl, int nl, const float* Tu,
-                                        const int* cu, int nu, int f, int s, int n)
+__device__ inline float exact_item_rows(const float* vW, int Lrt, const float* Tl, const int* cl, int nl,
+                                        const float* Tu, int ldu, const int* cu, int nu, int f, int s, int n)
 {
     const int L = LT > 0 ? LT : Lrt;
     const int fa = n - f, sa = n - s;
@@ -197,7 +198,7 @@ __device__ inline float exact_item_rows(const float* vW, int Lrt, const float* T
         const int c = min(c1, c2);
         const bool hf = c1 == c, hs = c2 == c;
         const float* tf = Tl + (hf ? il : 0) * L;
-        const float* tsr = Tu + (hs ? iu : 0) * L;
+        const float* tsr = Tu + (size_t)(hs ? iu : 0) * ldu;
         il += hf;
         iu += hs;
         const float* vr = vW + c * L;
@@ -216,23 +217,22 @@ __device__ inline float exact_item_rows(const float* vW, int Lrt, const float* T
     return lik;
 }
 
-// Copies the non-zero rows of template u (L floats each) and their channels
-// into LDS with the lanes of one wave; returns the row count.
-__device__ __forceinline__ int stage_rows_wave(const TemplateSet& ts, int u, float* dst, int* dch)
-{
-    const int r0 = ts.off[u], nr = ts.nch[u], L = ts.L;
-    for (int e = lane_id(); e < nr * L; e += 64) {
-        const int k = e / L, j = e - k * L;
-        dst[e] = ts.rows[(size_t)(r0 + k) * ts.Lp + j];
-    }
-    for (int k = lane_id(); k < nr; k += 64) dch[k] = ts.chan[r0 + k];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    return nr;
-}
-
-constexpr int kCandCap = 512;
+constexpr int kCandCap = 1024;
 constexpr int kListChunk = 2048;  // (unit, shift) pairs examined per work-list round
+
+// B(u, s) of the current window: element e = u*S + s.
+struct BView {
+    const float* Bg;
+    const int* brow;
+    const float* Ntr;
+    float sc;
+    int S;
+    bool gemm;
+    __device__ __forceinline__ float operator()(int e, int u, int s) const
+    {
+        return gemm ? fmaf(-0.5f, Ntr[e], Bg[brow[u] + s] * sc) : Bg[e];
+    }
+};
 
 // Selection, exact rescoring and the parse cascade for one window per block.
 template <int LT>
@@ -242,13 +242,11 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     extern __shared__ float sm[];
     __shared__ float red[kBlock / 64];
     __shared__ uint32_t redk[kBlock / 64];
-    __shared__ float cand_v[kCandCap];
-    __shared__ uint32_t cand_k[kCandCap];
+    __shared__ float cand_e[kCandCap];    // E-hat of each candidate
+    __shared__ uint32_t cand_k[kCandCap]; // item id
     __shared__ uint32_t wl[kListChunk];
     __shared__ uint32_t nwl;
     __shared__ uint32_t ncand_s;
-    __shared__ float best1_v;
-    __shared__ uint32_t best1_k;
     const int C = ts.C, U = ts.U;
     const int L = LT > 0 ? LT : ts.L;
     const int S = tb.S, n = tb.n, CM = tb.cmax;
@@ -267,13 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     float* Q = SMp + S;                   // [U]
     float* D = Q + U;                     // [U] per-unit bound delta
     float* Tl = D + U;                    // [CM][L] rows of the window's label
-    float* Tw = Tl + CM * L;              // [4][CM][L] rows staged by each wave
-    int* cl = reinterpret_cast<int*>(Tw + 4 * CM * L);  // [CM]
-    int* cw = cl + CM;                    // [4][CM]
-    uint32_t* posu = reinterpret_cast<uint32_t*>(cw + 4 * CM);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
+    int* cl = reinterpret_cast<int*>(Tl + CM * L);  // [CM]
+    uint32_t* posu = reinterpret_cast<uint32_t*>(cl + CM);  // [nwords] bit (u*S+s): LIK[u][b+s-n] > 0
     uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
-    float* Bl = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(posl + S) + 15) & ~(uintptr_t)15);  // [U*S] B(u, s)
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
@@ -288,18 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
             continue;
         }
-        const float* Bg = Bbuf + (size_t)wi * US;
-        if (tb.gemm) {
-            // B(u, s) = column of the split-fp16 GEMM, unscaled, minus half the kept energy
-            const float sc = tb.inv_tscale / tb.vscale[wi];
-            for (int e = threadIdx.x; e < US; e += blockDim.x) {
-                const int u = e / S, s = e - u * S;
-                Bl[e] = fmaf(-0.5f, tb.Ntr[e], Bg[tb.brow[u] + s] * sc);
-            }
-        } else {
-            for (int e = threadIdx.x; e < US; e += blockDim.x) Bl[e] = Bg[e];
-        }
-        const float* Bw = Bl;
+        BView Bv{Bbuf + (size_t)wi * US, tb.brow, tb.Ntr, tb.gemm ? tb.inv_tscale / tb.vscale[wi] : 1.0f, S,
+                 tb.gemm != 0};
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
             const int c = e / L, j = e - c * L;
             vW[e] = st.V[(size_t)c * N + b + j];
@@ -315,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             const bool in = af >= 0 && af < N - L;
             fin[f] = in;
             posl[f] = in && st.LIK[(size_t)lab * N + af] > 0.0f;
-            A[f] = 0.0f;
+            A[f] = Bv((int)lab * S + f, (int)lab, f);
         }
         // sign bitmap of LIK[u][b+s-n], one ballot per 64 consecutive pairs;
         // each wave keeps 4 loads in flight
@@ -342,8 +327,6 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             }
         }
         if (threadIdx.x == 0) ncand_s = 0;
-        __syncthreads();
-        for (int f = threadIdx.x; f < S; f += blockDim.x) A[f] = Bl[(size_t)lab * S + f];
         __syncthreads();
         for (int c = threadIdx.x; c < C; c += blockDim.x) {
             float m = 0.0f;
@@ -420,7 +403,11 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int e = e0 + r * kBlock;
-                bv4[r] = e < US ? Bw[e] : 0.0f;
+                bv4[r] = 0.0f;
+                if (e < US) {
+                    const int u = e / S;
+                    bv4[r] = Bv(e, u, e - u * S);
+                }
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
@@ -437,38 +424,12 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             }
         }
         const float Mns = block_max(mns, red);
-        // phase 2: pairs sharing a channel, pruned by max_f A + B - min_f G;
-        // survivors go to a work list, each enumerated by one wave over f
+        // phase 2: (unit, shift) pairs whose bound reaches tp_u = max(M' - delta_u,
+        // bl - delta_u) -- pairs sharing a channel via max_f A + B - min_f G,
+        // the others exactly via the range maxima -- go to a work list; one
+        // wave per pair enumerates f, tracks M' and keeps items with
+        // E-hat >= tp_u as candidates.
         float msh = -INFINITY;
-        for (int e0 = 0; e0 < US; e0 += kListChunk) {
-            if (threadIdx.x == 0) nwl = 0;
-            __syncthreads();
-            for (int e = e0 + threadIdx.x; e < min(US, e0 + kListChunk); e += blockDim.x) {
-                const int u = e / S, s = e - u * S;
-                const int p = pidl[u];
-                if (p < 0) continue;
-                const int64_t as = b + s - n;
-                if (as < 0 || as >= N - L) continue;
-                const float tp = fmaxf(Mns - D[u], bl - D[u]);
-                if ((maxA + Bw[e]) - tb.minG[(size_t)p * S + s] < tp) continue;
-                wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
-            }
-            __syncthreads();
-            for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
-                const int e = (int)wl[i];
-                const int u = e / S, s = e - u * S;
-                const float Bv = Bw[e];
-                const float Du = D[u];
-                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-                const float* Gr = tb.G + ((size_t)pidl[u] * S + s) * S;
-                for (int f = lane; f < S; f += 64) {
-                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                    msh = fmaxf(msh, ((A[f] + Bv) - Gr[f]) - Du);
-                }
-            }
-        }
-        const float Mp = fmaxf(Mns, block_max(msh, red));
-        // phase 3: every item whose E-hat reaches theta_u = max(M' - delta_u, bl - delta_u)
         bool overflow = false;
         for (int e0 = 0; e0 < US; e0 += kListChunk) {
             if (threadIdx.x == 0) nwl = 0;
@@ -477,14 +438,14 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int u = e / S, s = e - u * S;
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
-                const float Bv = Bw[e];
                 const int p = pidl[u];
-                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const float Bu = Bv(e, u, s);
+                const float tp = fmaxf(Mns - D[u], bl - D[u]);
                 if (p < 0) {
                     const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
                     const float ra = pu ? RAall[s] : RApos[s];
-                    if (ra == -INFINITY || ra + Bv < theta) continue;
-                } else if ((maxA + Bv) - tb.minG[(size_t)p * S + s] < theta) {
+                    if (ra == -INFINITY || ra + Bu < tp) continue;
+                } else if ((maxA + Bu) - tb.minG[(size_t)p * S + s] < tp) {
                     continue;
                 }
                 wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
@@ -493,84 +454,64 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
                 const int e = (int)wl[i];
                 const int u = e / S, s = e - u * S;
-                const float Bv = Bw[e];
+                const float Bu = Bv(e, u, s);
+                const float Du = D[u];
+                const float tp = fmaxf(Mns - Du, bl - Du);
                 const int p = pidl[u];
                 const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-                const float theta = fmaxf(Mp - D[u], bl - D[u]);
                 const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
                 for (int f = lane; f < S; f += 64) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                    const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
-                    if (eh < theta) continue;
+                    const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
+                    msh = fmaxf(msh, eh - Du);
+                    if (eh < tp) continue;
                     const uint32_t slot = atomicAdd(&ncand_s, 1u);
                     if (slot < kCandCap) {
-                        cand_v[slot] = eh + D[u];  // upper bound of the exact value
+                        cand_e[slot] = eh;
                         cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
                     } else {
                         overflow = true;
                     }
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
+        const float Mp = fmaxf(Mns, block_max(msh, red));
         const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
         float bv = 0.0f;
         uint32_t bk = kNoItem;
         if (__syncthreads_or(overflow)) {
-            // more candidates than the list holds: rescore them where they are found
+            // more candidates than the list holds: enumerate and rescore in place
             for (int e = threadIdx.x; e < US; e += blockDim.x) {
                 const int u = e / S, s = e - u * S;
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
-                const float Bv = Bw[e];
+                const float Bu = Bv(e, u, s);
                 const int p = pidl[u];
                 const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
                 const float theta = fmaxf(Mp - D[u], bl - D[u]);
                 const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
                 for (int f = 0; f < S; f++) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                    const float eh = p >= 0 ? (A[f] + Bv) - Gr[f] : A[f] + Bv;
+                    const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
                     if (eh < theta) continue;
                     const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
                     argmax_merge(bv, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
                 }
             }
-        } else if (nc > 0) {
-            // round 1: the candidate with the largest upper bound, rescored exactly
-            float tv = -INFINITY;
-            uint32_t tk = kNoItem;
-            for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) argmax_merge(tv, tk, cand_v[i], cand_k[i]);
-            wave_argmax(tv, tk);
-            if (lane == 0) { red[wv] = tv; redk[wv] = tk; }
-            __syncthreads();
-            if (wv == 0) {
-                float v0 = red[0];
-                uint32_t k0 = redk[0];
-                for (int i = 1; i < kBlock / 64; i++) argmax_merge(v0, k0, red[i], redk[i]);
-                const int u = (int)(k0 % (uint32_t)U);
-                const int f = (int)((k0 / (uint32_t)U) % (uint32_t)S), s = (int)(k0 / ((uint32_t)U * S));
-                const int nu = stage_rows_wave(ts, u, Tw, cw);
-                if (lane == 0) {
-                    best1_v = exact_item_rows<LT>(vW, L, Tl, cl, nl, Tw, cw, nu, f, s, n);
-                    best1_k = k0;
-                }
-            }
-            __syncthreads();
-            bv = best1_v;
-            bk = best1_k;
-            // round 2: every other candidate whose upper bound reaches the exact value found
-            float* Tmy = Tw + wv * CM * L;
-            int* cmy = cw + wv * CM;
-            for (uint32_t i = wv; i < nc; i += kBlock / 64) {
+        } else {
+            // phase 3: candidates reaching theta_u = max(M' - delta_u, bl - delta_u),
+            // each rescored exactly by one thread
+            for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
                 const uint32_t k = cand_k[i];
-                if (k == best1_k || cand_v[i] < best1_v) continue;
                 const int u = (int)(k % (uint32_t)U);
                 const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
-                const int nu = stage_rows_wave(ts, u, Tmy, cmy);
-                if (lane == 0) argmax_merge(bv, bk, exact_item_rows<LT>(vW, L, Tl, cl, nl, Tmy, cmy, nu, f, s, n), k);
-                __builtin_amdgcn_wave_barrier();
+                if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                const int r0 = ts.off[u];
+                const float v = exact_item_rows<LT>(vW, L, Tl, cl, nl, ts.rows + (size_t)r0 * ts.Lp, ts.Lp,
+                                                    ts.chan + r0, ts.nch[u], f, s, n);
+                argmax_merge(bv, bk, v, k);
             }
-            if (lane != 0) { bv = 0.0f; bk = kNoItem; }
         }
         wave_argmax(bv, bk);
         __syncthreads();
@@ -582,6 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             for (int i = 1; i < kBlock / 64; i++) argmax_merge(v, k, red[i], redk[i]);
             apply_parse(ts, st, n, id, v, k);
             atomicAdd(st.counters + CNT_EVAL, ncand_s);
+            if (ncand_s > (uint32_t)kCandCap) atomicAdd(st.counters + CNT_CAND, 1u);
         }
     }
 }
@@ -616,8 +558,8 @@ size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
     const int nwords = (ts.U * S + 63) / 64 * 2;
-    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 2 * ts.U + 5 * cmax * ts.L + 5 * cmax) * 4 + (size_t)nwords * 4 +
-           2 * (size_t)S + 16 + (size_t)ts.U * S * 4 + 16;
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 2 * ts.U + cmax * ts.L + cmax) * 4 + (size_t)nwords * 4 +
+           2 * (size_t)S + 16;
 }
 
 // Dynamic LDS above 64 KiB needs the per-kernel attribute (gfx950 allows 160 KiB).

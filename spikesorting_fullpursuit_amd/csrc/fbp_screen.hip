@@ -218,6 +218,15 @@ __device__ inline float exact_item_rows(const float* vW, int Lrt, const float* T
 }
 
 constexpr int kCandCap = 1024;
+
+// e = q * S + r for 0 <= e < 2^24 with a float reciprocal and one correction.
+__device__ __forceinline__ void divmod_small(int e, int S, float invS, int& q, int& r)
+{
+    q = (int)((float)e * invS);
+    r = e - q * S;
+    if (r < 0) { q--; r += S; }
+    else if (r >= S) { q++; r -= S; }
+}
 constexpr int kListChunk = 2048;  // (unit, shift) pairs examined per work-list round
 
 // B(u, s) of the current window: element e = u*S + s.
@@ -270,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
     uint8_t* fin = reinterpret_cast<uint8_t*>(posu + nwords);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
     const int wv = threadIdx.x >> 6, lane = lane_id();
+    const float invS = 1.0f / (float)S;
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
         const uint32_t w = f0 + wi;
@@ -311,7 +321,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int e = base0 + r * kBlock + lane;
                 lv[r] = 0.0f;
                 if (e < US) {
-                    const int u = e / S, s = e - u * S;
+                    int u, s;
+                divmod_small(e, S, invS, u, s);
                     const int64_t as = b + s - n;
                     if (as >= 0 && as < N - L) lv[r] = st.LIK[(size_t)u * N + as];
                 }
@@ -405,15 +416,17 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int e = e0 + r * kBlock;
                 bv4[r] = 0.0f;
                 if (e < US) {
-                    const int u = e / S;
-                    bv4[r] = Bv(e, u, e - u * S);
+                    int u, sx;
+                    divmod_small(e, S, invS, u, sx);
+                    bv4[r] = Bv(e, u, sx);
                 }
             }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int e = e0 + r * kBlock;
                 if (e >= US) continue;
-                const int u = e / S, s = e - u * S;
+                int u, s;
+                divmod_small(e, S, invS, u, s);
                 if (pidl[u] >= 0) continue;
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
@@ -424,10 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             }
         }
         const float Mns = block_max(mns, red);
-        // phase 2: (unit, shift) pairs whose bound reaches tp_u = max(M' - delta_u,
-        // bl - delta_u) -- pairs sharing a channel via max_f A + B - min_f G,
-        // the others exactly via the range maxima -- go to a work list; one
-        // wave per pair enumerates f, tracks M' and keeps items with
+        // phase 2: pairs sharing a channel whose bound max_f A + B - min_f G
+        // reaches tp_u = max(M' - delta_u, bl - delta_u) go to a work list;
+        // one wave per pair enumerates f, tracks M' and keeps items with
         // E-hat >= tp_u as candidates.
         float msh = -INFINITY;
         bool overflow = false;
@@ -435,34 +447,29 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             if (threadIdx.x == 0) nwl = 0;
             __syncthreads();
             for (int e = e0 + threadIdx.x; e < min(US, e0 + kListChunk); e += blockDim.x) {
-                const int u = e / S, s = e - u * S;
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const int p = pidl[u];
+                if (p < 0) continue;
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
-                const int p = pidl[u];
-                const float Bu = Bv(e, u, s);
                 const float tp = fmaxf(Mns - D[u], bl - D[u]);
-                if (p < 0) {
-                    const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-                    const float ra = pu ? RAall[s] : RApos[s];
-                    if (ra == -INFINITY || ra + Bu < tp) continue;
-                } else if ((maxA + Bu) - tb.minG[(size_t)p * S + s] < tp) {
-                    continue;
-                }
+                if ((maxA + Bv(e, u, s)) - tb.minG[(size_t)p * S + s] < tp) continue;
                 wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
             }
             __syncthreads();
             for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
                 const int e = (int)wl[i];
-                const int u = e / S, s = e - u * S;
+                int u, s;
+                divmod_small(e, S, invS, u, s);
                 const float Bu = Bv(e, u, s);
                 const float Du = D[u];
                 const float tp = fmaxf(Mns - Du, bl - Du);
-                const int p = pidl[u];
                 const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                const float* Gr = tb.G + ((size_t)pidl[u] * S + s) * S;
                 for (int f = lane; f < S; f += 64) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
-                    const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
+                    const float eh = (A[f] + Bu) - Gr[f];
                     msh = fmaxf(msh, eh - Du);
                     if (eh < tp) continue;
                     const uint32_t slot = atomicAdd(&ncand_s, 1u);
@@ -477,13 +484,53 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             __syncthreads();
         }
         const float Mp = fmaxf(Mns, block_max(msh, red));
+        // phase 3: pairs sharing no channel whose exact best over f reaches
+        // theta_u = max(M' - delta_u, bl - delta_u) (few) add their items
+        for (int e0 = 0; e0 < US; e0 += kListChunk) {
+            if (threadIdx.x == 0) nwl = 0;
+            __syncthreads();
+            for (int e = e0 + threadIdx.x; e < min(US, e0 + kListChunk); e += blockDim.x) {
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                if (pidl[u] >= 0) continue;
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                const float ra = pu ? RAall[s] : RApos[s];
+                if (ra == -INFINITY || ra + Bv(e, u, s) < fmaxf(Mp - D[u], bl - D[u])) continue;
+                wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
+            }
+            __syncthreads();
+            for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
+                const int e = (int)wl[i];
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const float Bu = Bv(e, u, s);
+                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
+                for (int f = lane; f < S; f += 64) {
+                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                    const float eh = A[f] + Bu;
+                    if (eh < theta) continue;
+                    const uint32_t slot = atomicAdd(&ncand_s, 1u);
+                    if (slot < kCandCap) {
+                        cand_e[slot] = eh;
+                        cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
+            __syncthreads();
+        }
         const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
         float bv = 0.0f;
         uint32_t bk = kNoItem;
         if (__syncthreads_or(overflow)) {
             // more candidates than the list holds: enumerate and rescore in place
             for (int e = threadIdx.x; e < US; e += blockDim.x) {
-                const int u = e / S, s = e - u * S;
+                int u, s;
+                divmod_small(e, S, invS, u, s);
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
                 const float Bu = Bv(e, u, s);
@@ -500,8 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 }
             }
         } else {
-            // phase 3: candidates reaching theta_u = max(M' - delta_u, bl - delta_u),
-            // each rescored exactly by one thread
+            // candidates reaching theta_u, each rescored exactly by one thread
             for (uint32_t i = threadIdx.x; i < nc; i += blockDim.x) {
                 const uint32_t k = cand_k[i];
                 const int u = (int)(k % (uint32_t)U);

@@ -57,42 +57,54 @@ def main():
     log(f"generated {args.workload}: C={C} N={N} U={U} L={L} spikes={len(d['truth_onsets'])} "
         f"in {time.time() - t0:.1f}s")
 
-    def step(stats=None, literal=args.literal):
+    def step(precision, literal=args.literal):
         m = dict(d["separability_metrics"])
         return bpp.binary_pursuit(V, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"],
-                                  recheck_literal=literal, stats=stats)
+                                  recheck_literal=literal, precision=precision)
+
+    def timed(precision, k):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(k):
+            o = step(precision)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / k, o
 
     for i in range(args.warmup):
         t1 = time.time()
-        out = step()
+        out = step("mixed")
+        out32 = step("fp32")
         torch.cuda.synchronize()
         log(f"warmup {i}: {time.time() - t1:.3f}s spikes={len(out[0])}")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        out = step()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t1
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = N * C / (elapsed / args.steps) / 1e6
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, {value:.1f} Msamples*ch/s, spikes={len(out[0])}")
+    sec_mixed, out = timed("mixed", args.steps)
+    sec_fp32, out32 = timed("fp32", max(1, args.steps // 2))
+    identical = bool(np.array_equal(out[0], out32[0]) and np.array_equal(out[1], out32[1]))
+    ms_per_step = 1000.0 * sec_mixed
+    value = N * C / sec_mixed / 1e6
+    value32 = N * C / sec_fp32 / 1e6
+    log(f"mixed: {ms_per_step:.1f} ms/step, {value:.1f} Msamples*ch/s; fp32: {1000 * sec_fp32:.1f} ms/step, "
+        f"{value32:.1f}; identical spike set: {identical}; spikes={len(out[0])}")
 
-    # One extra (untimed) run with per-kernel HIP-event timing for the roofline.
+    # Untimed runs with per-kernel HIP-event timing for the roofline.
     from spikesorting_fullpursuit_amd import _native
     h = bpp._handle(0)
     T3, bias, thr = synthetic.kernel_inputs(d)
     h.set_templates(T3, bias, thr)
-    flags = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_TIMING | (_native.FLAG_RECHECK_LITERAL if args.literal else 0)
-    h.run(None, d["n_max_shift_inds"], flags, ld=V.stride(0), n_samples=N, device_ptr=V.data_ptr())
+    base = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_TIMING | (_native.FLAG_RECHECK_LITERAL if args.literal else 0)
+    h.run(None, d["n_max_shift_inds"], base, ld=V.stride(0), n_samples=N, device_ptr=V.data_ptr())
     st = h.stats()
-    log("kernel stats:", json.dumps(st))
-    classes = {"likelihood": st["ms_likelihood"], "recheck": st["ms_recheck"], "window_max": st["ms_window_max"],
-               "pursuit": st["ms_pursuit"], "residual": st["ms_residual"], "other": st["ms_other"]}
+    log("kernel stats fp32:", json.dumps(st))
+    h.run(None, d["n_max_shift_inds"], base | _native.FLAG_PRECISION_MIXED, ld=V.stride(0), n_samples=N,
+          device_ptr=V.data_ptr())
+    stm = h.stats()
+    log("kernel stats mixed:", json.dumps(stm))
+    classes = {"likelihood": stm["ms_likelihood"], "recheck": stm["ms_recheck"], "window_max": stm["ms_window_max"],
+               "pursuit": stm["ms_pursuit"], "residual": stm["ms_residual"], "other": stm["ms_other"]}
     dominant = max(classes, key=classes.get)
     lik_avg_ms = st["ms_likelihood"] / max(1, st["n_likelihood_launches"])
     lik_tflops = st["likelihood_flops"] / (st["ms_likelihood"] * 1e-3) / 1e12 if st["ms_likelihood"] > 0 else 0.0
     roofline = {
-        "kernel": "k_likelihood",
+        "kernel": "k_likelihood (FP32 path)",
         "bound": "mfma",
         "achieved": round(lik_tflops, 3),
         "peak": FP32_PEAK_TFLOPS,
@@ -101,9 +113,9 @@ def main():
         "traffic": None,
         "avg_launch_ms": round(lik_avg_ms, 4),
         "flops_per_launch": st["likelihood_flops"] / max(1, st["n_likelihood_launches"]),
-        "dominant_class": dominant,
-        "ms_by_class": {k: round(v, 3) for k, v in classes.items()},
-        "passes": st["passes"],
+        "dominant_class_mixed": dominant,
+        "ms_by_class_mixed": {k: round(v, 3) for k, v in classes.items()},
+        "passes": stm["passes"],
     }
 
     cpu = None
@@ -119,8 +131,9 @@ def main():
         cpu = {"value": round(ns * C / tc / 1e6, 6), "unit": "Msamples·ch/s", "cores": nthr, "kind": "port",
                "sample": f"first {args.cpu_sample_s} s of the same recording ({ns} samples x {C} ch), "
                          f"C restatement with OpenMP, {tc:.1f} s"}
-        h.run(None, d["n_max_shift_inds"], _native.FLAG_OVERLAP_RECHECK | (_native.FLAG_RECHECK_LITERAL if args.literal else 0),
-              ld=V.stride(0), n_samples=ns, device_ptr=V.data_ptr())
+        h.run(None, d["n_max_shift_inds"], _native.FLAG_OVERLAP_RECHECK | _native.FLAG_PRECISION_MIXED |
+              (_native.FLAG_RECHECK_LITERAL if args.literal else 0), ld=V.stride(0), n_samples=ns,
+              device_ptr=V.data_ptr())
         gi, gl = h.result()
         parity = bool(np.array_equal(gi, oi) and np.array_equal(gl, ol))
         log(f"cpu baseline {cpu['value']} Msamples*ch/s on {nthr} threads; slice parity {parity} ({len(oi)} spikes)")
@@ -136,7 +149,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f16 GEMM + f32 (decisions in exact f32)",
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: {C} ch, {w.fs / 1000:g} kHz, {N / w.fs:g} s, {U} units, L={L}, "
@@ -145,6 +158,11 @@ def main():
             "spikes_planted": int(len(d["truth_onsets"])),
             "recheck": "literal" if args.literal else "screened",
             "slice_parity_vs_oracle": parity,
+            "fp32_value": round(value32, 3),
+            "fp32_ms_per_step": round(1000 * sec_fp32, 3),
+            "mixed_over_fp32": round(value / value32, 3),
+            "identical_spike_set_mixed_vs_fp32": identical,
+            "likelihood_rescored_frac": round(stm["likelihood_rescored"] / max(1, stm["likelihood_positions"]), 5),
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -39,6 +39,7 @@ extern "C" {
 #define FBP_FLAG_RECHECK_LITERAL 4u  /* evaluate every overlap item literally (no screening) */
 #define FBP_FLAG_TIMING 8u           /* record per-kernel device time with HIP events */
 #define FBP_FLAG_SCREEN_FP32 16u     /* overlap screen correlations on FP32 vector units instead of split-fp16 GEMMs */
+#define FBP_FLAG_PRECISION_MIXED 32u /* likelihood by fp16 GEMM + exact FP32 rescoring near and above zero (same spikes) */
 
 typedef struct fbp_handle fbp_handle;
 
@@ -61,6 +62,7 @@ typedef struct fbp_stats {
     int64_t n_recheck_launches;
     int64_t recheck_items;        /* overlap items kept by the screen as candidates */
     int64_t recheck_overflow;     /* windows whose candidates exceeded the on-chip list (slow path) */
+    int64_t likelihood_rescored;  /* mixed precision: (position, template) pairs rescored in exact FP32 */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

@@ -23,6 +23,7 @@ FLAG_ADJUSTED_CLIPS = 2
 FLAG_RECHECK_LITERAL = 4
 FLAG_TIMING = 8
 FLAG_SCREEN_FP32 = 16
+FLAG_PRECISION_MIXED = 32
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (
@@ -52,6 +53,7 @@ class FbpStats(ctypes.Structure):
         ("n_recheck_launches", ctypes.c_int64),
         ("recheck_items", ctypes.c_int64),
         ("recheck_overflow", ctypes.c_int64),
+        ("likelihood_rescored", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -49,6 +49,14 @@ void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo
                    _Float16* Ah, _Float16* Al, hipStream_t s);
 void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
                       _Float16* Vl, float* vscale, hipStream_t s);
+void launch_likmix_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int nch, int P, float tscale,
+                     _Float16* Ah, hipStream_t s);
+void launch_likmix_pack(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, _Float16* seg,
+                        float* vscale, hipStream_t s);
+void launch_likmix_post(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, const float* Out,
+                        const int* orow, const float* vscale, const float* rowL1, float inv_tscale, float kappa_l,
+                        float kappa_b, hipStream_t s);
+size_t likmix_post_lds(const TemplateSet& ts, int P);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -134,10 +142,17 @@ struct fbp_handle {
         int units_off, nunits;
     };
     std::vector<Group> groups;
+    std::vector<int> h_gunits;
+    std::vector<size_t> lik_aoff;
+    std::vector<int> lik_row;
     bool gemm_ok = false;
     float tscale = 1.0f;
     rocblas_handle rb = nullptr;
     DevBuf Ah, Al, brow, gunits, Vh, Vl, vscale;
+    bool groups_ok = false;
+    // mixed-precision likelihood: Toeplitz template matrices per group
+    bool lik_ok = false;
+    DevBuf ALh, orow, seg, lvscale, lout;
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
@@ -287,14 +302,12 @@ static ScreenTables tables_view(const fbp_handle* h)
 }
 
 // Groups units whose non-zero channels fit in a window of at most 16 channels
-// (or one unit's own span when wider) and builds the split-fp16 matrices.
-static int build_gemm_tables(fbp_handle* h, int64_t n)
+// (or one unit's own span when wider); sets the template scale for fp16.
+static int build_groups(fbp_handle* h)
 {
-    h->gemm_ok = false;
+    if (h->groups_ok) return FBP_OK;
     const int U = h->U, C = h->C, L = h->L;
-    const int S = (int)(2 * n + 1);
     std::vector<int> first(U, 0), last(U, 0), order(U);
-    float tmax = 0.0f;
     for (int u = 0; u < U; u++) {
         int f = -1, l = -1;
         for (int c = 0; c < C; c++)
@@ -308,6 +321,7 @@ static int build_gemm_tables(fbp_handle* h, int64_t n)
     }
     std::vector<float> hT((size_t)U * C * L);
     HIP_TRY(hipMemcpy(hT.data(), h->T.p, hT.size() * 4, hipMemcpyDeviceToHost), "copy templates");
+    float tmax = 0.0f;
     for (float v : hT) tmax = std::max(tmax, std::fabs(v));
     int ex = 0;
     std::frexp(tmax > 0.0f ? tmax : 1.0f, &ex);
@@ -318,9 +332,6 @@ static int build_gemm_tables(fbp_handle* h, int64_t n)
     const int Wmax = 16;
     h->groups.clear();
     std::vector<int> gunits;
-    std::vector<int> brow(U, 0);
-    int row = 0;
-    size_t aoff = 0;
     size_t i = 0;
     while (i < order.size()) {
         const int lo = first[order[i]];
@@ -332,42 +343,132 @@ static int build_gemm_tables(fbp_handle* h, int64_t n)
         }
         fbp_handle::Group g;
         g.c_lo = lo;
-        g.K = (hi - lo + 1) * L;
+        g.K = (hi - lo + 1) * L;  // channel count * L; per use the sample extent differs
         g.nunits = (int)(j - i);
-        g.M = g.nunits * S;
-        g.row_off = row;
-        g.a_off = aoff;
+        g.M = 0;
+        g.row_off = 0;
+        g.a_off = 0;
         g.units_off = (int)gunits.size();
-        for (size_t k = i; k < j; k++) {
-            brow[order[k]] = row + (int)(k - i) * S;
-            gunits.push_back(order[k]);
-        }
-        row += g.M;
-        aoff += (size_t)g.M * g.K;
+        for (size_t k = i; k < j; k++) gunits.push_back(order[k]);
         h->groups.push_back(g);
         i = j;
     }
-    if (aoff * 2 * sizeof(_Float16) > kMaxGramBytes) return FBP_OK;
+    HIP_TRY(h->gunits.ensure(gunits.size() * 4), "alloc gemm rows");
+    HIP_TRY(hipMemcpy(h->gunits.p, gunits.data(), gunits.size() * 4, hipMemcpyHostToDevice), "copy");
+    h->h_gunits = gunits;
     if (!h->rb) {
         if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
             h->rb = nullptr;
-            return FBP_OK;
+            return set_err(h, FBP_E_HIP, "rocblas_create_handle failed");
         }
     }
-    if (rocblas_set_stream(h->rb, h->stream) != rocblas_status_success) return FBP_OK;
+    if (rocblas_set_stream(h->rb, h->stream) != rocblas_status_success)
+        return set_err(h, FBP_E_HIP, "rocblas_set_stream failed");
+    h->groups_ok = true;
+    return FBP_OK;
+}
+
+static int group_channels(const fbp_handle* h, const fbp_handle::Group& g) { return g.K / h->L; }
+
+// Split-fp16 matrices of the window-truncated shifted templates (screen B).
+static int build_gemm_tables(fbp_handle* h, int64_t n)
+{
+    h->gemm_ok = false;
+    if (build_groups(h) != FBP_OK) return FBP_OK;
+    const int U = h->U, L = h->L;
+    const int S = (int)(2 * n + 1);
+    std::vector<int> brow(U, 0);
+    int row = 0;
+    size_t aoff = 0;
+    for (auto& g : h->groups) {
+        g.M = g.nunits * S;
+        g.row_off = row;
+        g.a_off = aoff;
+        for (int k = 0; k < g.nunits; k++) brow[h->h_gunits[g.units_off + k]] = row + k * S;
+        row += g.M;
+        aoff += (size_t)g.M * g.K;
+    }
+    if (aoff * 2 * sizeof(_Float16) > kMaxGramBytes) return FBP_OK;
     HIP_TRY(h->Ah.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
     HIP_TRY(h->Al.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
     HIP_TRY(h->brow.ensure((size_t)U * 4), "alloc gemm rows");
-    HIP_TRY(h->gunits.ensure(gunits.size() * 4), "alloc gemm rows");
     HIP_TRY(hipMemcpy(h->brow.p, brow.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
-    HIP_TRY(hipMemcpy(h->gunits.p, gunits.data(), gunits.size() * 4, hipMemcpyHostToDevice), "copy");
     TemplateSet ts = template_view(h);
     for (const auto& g : h->groups)
         launch_gemm_A(ts, h->gunits.as<int>() + g.units_off, g.nunits, g.c_lo, g.K, S, (int)n, h->tscale,
                       h->Ah.as<_Float16>() + g.a_off, h->Al.as<_Float16>() + g.a_off, h->stream);
     HIP_TRY(hipGetLastError(), "gemm tables");
     HIP_TRY(hipStreamSynchronize(h->stream), "gemm tables");
+    (void)L;
     h->gemm_ok = true;
+    return FBP_OK;
+}
+
+// Toeplitz fp16 matrices of the mixed-precision likelihood: rows (t, x), x < L,
+// columns (c, j) over a segment of 2L-1 samples.
+static int build_lik_tables(fbp_handle* h)
+{
+    h->lik_ok = false;
+    int rc = build_groups(h);
+    if (rc != FBP_OK) return rc;
+    const int U = h->U, L = h->L, P = L, R = P + L - 1;
+    std::vector<int> orow(U, 0);
+    size_t aoff = 0;
+    int row = 0;
+    h->lik_aoff.clear();
+    h->lik_row.clear();
+    for (const auto& g : h->groups) {
+        h->lik_aoff.push_back(aoff);
+        h->lik_row.push_back(row);
+        for (int k = 0; k < g.nunits; k++) orow[h->h_gunits[g.units_off + k]] = row + k * P;
+        row += g.nunits * P;
+        aoff += (size_t)g.nunits * P * group_channels(h, g) * R;
+    }
+    if (aoff * sizeof(_Float16) > kMaxGramBytes) return set_err(h, FBP_E_UNSUPPORTED, "likelihood tables too large");
+    HIP_TRY(h->ALh.ensure(aoff * sizeof(_Float16)), "alloc likelihood tables");
+    HIP_TRY(h->orow.ensure((size_t)U * 4), "alloc likelihood rows");
+    HIP_TRY(hipMemcpy(h->orow.p, orow.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    TemplateSet ts = template_view(h);
+    for (size_t gi = 0; gi < h->groups.size(); gi++) {
+        const auto& g = h->groups[gi];
+        launch_likmix_A(ts, h->gunits.as<int>() + g.units_off, g.nunits, g.c_lo, group_channels(h, g), P, h->tscale,
+                        h->ALh.as<_Float16>() + h->lik_aoff[gi], h->stream);
+    }
+    HIP_TRY(hipGetLastError(), "likelihood tables");
+    HIP_TRY(hipStreamSynchronize(h->stream), "likelihood tables");
+    h->lik_ok = true;
+    return FBP_OK;
+}
+
+// Mixed-precision likelihood of the listed windows W[w0 .. w0+nb).
+static int run_lik_mixed(fbp_handle* h, const PassState& st, uint32_t w0, uint32_t nb)
+{
+    TemplateSet ts = template_view(h);
+    const int L = h->L, P = L, R = P + L - 1, U = h->U, C = h->C;
+    launch_likmix_pack(ts, st, w0, nb, P, h->seg.as<_Float16>(), h->lvscale.as<float>(), h->stream);
+    const float one = 1.0f, zero = 0.0f;
+    for (size_t gi = 0; gi < h->groups.size(); gi++) {
+        const auto& g = h->groups[gi];
+        const int M = g.nunits * P, K = group_channels(h, g) * R;
+        rocblas_status rs = rocblas_gemm_ex(
+            h->rb, rocblas_operation_transpose, rocblas_operation_none, M, (int)nb, K, &one,
+            h->ALh.as<_Float16>() + h->lik_aoff[gi], rocblas_datatype_f16_r, K,
+            h->seg.as<_Float16>() + (size_t)g.c_lo * R, rocblas_datatype_f16_r, C * R, &zero,
+            h->lout.as<float>() + h->lik_row[gi], rocblas_datatype_f32_r, U * P, h->lout.as<float>() + h->lik_row[gi],
+            rocblas_datatype_f32_r, U * P, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
+        if (rs != rocblas_status_success)
+            return set_err(h, FBP_E_HIP, std::string("rocblas_gemm_ex failed: ") + rocblas_status_to_string(rs));
+    }
+    // single fp16 pass: input rounding 2 * 2^-11 relative, fp32 accumulation of
+    // at most Cmax*L non-zero products (unit roundoff doubled), exact chain
+    // error on the same terms and on the bias; all doubled
+    const double u = std::ldexp(1.0, -24);
+    const double K1 = (double)h->Cmax * L + 2.0;
+    const double g2 = K1 * 2.0 * u / (1.0 - K1 * 2.0 * u);
+    const float kl = (float)(2.0 * (std::ldexp(1.0, -10) + std::ldexp(1.0, -21) + 2.0 * g2));
+    const float kb = (float)(2.0 * 2.0 * g2);
+    launch_likmix_post(ts, st, w0, nb, P, h->lout.as<float>(), h->orow.as<int>(), h->lvscale.as<float>(),
+                       h->rowL1.as<float>(), 1.0f / h->tscale, kl, kb, h->stream);
     return FBP_OK;
 }
 
@@ -437,14 +538,11 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * S * 4), "alloc gram table");
     HIP_TRY(h->minG.ensure((size_t)std::max<int64_t>(np, 1) * S * 4), "alloc gram minima");
     HIP_TRY(h->Ntr.ensure((size_t)U * S * 4), "alloc energies");
-    HIP_TRY(h->rowL1.ensure(std::max<size_t>(h->h_rowL1.size(), 1) * 4), "alloc norms");
-    HIP_TRY(h->SS.ensure((size_t)U * 4), "alloc norms");
+
     HIP_TRY(hipMemcpy(h->pid.p, pid.data(), pid.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->pair_l.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->pair_u.p, pu.data(), pu.size() * 4, hipMemcpyHostToDevice), "copy");
-    if (!h->h_rowL1.empty())
-        HIP_TRY(hipMemcpy(h->rowL1.p, h->h_rowL1.data(), h->h_rowL1.size() * 4, hipMemcpyHostToDevice), "copy");
-    HIP_TRY(hipMemcpy(h->SS.p, h->h_SS.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+
     h->npairs = np;
     launch_screen_tables(ts, tables_view(h), np, h->stream);
     HIP_TRY(hipGetLastError(), "screen tables");
@@ -490,7 +588,8 @@ int fbp_destroy(fbp_handle* h)
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
-                      &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale};
+                      &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale,
+                      &h->ALh, &h->orow, &h->seg, &h->lvscale, &h->lout};
     for (DevBuf* b : bufs) b->release();
     if (h->rb) (void)rocblas_destroy_handle(h->rb);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
@@ -583,7 +682,13 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
         }
         h->h_SS[u] = (float)(ss * (1.0 + 1e-6));
     }
+    HIP_TRY(h->rowL1.ensure(std::max<size_t>(h->h_rowL1.size(), 1) * 4), "alloc norms");
+    HIP_TRY(h->SS.ensure((size_t)U * 4), "alloc norms");
+    HIP_TRY(hipMemcpy(h->rowL1.p, h->h_rowL1.data(), h->h_rowL1.size() * 4, hipMemcpyHostToDevice), "copy norms");
+    HIP_TRY(hipMemcpy(h->SS.p, h->h_SS.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy norms");
     h->tab_n = -1;
+    h->groups_ok = false;
+    h->lik_ok = false;
     h->have_templates = true;
     h->have_result = false;
     return FBP_OK;
@@ -636,6 +741,18 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(grow_keep(h->sp_idx, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
     HIP_TRY(grow_keep(h->sp_lab, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
 
+    const bool mixed = (flags & FBP_FLAG_PRECISION_MIXED) != 0;
+    const uint32_t lik_batch = 65536;
+    if (mixed) {
+        if (!h->lik_ok) {
+            int rc = build_lik_tables(h);
+            if (rc != FBP_OK) return rc;
+        }
+        const int R = 2 * (int)L - 1;
+        HIP_TRY(h->seg.ensure((size_t)lik_batch * C * R * sizeof(_Float16)), "alloc likelihood segments");
+        HIP_TRY(h->lvscale.ensure((size_t)lik_batch * 4), "alloc likelihood segments");
+        HIP_TRY(h->lout.ensure((size_t)lik_batch * U * L * 4), "alloc likelihood output");
+    }
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0), "event");
@@ -692,7 +809,14 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         stats.likelihood_positions += (int64_t)nW * L * U;
         stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
         tm.begin(TM_LIK);
-        launch_likelihood(ts, st, nW, s);
+        if (mixed) {
+            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
+                int rc = run_lik_mixed(h, st, w0, std::min(lik_batch, nW - w0));
+                if (rc != FBP_OK) return rc;
+            }
+        } else {
+            launch_likelihood(ts, st, nW, s);
+        }
         tm.end(TM_LIK);
         stats.n_likelihood_launches++;
         tm.begin(TM_MAX);
@@ -780,6 +904,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
     stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.recheck_overflow = h->h_counters[CNT_CAND];
+    stats.likelihood_rescored = h->h_counters[CNT_LIKX];
     stats.passes = passes;
     stats.spikes = total;
     h->stats = stats;

@@ -79,7 +79,7 @@ def _is_device_tensor(x):
 
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
                    kernels_path=None, max_gpu_memory=None, *, device=0, return_residual=False,
-                   recheck_literal=False, stats=None):
+                   recheck_literal=False, precision="mixed", stats=None):
     """Binary pursuit over every chunk of `voltage` ([C][N], any real dtype;
     a float32 torch tensor already on the GPU is used in place).
 
@@ -88,8 +88,14 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     empty lists when nothing is found.  `kernels_path` is accepted and ignored
     (there is no OpenCL source).  Keyword-only extras: `return_residual` appends
     the list of per-chunk residual voltages; `stats` (a list) receives the
-    device statistics of every chunk.
+    device statistics of every chunk; `precision` "mixed" (default) computes the
+    likelihood with an fp16 GEMM and rescores in exact FP32 every value that
+    can influence a decision, "fp32" computes every value in FP32 -- both give
+    the same spikes and residual; `recheck_literal` evaluates every overlap
+    item instead of screening them.
     """
+    if precision not in ("mixed", "fp32"):
+        raise ValueError("precision must be 'mixed' or 'fp32'")
     n_chans = int(voltage.shape[0])
     n_samples = int(voltage.shape[1])
     sampling_rate = sort_info["sampling_rate"]
@@ -128,6 +134,8 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
         flags |= _native.FLAG_ADJUSTED_CLIPS
     if recheck_literal:
         flags |= _native.FLAG_RECHECK_LITERAL
+    if precision == "mixed":
+        flags |= _native.FLAG_PRECISION_MIXED
 
     on_device = _is_device_tensor(voltage)
     if on_device:

@@ -25,7 +25,7 @@ def handle():
 MODES = ["literal", "gemm", "valu"]
 
 
-def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="literal"):
+def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="literal", mixed=False):
     """literal: "literal" (every overlap item), "gemm" (screen with split-fp16
     GEMM correlations, the default path) or "valu" (screen with FP32 vector
     correlations); booleans map to literal / gemm."""
@@ -36,6 +36,8 @@ def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="liter
         flags |= _native.FLAG_RECHECK_LITERAL
     elif mode == "valu":
         flags |= _native.FLAG_SCREEN_FP32
+    if mixed:
+        flags |= _native.FLAG_PRECISION_MIXED
     print(f"[gpu_run] C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n_max={n_max} "
           f"recheck={recheck}", flush=True)
     h.run(V, n_max, flags)
@@ -85,6 +87,32 @@ def test_small_workload_3s_matches_oracle(handle, recheck, literal):
     V = d["voltage"]
     assert_same(gpu_run(handle, V, T, bias, thr, n, recheck=recheck, literal=literal),
                 fbp_oracle.pursuit(V, T, bias, thr, n, do_recheck=recheck, return_residual=True))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_mixed_precision_noisy_cases_match_oracle(handle, seed):
+    V, T, bias, thr, on, lab = cases.noisy_case(seed)
+    n = T.shape[2] - 1
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+@pytest.mark.parametrize("name,dur", [("small", 3.0), ("probe16", 0.5), ("probe64", 0.1)])
+def test_mixed_precision_workloads_match_oracle(handle, name, dur):
+    d = synthetic.generate(name, duration_s=dur)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = d["voltage"]
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+def test_mixed_precision_rescores_few_positions(handle):
+    d = synthetic.generate("probe64", duration_s=0.5)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    gpu_run(handle, d["voltage"], T, bias, thr, d["n_max_shift_inds"], literal="gemm", mixed=True)
+    st = handle.stats()
+    assert 0 < st["likelihood_rescored"] < 0.2 * st["likelihood_positions"]
 
 
 def test_adjusted_clips_match_oracle(handle):
@@ -226,6 +254,18 @@ def test_dropin_adjusted_clips_dtype_and_values():
     i, l, R = fbp_oracle.pursuit(d["voltage"], T, bias, thr, d["n_max_shift_inds"], return_residual=True)
     assert clips.dtype == np.float64
     np.testing.assert_array_equal(clips, fbp_oracle.adjusted_clips(R, T, i, l).astype(np.float64))
+
+
+def test_dropin_mixed_precision_identical_to_fp32():
+    d = synthetic.generate("probe16", duration_s=1.0)
+    m1, m2 = dict(d["separability_metrics"]), dict(d["separability_metrics"])
+    a = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m1, n_max_shift_inds=d["n_max_shift_inds"],
+                           precision="fp32", return_residual=True)
+    b = bpp.binary_pursuit(d["voltage"], np.float32, d["sort_info"], m2, n_max_shift_inds=d["n_max_shift_inds"],
+                           precision="mixed", return_residual=True)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[4][0], b[4][0])
 
 
 def test_dropin_device_resident_voltage_matches_host():

@@ -63,6 +63,8 @@ typedef struct fbp_stats {
     int64_t recheck_items;        /* overlap items kept by the screen as candidates */
     int64_t recheck_overflow;     /* windows whose candidates exceeded the on-chip list (slow path) */
     int64_t likelihood_rescored;  /* mixed precision: (position, template) pairs rescored in exact FP32 */
+    int64_t screen_shared_pairs;  /* (unit, shift) pairs sharing a channel enumerated over f by the screen */
+    int64_t screen_separable_pairs; /* (unit, shift) pairs sharing no channel enumerated over f */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

@@ -54,6 +54,8 @@ class FbpStats(ctypes.Structure):
         ("recheck_items", ctypes.c_int64),
         ("recheck_overflow", ctypes.c_int64),
         ("likelihood_rescored", ctypes.c_int64),
+        ("screen_shared_pairs", ctypes.c_int64),
+        ("screen_separable_pairs", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -905,6 +905,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.recheck_overflow = h->h_counters[CNT_CAND];
     stats.likelihood_rescored = h->h_counters[CNT_LIKX];
+    stats.screen_shared_pairs = h->h_counters[CNT_SURV];
+    stats.screen_separable_pairs = h->h_counters[CNT_NSP];
     stats.passes = passes;
     stats.spikes = total;
     h->stats = stats;

@@ -71,6 +71,6 @@ struct ScreenTables {
     float inv_tscale;
 };
 
-enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_N = 8 };
+enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7, CNT_N = 8 };
 
 }  // namespace fbp

@@ -458,6 +458,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
             }
             __syncthreads();
+            if (threadIdx.x == 0 && nwl) atomicAdd(st.counters + CNT_SURV, nwl);
             for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
                 const int e = (int)wl[i];
                 int u, s;
@@ -501,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 wl[atomicAdd(&nwl, 1u)] = (uint32_t)e;
             }
             __syncthreads();
+            if (threadIdx.x == 0 && nwl) atomicAdd(st.counters + CNT_NSP, nwl);
             for (uint32_t i = wv; i < nwl; i += kBlock / 64) {
                 const int e = (int)wl[i];
                 int u, s;

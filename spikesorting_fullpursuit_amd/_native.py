@@ -24,6 +24,7 @@ FLAG_RECHECK_LITERAL = 4
 FLAG_TIMING = 8
 FLAG_SCREEN_FP32 = 16
 FLAG_PRECISION_MIXED = 32
+FLAG_SELECT_STREAMING = 64
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (

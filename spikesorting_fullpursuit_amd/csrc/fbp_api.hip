@@ -44,7 +44,7 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
                         float* Bbuf, hipStream_t s);
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
-                          uint32_t nbatch, const float* Bbuf, hipStream_t s);
+                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s);
 void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int K, int S, int n, float tscale,
                    _Float16* Ah, _Float16* Al, hipStream_t s);
 void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
@@ -132,9 +132,10 @@ struct fbp_handle {
     int Cmax = 0;
     // screened-recheck tables (valid for tab_n)
     int64_t tab_n = -1;
+    int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, Ntr, rowL1, SS, Bbuf;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, rowL1, SS, Bbuf;
     // split-fp16 GEMM form of the screen correlations
     struct Group {
         int c_lo, K, M, row_off;
@@ -277,6 +278,9 @@ static ScreenTables tables_view(const fbp_handle* h)
     tb.pair_u = h->pair_u.as<int>();
     tb.G = h->G.as<float>();
     tb.minG = h->minG.as<float>();
+    tb.minGb = h->minGb.as<float>();
+    tb.nbg = (int)((2 * h->tab_n + 1 + kGB - 1) / kGB);
+    tb.npl_max = h->npl_max;
     tb.Ntr = h->Ntr.as<float>();
     tb.rowL1 = h->rowL1.as<float>();
     tb.SS = h->SS.as<float>();
@@ -530,6 +534,12 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
         }
     const int64_t np = (int64_t)pl.size();
     if ((size_t)np * S * S * 4 > kMaxGramBytes) return FBP_OK;
+    h->npl_max = 0;
+    for (int l = 0; l < U; l++) {
+        int c = 0;
+        for (int u = 0; u < U; u++) c += pid[(size_t)l * U + u] >= 0;
+        h->npl_max = std::max(h->npl_max, c);
+    }
     if (pl.empty()) { pl.push_back(0); pu.push_back(0); }
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     HIP_TRY(h->pid.ensure(pid.size() * 4), "alloc pid");
@@ -537,6 +547,7 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     HIP_TRY(h->pair_u.ensure(pu.size() * 4), "alloc pairs");
     HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * S * 4), "alloc gram table");
     HIP_TRY(h->minG.ensure((size_t)std::max<int64_t>(np, 1) * S * 4), "alloc gram minima");
+    HIP_TRY(h->minGb.ensure((size_t)std::max<int64_t>(np, 1) * S * ((S + kGB - 1) / kGB) * 4), "alloc gram minima");
     HIP_TRY(h->Ntr.ensure((size_t)U * S * 4), "alloc energies");
 
     HIP_TRY(hipMemcpy(h->pid.p, pid.data(), pid.size() * 4, hipMemcpyHostToDevice), "copy");
@@ -587,7 +598,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
-                      &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
+                      &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale,
                       &h->ALh, &h->orow, &h->seg, &h->lvscale, &h->lout};
     for (DevBuf* b : bufs) b->release();
@@ -845,7 +856,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                         } else {
                             launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
                         }
-                        launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
+                        launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(),
+                                             (flags & FBP_FLAG_SELECT_STREAMING) != 0, s);
                     }
                 } else {
                     HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");

@@ -8,6 +8,7 @@ namespace fbp {
 constexpr int kBlock = 256;          // threads per block for the streaming kernels
 constexpr int kCompactTile = 1024;   // elements per block in ordered compaction
 constexpr uint32_t kNoItem = 0xffffffffu;
+constexpr int kGB = 8;                 // shifts f per block of the blocked Gram minima
 
 // Device view of the template set.  Rows of the compact layout are the
 // non-zero channels of each template, ascending, zero padded to Lp floats.
@@ -55,6 +56,9 @@ struct ScreenTables {
     const int* pair_u;    // [npairs]
     float* G;             // [npairs][S][S]  G[p][s][f]: window-truncated cross product
     float* minG;          // [npairs][S]     min over f
+    float* minGb;         // [npairs][S][nbg] min over each block of kGB consecutive f
+    int nbg;              // blocks of f: ceil(S / kGB)
+    int npl_max;          // most units sharing a channel with one unit
     float* Ntr;           // [U][S]          energy of the window-truncated template
     const float* rowL1;   // [R] L1 norm of each compact template row (rounded up)
     const float* SS;      // [U] sum of squares of each template (rounded up)

@@ -15,6 +15,8 @@
 // winner (value, then lowest item id) is therefore the literal kernel's.
 #include <math.h>
 
+#include <algorithm>
+
 #include "fbp_device.h"
 
 namespace fbp {
@@ -55,8 +57,13 @@ __global__ __launch_bounds__(kBlock) void k_tab_min_ntr(TemplateSet ts, ScreenTa
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         if (e < total_min) {
             const float* row = tb.G + e * S;
-            float m = row[0];
-            for (int f = 1; f < S; f++) m = fminf(m, row[f]);
+            float m = INFINITY;
+            for (int g = 0; g < tb.nbg; g++) {
+                float mb = INFINITY;
+                for (int f = g * kGB; f < min(S, (g + 1) * kGB); f++) mb = fminf(mb, row[f]);
+                tb.minGb[e * tb.nbg + g] = mb;
+                m = fminf(m, mb);
+            }
             tb.minG[e] = m;
         } else {
             const int64_t r = e - total_min;
@@ -184,7 +191,7 @@ __device__ __forceinline__ float block_max(float v, float* red)
 // operation order as recheck_item_value: channels of either template
 // ascending, then samples.
 template <int LT>
-__device__ inline float exact_item_rows(const float* vW, int Lrt, const float* Tl, const int* cl, int nl,
+__device__ __forceinline__ float exact_item_rows(const float* vW, int Lrt, const float* Tl, const int* cl, int nl,
                                         const float* Tu, int ldu, const int* cu, int nu, int f, int s, int n)
 {
     const int L = LT > 0 ? LT : Lrt;
@@ -203,14 +210,28 @@ __device__ inline float exact_item_rows(const float* vW, int Lrt, const float* T
         iu += hs;
         const float* vr = vW + c * L;
         float ss = 0.0f;
-#pragma unroll 8
-        for (int j = 0; j < L; j++) {
-            const int jf = fa + j, js = sa + j;
-            const float fv = (hf && j >= ffj && jf < L) ? tf[jf] : 0.0f;
-            const float sv = (hs && j >= sfj && js < L) ? tsr[js] : 0.0f;
-            const float sum = fv + sv;
-            ss = fmaf(sum, sum, ss);
-            lik = fmaf(sum, vr[j], lik);
+        // 8 samples at a time: all loads (from clamped, valid addresses) are
+        // issued before the two FMA chains consume them
+        for (int j0 = 0; j0 < L; j0 += 8) {
+            float a[8], c2[8], vv[8];
+            bool okf[8], oks[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = j0 + q, jf = fa + j, js = sa + j;
+                okf[q] = hf && j >= ffj && jf < L;
+                oks[q] = hs && j >= sfj && js < L;
+                a[q] = tf[okf[q] ? jf : 0];
+                c2[q] = tsr[oks[q] ? js : 0];
+                vv[q] = vr[j < L ? j : 0];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the 24 loads ahead of the chains
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (LT == 0 && j0 + q >= L) break;
+                const float sum = (okf[q] ? a[q] : 0.0f) + (oks[q] ? c2[q] : 0.0f);
+                ss = fmaf(sum, sum, ss);
+                lik = fmaf(sum, vv[q], lik);
+            }
         }
         if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
     }
@@ -577,6 +598,522 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 }
 
 // ---------------------------------------------------------------------------
+// Register-resident selection: the same bounds, candidates and rescoring as
+// k_screen_select, organised so that every (unit, shift) pair of the window is
+// read once.  Thread t of a block of bd threads owns the pairs e = t + bd * i,
+// i < kSelR; it keeps B(u, s) (-inf when b+s-n is out of bounds) and two bit
+// masks (LIK[u][b+s-n] > 0; u shares a channel with the label) in registers
+// through the three phases.  Each wave enumerates the shifts f of its own
+// surviving pairs one pair at a time, so no phase needs a block-wide list.
+// ---------------------------------------------------------------------------
+constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
+static_assert(kSelR <= 32, "pair masks are 32-bit");
+constexpr int kSelMaxBlock = 1024;  // threads per block at most (U * S <= kSelR * kSelMaxBlock)
+constexpr size_t kSelMaxLds = 96 * 1024;  // dynamic LDS of the register-resident selection at most
+constexpr int kStageRows = 128;     // template rows of candidate units staged in LDS for rescoring
+
+__device__ __forceinline__ float block_max_n(float v, float* red, int nw)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+    __syncthreads();
+    if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    float m = red[0];
+    for (int i = 1; i < nw; i++) m = fmaxf(m, red[i]);
+    return m;
+}
+
+__device__ __forceinline__ int wave_sum_u32(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Advance (u, s) by bd pairs: e -> e + bd with bd = du * S + ds, ds < S.
+__device__ __forceinline__ void pair_step(int& u, int& s, int du, int ds, int S)
+{
+    s += ds;
+    u += du;
+    if (s >= S) { s -= S; u++; }
+}
+
+template <int LT>
+__global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet ts, PassState st, ScreenTables tb,
+                                                                    uint32_t f0, uint32_t nbatch,
+                                                                    const float* __restrict__ Bbuf)
+{
+    extern __shared__ float sm[];
+    __shared__ float red[kSelMaxBlock / 64];
+    __shared__ uint32_t redk[kSelMaxBlock / 64];
+    __shared__ float cand_e[kCandCap];
+    __shared__ uint32_t cand_k[kCandCap];
+    __shared__ uint32_t ncand_s;
+    __shared__ int plo_sh, phi_sh;  // first / last f with LIK[l][b+f-n] > 0 (S, -1: none)
+    __shared__ int pfirst_sh, npl_sh;  // pair ids of the label: pfirst .. pfirst + npl - 1
+    __shared__ uint32_t nsurv_s, nsurvp_s, nrow_s;
+    __shared__ float Mp_sh;  // max over the enumerated shared-channel items of E-hat - delta_u
+    const int C = ts.C, U = ts.U;
+    const int L = LT > 0 ? LT : ts.L;
+    const int S = tb.S, n = tb.n, CM = tb.cmax;
+    const int US = U * S;
+    const int64_t N = st.N;
+    const int bd = (int)blockDim.x, nw = bd >> 6;
+    float* vW = sm;                       // [C][L]
+    float* vmax = vW + C * L;             // [C]
+    float* A = vmax + C;                  // [S]
+    float* RAall = A + S;                 // [S]
+    float* RApos = RAall + S;             // [S]
+    float* PMa = RApos + S;               // [S] prefix / suffix maxima of A
+    float* PMp = PMa + S;
+    float* SMa = PMp + S;
+    float* SMp = SMa + S;
+    float* Q = SMp + S;                   // [U]
+    float* D = Q + U;                     // [U] per-unit bound delta
+    float* Tl = D + U;                    // [CM][L] rows of the window's label
+    int* cl = reinterpret_cast<int*>(Tl + CM * L);  // [CM]
+    int* pidw = cl + CM;                  // [U] pair id of (label, u); -1: no shared channel
+    int* browl = pidw + U;                // [U] GEMM row of (u, s = 0)
+    float* gmB = reinterpret_cast<float*>(browl + U);  // [npl][S] least G over the blocks of f with LIK[l] > 0
+    float* stg = gmB + tb.npl_max * S;    // [kStageRows][L] candidate units' rows; survivor list before that
+    int* stg_ch = reinterpret_cast<int*>(stg + kStageRows * L);  // [kStageRows] channel of each staged row
+    int* slot_of = stg_ch + kStageRows;   // [U] first staged row of unit u (-1: not staged)
+    uint32_t* surv_e = reinterpret_cast<uint32_t*>(stg);  // [survcap] pair of each listed survivor
+    float* surv_b = stg + kStageRows * L / 2;             // [survcap] its B
+    const uint32_t survcap = (uint32_t)(kStageRows * L / 2);
+    uint8_t* fin = reinterpret_cast<uint8_t*>(slot_of + U);  // [S] b+f-n in [0, N-L)
+    uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
+    const int wv = threadIdx.x >> 6, lane = lane_id();
+    const int wbase = threadIdx.x & ~63;
+    const float invS = 1.0f / (float)S;
+    const int du = bd / S, ds = bd - du * S;
+    int u_t, s_t;  // pair of i = 0
+    divmod_small((int)threadIdx.x, S, invS, u_t, s_t);
+    const uint32_t nF = st.counters[CNT_F];
+    // one window per block (the grid covers the batch)
+    {
+        const uint32_t wi = blockIdx.x;
+        const uint32_t w = f0 + wi;
+        if (wi >= nbatch || w >= nF) return;
+        const uint32_t id = st.F[w];
+        const uint32_t lab = st.best_lab[id];
+        const int64_t b = st.best_idx[id];
+        const float bl = st.best_lik[id];
+        __syncthreads();
+        if (window_skipped(ts, st, id)) {
+            if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
+            return;
+        }
+        const float* Bw = Bbuf + (size_t)wi * US;
+        const float sc = tb.gemm ? tb.inv_tscale / tb.vscale[wi] : 1.0f;
+        const bool gemm = tb.gemm != 0;
+        auto bload = [&](int e, int u, int s) -> float {
+            return gemm ? fmaf(-0.5f, tb.Ntr[e], Bw[browl[u] + s] * sc) : Bw[e];
+        };
+        for (int e = threadIdx.x; e < C * L; e += bd) {
+            const int c = e / L, j = e - c * L;
+            vW[e] = st.V[(size_t)c * N + b + j];
+        }
+        const int nl = ts.nch[lab];
+        for (int e = threadIdx.x; e < nl * L; e += bd) {
+            const int k = e / L, j = e - k * L;
+            Tl[e] = ts.rows[(size_t)(ts.off[lab] + k) * ts.Lp + j];
+        }
+        for (int k = threadIdx.x; k < nl; k += bd) cl[k] = ts.chan[ts.off[lab] + k];
+        if (threadIdx.x == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += bd) {
+            const int p = tb.pid[(size_t)lab * U + u];
+            pidw[u] = p;
+            browl[u] = gemm ? tb.brow[u] : 0;
+            if (p >= 0) { atomicMin(&pfirst_sh, p); atomicAdd(&npl_sh, 1); }
+        }
+        __syncthreads();
+        for (int f = threadIdx.x; f < S; f += bd) {
+            const int64_t af = b + f - n;
+            const bool in = af >= 0 && af < N - L;
+            const bool pl = in && st.LIK[(size_t)lab * N + af] > 0.0f;
+            fin[f] = in;
+            posl[f] = pl;
+            if (pl) { atomicMin(&plo_sh, f); atomicMax(&phi_sh, f); }
+            A[f] = bload((int)lab * S + f, (int)lab, f);
+        }
+        // per-pair state: B, sign of LIK[u][b+s-n], shared-channel bit.  Loads
+        // are issued 8 pairs at a time from clamped (always valid) addresses.
+        float bv[kSelR];
+        uint32_t pum = 0, shm = 0;
+        __syncthreads();
+        {
+            int u = u_t, s = s_t, t0 = (int)threadIdx.x;
+            asm volatile("" : "+v"(u), "+v"(s), "+v"(t0));  // recompute the pair walk, do not keep it live
+#pragma unroll
+            for (int i0 = 0; i0 < kSelR; i0 += 8) {
+                float gb[8], gn[8], lv[8];
+                int pw[8], bi[8];
+                bool ok[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    const int e = t0 + bd * (i0 + j);
+                    const int64_t as = b + s - n;
+                    ok[j] = e < US && as >= 0 && as < N - L;
+                    const int uc = ok[j] ? u : 0, scl = ok[j] ? s : 0, ec = ok[j] ? e : 0;
+                    lv[j] = st.LIK[(size_t)uc * N + (ok[j] ? as : 0)];
+                    gn[j] = tb.Ntr[ec];
+                    bi[j] = gemm ? browl[uc] + scl : ec;
+                    pw[j] = pidw[uc];
+                    pair_step(u, s, du, ds, S);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < 8; j++) gb[j] = Bw[bi[j]];
+                __builtin_amdgcn_sched_barrier(0);  // issue the 8 pairs' loads together
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    bv[i0 + j] = ok[j] ? (gemm ? fmaf(-0.5f, gn[j], gb[j] * sc) : gb[j]) : -INFINITY;
+                    if (ok[j] && lv[j] > 0.0f) pum |= 1u << (i0 + j);
+                    if (ok[j] && pw[j] >= 0) shm |= 1u << (i0 + j);
+                }
+            }
+        }
+        // least G of every pair of the label over the blocks of f that hold
+        // the f with LIK[l][b+f-n] > 0 (the whole row when they span > 3 blocks)
+        {
+            const int plo = plo_sh, phi = phi_sh, pf = pfirst_sh, np = npl_sh;
+            if (phi >= 0) {
+                const int j1 = plo / kGB, j3 = phi / kGB, j2 = min(j1 + 1, j3);
+                const bool wide = j3 - j1 > 2;
+#pragma unroll 4
+                for (int k = threadIdx.x; k < np * S; k += bd) {
+                    const size_t row = (size_t)pf * S + k;
+                    if (wide) {
+                        gmB[k] = tb.minG[row];
+                    } else {
+                        const float* g = tb.minGb + row * tb.nbg;
+                        gmB[k] = fminf(fminf(g[j1], g[j2]), g[j3]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < C; c += bd) {
+            float m = 0.0f;
+            for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * L + j]));
+            vmax[c] = m;
+        }
+        // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
+        if (n <= L) {
+            if (wv < 2) {
+                const bool suffix = wv == 1;
+                float* Xa = suffix ? SMa : PMa;
+                float* Xp = suffix ? SMp : PMp;
+                float ca = -INFINITY, cp = -INFINITY;
+                for (int c0 = 0; c0 < S; c0 += 64) {
+                    const int f = suffix ? S - 1 - (c0 + lane) : c0 + lane;
+                    const bool ok = c0 + lane < S;
+                    float va = -INFINITY, vp = -INFINITY;
+                    if (ok) {
+                        if (fin[f]) va = A[f];
+                        if (posl[f]) vp = A[f];
+                    }
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const float ta = __shfl_up(va, off, 64), tq = __shfl_up(vp, off, 64);
+                        if (lane >= off) { va = fmaxf(va, ta); vp = fmaxf(vp, tq); }
+                    }
+                    va = fmaxf(va, ca);
+                    vp = fmaxf(vp, cp);
+                    if (ok) { Xa[f] = va; Xp[f] = vp; }
+                    ca = __shfl(va, 63, 64);
+                    cp = __shfl(vp, 63, 64);
+                }
+            }
+            __syncthreads();
+            for (int s = threadIdx.x; s < S; s += bd) {
+                const int fl = max(0, s - L), fh = min(S - 1, s + L);
+                RAall[s] = (fl == 0) ? PMa[fh] : SMa[fl];
+                RApos[s] = (fl == 0) ? PMp[fh] : SMp[fl];
+            }
+        } else {
+            for (int s = threadIdx.x; s < S; s += bd) {
+                float ma = -INFINITY, mp = -INFINITY;
+                const int fl = max(0, s - L), fh = min(S - 1, s + L);
+                for (int f = fl; f <= fh; f++) {
+                    if (fin[f]) ma = fmaxf(ma, A[f]);
+                    if (posl[f]) mp = fmaxf(mp, A[f]);
+                }
+                RAall[s] = ma;
+                RApos[s] = mp;
+            }
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += bd) {
+            float q = 0.0f;
+            const int r0 = ts.off[u], nr = ts.nch[u];
+            for (int k = 0; k < nr; k++) q += tb.rowL1[r0 + k] * vmax[ts.chan[r0 + k]];
+            Q[u] = q;
+        }
+        __syncthreads();
+        for (int u = threadIdx.x; u < U; u += bd)
+            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])) + tb.kappa2 * (Q[lab] + Q[u]);
+        __syncthreads();
+        // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable;
+        // M' = max(E-hat - delta_u) is a lower bound of the exact maximum
+        float mns = -INFINITY;
+        {
+            int u = u_t, s = s_t;
+            asm volatile("" : "+v"(u), "+v"(s));  // recompute the pair walk, do not keep it live
+#pragma unroll
+            for (int i = 0; i < kSelR; i++) {
+                if (bv[i] != -INFINITY && !((shm >> i) & 1u)) {
+                    const float ra = ((pum >> i) & 1u) ? RAall[s] : RApos[s];
+                    if (ra != -INFINITY) mns = fmaxf(mns, (ra + bv[i]) - D[u]);
+                }
+                pair_step(u, s, du, ds, S);
+            }
+        }
+        const float Mns = block_max_n(mns, red, nw);
+        // phase 2: pairs sharing a channel whose bound reaches
+        // tp_u = max(M' - delta_u, bl - delta_u); the wave enumerates f of
+        // each, tracks M' and keeps items with E-hat >= tp_u.  Bound: the
+        // largest A over the allowed f within L of s (any in-bound f when
+        // LIK[u][b+s-n] > 0, else those with LIK[l][b+f-n] > 0) plus B minus
+        // the least G over the blocks of f that can hold an allowed f.
+        uint32_t sv = 0;
+        {
+            const int pf = pfirst_sh;
+            int u = u_t, s = s_t;
+            asm volatile("" : "+v"(u), "+v"(s));  // recompute the pair walk, do not keep it live
+#pragma unroll
+            for (int i = 0; i < kSelR; i++) {
+                if ((shm >> i) & 1u) {
+                    const bool pu = (pum >> i) & 1u;
+                    const float ra = pu ? RAall[s] : RApos[s];
+                    if (ra != -INFINITY) {
+                        const int k = (pidw[u] - pf) * S + s;
+                        const float gmin = pu ? tb.minG[(size_t)pf * S + k] : gmB[k];
+                        const float tp = fmaxf(Mns - D[u], bl - D[u]);
+                        if (!((ra + bv[i]) - gmin < tp)) sv |= 1u << i;
+                    }
+                }
+                pair_step(u, s, du, ds, S);
+            }
+        }
+        float msh = -INFINITY;
+        bool overflow = false;
+        auto push = [&](float eh, int u, int f, int s) {
+            const uint32_t slot = atomicAdd(&ncand_s, 1u);
+            if (slot < kCandCap) {
+                cand_e[slot] = eh;
+                cand_k[slot] = (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s);
+            } else {
+                overflow = true;
+            }
+        };
+        const int plo = plo_sh, phi = phi_sh;
+        {
+            const int c = wave_sum_u32(__popc(sv));
+            if (lane == 0 && c) atomicAdd(st.counters + CNT_SURV, (uint32_t)c);
+        }
+        // Surviving pairs go to two lists with their B: pairs with
+        // LIK[u][b+s-n] <= 0 allow only the f with LIK[l][b+f-n] > 0, all in
+        // [plo, phi] (one thread per pair); pairs with LIK[u][b+s-n] > 0 allow
+        // every in-bound f within L of s (one thread per (pair, f)).
+        // Phase 2 (shared channel, G term, threshold tp_u, tracks M') and
+        // phase 3 (no shared channel, threshold theta_u) use the same code.
+        const uint32_t capN = survcap * 3 / 4, capP = survcap - capN;
+#pragma unroll 1
+        for (int ph = 2; ph <= 3; ph++) {
+            const bool shared = ph == 2;
+            const float Mref = shared ? Mns : fmaxf(Mns, Mp_sh);
+            auto thr_u = [&](int u) { return fmaxf(Mref - D[u], bl - D[u]); };
+            auto gterm = [&](int u, int s, int f) -> float {
+                return shared ? tb.G[((size_t)pidw[u] * S + s) * S + f] : 0.0f;
+            };
+            auto eval_pl = [&](int e, float Bu) {
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const float Du = D[u];
+                const float tp = thr_u(u);
+                const int fa = max(plo, s - L), fb = min(phi, s + L);
+                for (int f0 = fa; f0 <= fb; f0 += 4) {
+                    float g[4], a[4];
+                    bool pl[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int f = min(f0 + q, fb);
+                        g[q] = gterm(u, s, f);
+                        a[q] = A[f];
+                        pl[q] = posl[f] != 0;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (f0 + q > fb || !pl[q]) continue;
+                        const float eh = shared ? (a[q] + Bu) - g[q] : a[q] + Bu;
+                        if (shared) msh = fmaxf(msh, eh - Du);
+                        if (!(eh < tp)) push(eh, u, f0 + q, s);
+                    }
+                }
+            };
+            auto eval_all = [&](int e, float Bu, int f) {
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const float g = gterm(u, s, f);
+                if (!fin[f] || abs(f - s) > L) return;
+                const float eh = shared ? (A[f] + Bu) - g : A[f] + Bu;
+                if (shared) msh = fmaxf(msh, eh - D[u]);
+                if (!(eh < thr_u(u))) push(eh, u, f, s);
+            };
+            if (!shared) {
+                // pairs sharing no channel whose exact best over f reaches theta_u
+                int u = u_t, s = s_t;
+                uint32_t pm = pum, sm2 = shm;
+                // recompute the pair walk and masks here, do not keep them live
+                asm volatile("" : "+v"(u), "+v"(s), "+v"(pm), "+v"(sm2));
+#pragma unroll
+                for (int i = 0; i < kSelR; i++) {
+                    if (bv[i] != -INFINITY && !((sm2 >> i) & 1u)) {
+                        const float ra = ((pm >> i) & 1u) ? RAall[s] : RApos[s];
+                        if (!(ra == -INFINITY || ra + bv[i] < thr_u(u))) sv |= 1u << i;
+                    }
+                    pair_step(u, s, du, ds, S);
+                }
+                const int c = wave_sum_u32(__popc(sv));
+                if (lane == 0 && c) atomicAdd(st.counters + CNT_NSP, (uint32_t)c);
+            }
+            if (threadIdx.x == 0) { nsurv_s = 0; nsurvp_s = 0; }
+            __syncthreads();
+            uint32_t spill = 0;
+            uint32_t pm = pum;
+            asm volatile("" : "+v"(pm));  // do not keep per-pair selections live across the phases
+#pragma unroll
+            for (int i = 0; i < kSelR; i++) {
+                if ((sv >> i) & 1u) {
+                    const bool pu = (pm >> i) & 1u;
+                    const uint32_t slot = atomicAdd(pu ? &nsurvp_s : &nsurv_s, 1u);
+                    if (slot < (pu ? capP : capN)) {
+                        const uint32_t o = pu ? capN + slot : slot;
+                        surv_e[o] = threadIdx.x + (uint32_t)bd * i;
+                        surv_b[o] = bv[i];
+                    } else {
+                        spill |= 1u << i;
+                    }
+                }
+            }
+            sv = 0;
+            __syncthreads();
+            const uint32_t nN = min(nsurv_s, capN), nP = min(nsurvp_s, capP);
+            for (uint32_t j = threadIdx.x; j < nN; j += bd) eval_pl((int)surv_e[j], surv_b[j]);
+            for (uint32_t j = threadIdx.x; j < nP * (uint32_t)S; j += bd) {
+                int pi, f;
+                divmod_small((int)j, S, invS, pi, f);
+                eval_all((int)surv_e[capN + pi], surv_b[capN + pi], f);
+            }
+            for (uint32_t m = spill; m; m &= m - 1) {
+                const int i = __builtin_ctz(m);
+                const int e = (int)threadIdx.x + bd * i;
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const float Bu = bload(e, u, s);
+                if ((pum >> i) & 1u) {
+                    for (int f = 0; f < S; f++) eval_all(e, Bu, f);
+                } else {
+                    eval_pl(e, Bu);
+                }
+            }
+            if (shared) {
+                const float m2 = block_max_n(msh, red, nw);
+                if (threadIdx.x == 0) Mp_sh = m2;
+                __syncthreads();
+            }
+        }
+        const float Mp = fmaxf(Mns, Mp_sh);
+        __syncthreads();
+        const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
+        float bv_best = 0.0f;
+        uint32_t bk = kNoItem;
+        if (__syncthreads_or(overflow)) {
+            // more candidates than the list holds: enumerate and rescore in place
+            for (int e = threadIdx.x; e < US; e += bd) {
+                int u, s;
+                divmod_small(e, S, invS, u, s);
+                const int64_t as = b + s - n;
+                if (as < 0 || as >= N - L) continue;
+                const float Bu = bload(e, u, s);
+                const int p = pidw[u];
+                const bool pu = st.LIK[(size_t)u * N + as] > 0.0f;
+                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                for (int f = 0; f < S; f++) {
+                    if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
+                    const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
+                    if (eh < theta) continue;
+                    const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
+                    argmax_merge(bv_best, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
+                }
+            }
+        } else {
+            // candidates reaching theta_u, each rescored exactly by one thread;
+            // the rows of their units are staged in LDS first (each unit once,
+            // while kStageRows rows last)
+            for (int u = threadIdx.x; u < U; u += bd) slot_of[u] = -1;
+            if (threadIdx.x == 0) nrow_s = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < nc; i += bd) {
+                const int u = (int)(cand_k[i] % (uint32_t)U);
+                if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                if (atomicCAS(&slot_of[u], -1, -2) == -1) {
+                    const int nr = ts.nch[u];
+                    const int r = (int)atomicAdd(&nrow_s, (uint32_t)nr);
+                    slot_of[u] = (r + nr <= kStageRows) ? r : -3;
+                }
+            }
+            __syncthreads();
+            for (int u = wv; u < U; u += nw) {
+                const int sl = slot_of[u];
+                if (sl < 0) continue;
+                const int r0 = ts.off[u], nr = ts.nch[u];
+                for (int e = lane; e < nr * L; e += 64) {
+                    const int k = e / L, j = e - k * L;
+                    stg[(sl + k) * L + j] = ts.rows[(size_t)(r0 + k) * ts.Lp + j];
+                }
+                for (int k = lane; k < nr; k += 64) stg_ch[sl + k] = ts.chan[r0 + k];
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < nc; i += bd) {
+                const uint32_t k = cand_k[i];
+                const int u = (int)(k % (uint32_t)U);
+                const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
+                if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                const int sl = slot_of[u];
+                float v;
+                if (sl >= 0) {
+                    v = exact_item_rows<LT>(vW, L, Tl, cl, nl, stg + sl * L, L, stg_ch + sl, ts.nch[u], f, s, n);
+                } else {
+                    const int r0 = ts.off[u];
+                    v = exact_item_rows<LT>(vW, L, Tl, cl, nl, ts.rows + (size_t)r0 * ts.Lp, ts.Lp, ts.chan + r0,
+                                            ts.nch[u], f, s, n);
+                }
+                argmax_merge(bv_best, bk, v, k);
+            }
+        }
+        wave_argmax(bv_best, bk);
+        __syncthreads();
+        if (lane == 0) { red[wv] = bv_best; redk[wv] = bk; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float v = red[0];
+            uint32_t k = redk[0];
+            for (int i = 1; i < nw; i++) argmax_merge(v, k, red[i], redk[i]);
+            apply_parse(ts, st, n, id, v, k);
+            atomicAdd(st.counters + CNT_EVAL, ncand_s);
+            if (ncand_s > (uint32_t)kCandCap) atomicAdd(st.counters + CNT_CAND, 1u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline unsigned sgrid(int64_t n, int64_t per, unsigned cap)
@@ -607,6 +1144,13 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
 {
     const int nwords = (ts.U * S + 63) / 64 * 2;
     return (size_t)(ts.C * ts.L + ts.C + 7 * S + 2 * ts.U + cmax * ts.L + cmax) * 4 + (size_t)nwords * 4 +
+           2 * (size_t)S + 16;
+}
+
+size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max)
+{
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 5 * ts.U + cmax * ts.L + cmax + (size_t)npl_max * S +
+                    (size_t)kStageRows * (ts.L + 1)) * 4 +
            2 * (size_t)S + 16;
 }
 
@@ -710,9 +1254,31 @@ void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, u
 }
 
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
-                          uint32_t nbatch, const float* Bbuf, hipStream_t s)
+                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s)
 {
     if (nbatch == 0) return;
+    const int64_t US = (int64_t)ts.U * tb.S;
+    if (!streaming && US <= (int64_t)kSelR * kSelMaxBlock &&
+        screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max) <= kSelMaxLds) {
+        const int64_t need = (US + kSelR - 1) / kSelR;
+        const int bd = (int)std::max<int64_t>(kBlock, (need + 63) / 64 * 64);
+        const dim3 g(nbatch), b(bd);
+        const size_t lds = screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max);
+        if (ts.L == 48) {
+            allow_lds(k_screen_select_reg<48>, lds);
+            hipLaunchKernelGGL(k_screen_select_reg<48>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
+        } else if (ts.L == 64) {
+            allow_lds(k_screen_select_reg<64>, lds);
+            hipLaunchKernelGGL(k_screen_select_reg<64>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
+        } else if (ts.L == 32) {
+            allow_lds(k_screen_select_reg<32>, lds);
+            hipLaunchKernelGGL(k_screen_select_reg<32>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
+        } else {
+            allow_lds(k_screen_select_reg<0>, lds);
+            hipLaunchKernelGGL(k_screen_select_reg<0>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
+        }
+        return;
+    }
     const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
     const size_t lds2 = screen_select_lds(ts, tb.S, tb.cmax);
     if (ts.L == 48) {

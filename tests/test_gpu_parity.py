@@ -22,13 +22,14 @@ def handle():
     h.close()
 
 
-MODES = ["literal", "gemm", "valu"]
+MODES = ["literal", "gemm", "valu", "stream"]
 
 
 def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="literal", mixed=False):
     """literal: "literal" (every overlap item), "gemm" (screen with split-fp16
-    GEMM correlations, the default path) or "valu" (screen with FP32 vector
-    correlations); booleans map to literal / gemm."""
+    GEMM correlations, the default path), "valu" (screen with FP32 vector
+    correlations) or "stream" (GEMM screen with the streaming selection
+    kernel, the path for U*(2n+1) > 24576); booleans map to literal / gemm."""
     h.set_templates(T, bias, thr)
     flags = (_native.FLAG_OVERLAP_RECHECK if recheck else 0) | (_native.FLAG_ADJUSTED_CLIPS if clips else 0)
     mode = {True: "literal", False: "gemm"}.get(literal, literal)
@@ -36,6 +37,8 @@ def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="liter
         flags |= _native.FLAG_RECHECK_LITERAL
     elif mode == "valu":
         flags |= _native.FLAG_SCREEN_FP32
+    elif mode == "stream":
+        flags |= _native.FLAG_SELECT_STREAMING
     if mixed:
         flags |= _native.FLAG_PRECISION_MIXED
     print(f"[gpu_run] C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n_max={n_max} "

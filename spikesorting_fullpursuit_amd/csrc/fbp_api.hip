@@ -21,7 +21,8 @@ namespace fbp {
 void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
 void launch_fill_u8(uint8_t* p, uint8_t v, int64_t n, hipStream_t s);
 void launch_iota(uint32_t* p, int64_t n, hipStream_t s);
-void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
+void launch_to_sample_major(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
+void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N, hipStream_t s);
 bool likelihood_fits(const TemplateSet& ts);
 void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
@@ -49,14 +50,24 @@ void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo
                    _Float16* Ah, _Float16* Al, hipStream_t s);
 void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
                       _Float16* Vl, float* vscale, hipStream_t s);
-void launch_likmix_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int nch, int P, float tscale,
-                     _Float16* Ah, hipStream_t s);
-void launch_likmix_pack(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, _Float16* seg,
-                        float* vscale, hipStream_t s);
-void launch_likmix_post(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, const float* Out,
-                        const int* orow, const float* vscale, const float* rowL1, float inv_tscale, float kappa_l,
-                        float kappa_b, hipStream_t s);
-size_t likmix_post_lds(const TemplateSet& ts, int P);
+// MFMA likelihood tables (fbp_likmfma.hip).
+constexpr int kMfMaxBlk = 32;
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+struct MfTables {
+    const half8* frag;
+    const int* units;
+    const int* nkb;
+    const int* nblk;
+    const int* aoff;
+    const int* chb;
+    const float* T1;
+    int ntiles;
+    float inv_tscale;
+    float kl, kb, ka;
+};
+void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s);
+void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
+                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -144,19 +155,21 @@ struct fbp_handle {
     };
     std::vector<Group> groups;
     std::vector<int> h_gunits;
-    std::vector<size_t> lik_aoff;
-    std::vector<int> lik_row;
     bool gemm_ok = false;
     float tscale = 1.0f;
     rocblas_handle rb = nullptr;
     DevBuf Ah, Al, brow, gunits, Vh, Vl, vscale;
     bool groups_ok = false;
     // mixed-precision likelihood: Toeplitz template matrices per group
-    bool lik_ok = false;
-    DevBuf ALh, orow, seg, lvscale, lout;
+    bool mf_built = false;
+    // MFMA likelihood: A fragments of 16-template tiles and the rescoring list
+    DevBuf mf_frag, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mask, mf_mbits, mf_vs;
+    int mf_ntiles = 0;
+    float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
+    bool mf_ok = false;
     // chunk state
     int64_t N = 0, nwin = 0;
-    DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
+    DevBuf V, Vtmp, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
     DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf;
     DevBuf part_v, part_k;
     DevBuf sp_idx, sp_lab, clips;
@@ -199,6 +212,7 @@ static TemplateSet template_view(const fbp_handle* h)
     ts.C = h->C;
     ts.L = h->L;
     ts.Lp = h->Lp;
+    ts.R = (int)h->h_rowL1.size();
     return ts;
 }
 
@@ -408,72 +422,125 @@ static int build_gemm_tables(fbp_handle* h, int64_t n)
     return FBP_OK;
 }
 
-// Toeplitz fp16 matrices of the mixed-precision likelihood: rows (t, x), x < L,
-// columns (c, j) over a segment of 2L-1 samples.
-static int build_lik_tables(fbp_handle* h)
+// Tables of the MFMA likelihood: templates ordered by their first non-zero
+// channel, cut into tiles of 16; each tile covers the contiguous 8-channel
+// blocks from its first to its last non-zero channel; the A operand in
+// fragment order (tile, k-block of 32, lane, 8 values), k = (sample j, block
+// b, channel in block) with run 4 * kb + lane / 16 = j * blocks + b.  mf_ok stays false (FP32 likelihood) when
+// L > 64 or the probe has more than 256 channels.
+static int build_mf_tables(fbp_handle* h)
 {
-    h->lik_ok = false;
-    int rc = build_groups(h);
-    if (rc != FBP_OK) return rc;
-    const int U = h->U, L = h->L, P = L, R = P + L - 1;
-    std::vector<int> orow(U, 0);
-    size_t aoff = 0;
-    int row = 0;
-    h->lik_aoff.clear();
-    h->lik_row.clear();
-    for (const auto& g : h->groups) {
-        h->lik_aoff.push_back(aoff);
-        h->lik_row.push_back(row);
-        for (int k = 0; k < g.nunits; k++) orow[h->h_gunits[g.units_off + k]] = row + k * P;
-        row += g.nunits * P;
-        aoff += (size_t)g.nunits * P * group_channels(h, g) * R;
+    h->mf_built = true;
+    h->mf_ok = false;
+    const int U = h->U, C = h->C, L = h->L;
+    if (L > 64 || (C + 7) / 8 > kMfMaxBlk) return FBP_OK;
+    std::vector<float> hT((size_t)U * C * L);
+    HIP_TRY(hipMemcpy(hT.data(), h->T.p, hT.size() * 4, hipMemcpyDeviceToHost), "copy templates");
+    std::vector<int> first(U, C), order(U);
+    for (int u = 0; u < U; u++) {
+        for (int c = 0; c < C; c++)
+            if (h->h_nz[(size_t)u * C + c]) { first[u] = c; break; }
+        order[u] = u;
     }
-    if (aoff * sizeof(_Float16) > kMaxGramBytes) return set_err(h, FBP_E_UNSUPPORTED, "likelihood tables too large");
-    HIP_TRY(h->ALh.ensure(aoff * sizeof(_Float16)), "alloc likelihood tables");
-    HIP_TRY(h->orow.ensure((size_t)U * 4), "alloc likelihood rows");
-    HIP_TRY(hipMemcpy(h->orow.p, orow.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
-    TemplateSet ts = template_view(h);
-    for (size_t gi = 0; gi < h->groups.size(); gi++) {
-        const auto& g = h->groups[gi];
-        launch_likmix_A(ts, h->gunits.as<int>() + g.units_off, g.nunits, g.c_lo, group_channels(h, g), P, h->tscale,
-                        h->ALh.as<_Float16>() + h->lik_aoff[gi], h->stream);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
+    const int ntiles = (U + 15) / 16;
+    std::vector<int> units((size_t)ntiles * 16, -1), nkb(ntiles), nblk(ntiles), aoff(ntiles), chb(ntiles, 0);
+    int total_kb = 0, maxnb = 1;
+    for (int t = 0; t < ntiles; t++) {
+        int blo = kMfMaxBlk, bhi = -1;
+        for (int r = 0; r < 16 && t * 16 + r < U; r++) {
+            const int u = order[t * 16 + r];
+            units[(size_t)t * 16 + r] = u;
+            for (int c = 0; c < C; c++)
+                if (h->h_nz[(size_t)u * C + c]) {
+                    blo = std::min(blo, c / 8);
+                    bhi = std::max(bhi, c / 8);
+                }
+        }
+        if (bhi < 0) blo = bhi = 0;
+        chb[t] = blo;  // the tile covers the contiguous 8-channel blocks blo .. bhi
+        const int nb = bhi - blo + 1;
+        nblk[t] = nb;
+        nkb[t] = (L * nb + 3) / 4;
+        aoff[t] = total_kb;
+        total_kb += nkb[t];
+        maxnb = std::max(maxnb, nb);
     }
-    HIP_TRY(hipGetLastError(), "likelihood tables");
-    HIP_TRY(hipStreamSynchronize(h->stream), "likelihood tables");
-    h->lik_ok = true;
+    float tmax = 0.0f;
+    for (float v : hT) tmax = std::max(tmax, std::fabs(v));
+    int ex = 0;
+    std::frexp(tmax > 0.0f ? tmax : 1.0f, &ex);
+    h->mf_tscale = std::ldexp(1.0f, 14 - ex);
+    std::vector<_Float16> frag((size_t)total_kb * 64 * 8);
+    for (int t = 0; t < ntiles; t++)
+        for (int kb = 0; kb < nkb[t]; kb++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int i = 0; i < 8; i++) {
+                    const int u = units[(size_t)t * 16 + (lane & 15)];
+                    const int r = 4 * kb + (lane >> 4);
+                    float v = 0.0f;
+                    if (u >= 0 && r < L * nblk[t]) {
+                        const int j = r / nblk[t], b = r - j * nblk[t];
+                        const int c = (chb[t] + b) * 8 + i;
+                        if (c < C) v = hT[((size_t)u * C + c) * L + j] * h->mf_tscale;
+                    }
+                    frag[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = (_Float16)v;
+                }
+    std::vector<float> T1(U);
+    for (int u = 0; u < U; u++) {
+        double a = 0.0;
+        for (size_t k = 0; k < (size_t)C * L; k++) a += std::fabs((double)hT[(size_t)u * C * L + k]);
+        T1[u] = (float)(a * (1.0 + 1e-6));
+    }
+    HIP_TRY(h->mf_frag.ensure(frag.size() * sizeof(_Float16)), "alloc likelihood tables");
+    HIP_TRY(h->mf_units.ensure(units.size() * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_nkb.ensure((size_t)ntiles * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_nblk.ensure((size_t)ntiles * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_aoff.ensure((size_t)ntiles * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_chb.ensure(chb.size() * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_T1.ensure((size_t)U * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_mbits.ensure(4), "alloc likelihood tables");
+    HIP_TRY(h->mf_vs.ensure(4), "alloc likelihood tables");
+    HIP_TRY(hipMemcpy(h->mf_frag.p, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_units.p, units.data(), units.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_nkb.p, nkb.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_nblk.p, nblk.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_aoff.p, aoff.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_chb.p, chb.data(), chb.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_T1.p, T1.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    h->mf_ntiles = ntiles;
+    // delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts): fp16 rounding
+    // of both operands (2^-11 relative each, 2^-25 absolute below the normal
+    // range), FP32 accumulation of K products (gamma_K, doubled for an unknown
+    // order), the exact chain's own error (gamma_K2 on the products and the
+    // bias), the final scaling and bias add; all doubled
+    const double u = std::ldexp(1.0, -24);
+    const double K = (double)L * 8.0 * maxnb, K2 = (double)h->Cmax * L + 1.0;
+    const double gK = K * u / (1.0 - K * u), gK2 = K2 * u / (1.0 - K2 * u);
+    h->mf_kl = (float)(2.0 * (std::ldexp(1.0, -10) * (1.0 + std::ldexp(1.0, -11)) + 2.0 * gK + gK2 +
+                              std::ldexp(1.0, -23)));
+    h->mf_kb = (float)(2.0 * (gK2 + std::ldexp(1.0, -24)));
+    h->mf_ka = (float)(2.0 * std::ldexp(1.0, -25) * (1.0 + std::ldexp(1.0, -10)));
+    h->mf_ok = true;
     return FBP_OK;
 }
 
-// Mixed-precision likelihood of the listed windows W[w0 .. w0+nb).
-static int run_lik_mixed(fbp_handle* h, const PassState& st, uint32_t w0, uint32_t nb)
+static MfTables mf_view(const fbp_handle* h)
 {
-    TemplateSet ts = template_view(h);
-    const int L = h->L, P = L, R = P + L - 1, U = h->U, C = h->C;
-    launch_likmix_pack(ts, st, w0, nb, P, h->seg.as<_Float16>(), h->lvscale.as<float>(), h->stream);
-    const float one = 1.0f, zero = 0.0f;
-    for (size_t gi = 0; gi < h->groups.size(); gi++) {
-        const auto& g = h->groups[gi];
-        const int M = g.nunits * P, K = group_channels(h, g) * R;
-        rocblas_status rs = rocblas_gemm_ex(
-            h->rb, rocblas_operation_transpose, rocblas_operation_none, M, (int)nb, K, &one,
-            h->ALh.as<_Float16>() + h->lik_aoff[gi], rocblas_datatype_f16_r, K,
-            h->seg.as<_Float16>() + (size_t)g.c_lo * R, rocblas_datatype_f16_r, C * R, &zero,
-            h->lout.as<float>() + h->lik_row[gi], rocblas_datatype_f32_r, U * P, h->lout.as<float>() + h->lik_row[gi],
-            rocblas_datatype_f32_r, U * P, rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
-        if (rs != rocblas_status_success)
-            return set_err(h, FBP_E_HIP, std::string("rocblas_gemm_ex failed: ") + rocblas_status_to_string(rs));
-    }
-    // single fp16 pass: input rounding 2 * 2^-11 relative, fp32 accumulation of
-    // at most Cmax*L non-zero products (unit roundoff doubled), exact chain
-    // error on the same terms and on the bias; all doubled
-    const double u = std::ldexp(1.0, -24);
-    const double K1 = (double)h->Cmax * L + 2.0;
-    const double g2 = K1 * 2.0 * u / (1.0 - K1 * 2.0 * u);
-    const float kl = (float)(2.0 * (std::ldexp(1.0, -10) + std::ldexp(1.0, -21) + 2.0 * g2));
-    const float kb = (float)(2.0 * 2.0 * g2);
-    launch_likmix_post(ts, st, w0, nb, P, h->lout.as<float>(), h->orow.as<int>(), h->lvscale.as<float>(),
-                       h->rowL1.as<float>(), 1.0f / h->tscale, kl, kb, h->stream);
-    return FBP_OK;
+    MfTables mt;
+    mt.frag = h->mf_frag.as<half8>();
+    mt.units = h->mf_units.as<int>();
+    mt.nkb = h->mf_nkb.as<int>();
+    mt.nblk = h->mf_nblk.as<int>();
+    mt.aoff = h->mf_aoff.as<int>();
+    mt.chb = h->mf_chb.as<int>();
+    mt.T1 = h->mf_T1.as<float>();
+    mt.ntiles = h->mf_ntiles;
+    mt.inv_tscale = 1.0f / h->mf_tscale;
+    mt.kl = h->mf_kl;
+    mt.kb = h->mf_kb;
+    mt.ka = h->mf_ka;
+    return mt;
 }
 
 // B for the flagged windows F[f0 .. f0+nb) into Bbuf[w][row] by three
@@ -594,13 +661,14 @@ int fbp_destroy(fbp_handle* h)
     if (!h) return FBP_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK, &h->wmax_v, &h->wmax_p,
+    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->Vtmp, &h->LIK, &h->wmax_v, &h->wmax_p,
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale,
-                      &h->ALh, &h->orow, &h->seg, &h->lvscale, &h->lout};
+                      &h->mf_frag, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
+                      &h->mf_mask, &h->mf_mbits, &h->mf_vs};
     for (DevBuf* b : bufs) b->release();
     if (h->rb) (void)rocblas_destroy_handle(h->rb);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
@@ -699,7 +767,7 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
     HIP_TRY(hipMemcpy(h->SS.p, h->h_SS.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy norms");
     h->tab_n = -1;
     h->groups_ok = false;
-    h->lik_ok = false;
+    h->mf_built = false;
     h->have_templates = true;
     h->have_result = false;
     return FBP_OK;
@@ -752,18 +820,16 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(grow_keep(h->sp_idx, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
     HIP_TRY(grow_keep(h->sp_lab, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
 
-    const bool mixed = (flags & FBP_FLAG_PRECISION_MIXED) != 0;
+    bool mixed = (flags & FBP_FLAG_PRECISION_MIXED) != 0;
     const uint32_t lik_batch = 65536;
     if (mixed) {
-        if (!h->lik_ok) {
-            int rc = build_lik_tables(h);
+        if (!h->mf_built) {
+            int rc = build_mf_tables(h);
             if (rc != FBP_OK) return rc;
         }
-        const int R = 2 * (int)L - 1;
-        HIP_TRY(h->seg.ensure((size_t)lik_batch * C * R * sizeof(_Float16)), "alloc likelihood segments");
-        HIP_TRY(h->lvscale.ensure((size_t)lik_batch * 4), "alloc likelihood segments");
-        HIP_TRY(h->lout.ensure((size_t)lik_batch * U * L * 4), "alloc likelihood output");
+        mixed = h->mf_ok;  // outside the MFMA tables' limits the FP32 likelihood runs (same results)
     }
+    if (mixed) HIP_TRY(h->mf_mask.ensure((size_t)lik_batch * U), "alloc likelihood masks");
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0), "event");
@@ -771,11 +837,15 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(hipEventRecord(ev0, s), "event");
 
     PassState st = pass_view(h);
+    // the residual is kept sample-major ([N][C]): a window is one contiguous block
     if (v_on_device) {
-        launch_copy_rows(Vsrc, ld, st.V, C, N, s);
+        launch_to_sample_major(Vsrc, ld, st.V, C, N, s);
     } else {
-        HIP_TRY(hipMemcpy2DAsync(st.V, N * 4, Vsrc, ld * 4, N * 4, C, hipMemcpyHostToDevice, s), "copy voltage");
+        HIP_TRY(h->Vtmp.ensure((size_t)C * N * 4), "alloc voltage staging");
+        HIP_TRY(hipMemcpy2DAsync(h->Vtmp.p, N * 4, Vsrc, ld * 4, N * 4, C, hipMemcpyHostToDevice, s), "copy voltage");
+        launch_to_sample_major(h->Vtmp.as<float>(), N, st.V, C, N, s);
     }
+    if (mixed) launch_absmax(st.V, (int64_t)C * N, h->mf_mbits.as<uint32_t>(), h->mf_vs.as<float>(), s);
     // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
     // best_* zeros, flags zeros, window list 0..nwin-1).
     launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
@@ -821,10 +891,10 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
         tm.begin(TM_LIK);
         if (mixed) {
-            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
-                int rc = run_lik_mixed(h, st, w0, std::min(lik_batch, nW - w0));
-                if (rc != FBP_OK) return rc;
-            }
+            const MfTables mt = mf_view(h);
+            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch)
+                launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(), w0,
+                               std::min(lik_batch, nW - w0), h->mf_mask.as<uint8_t>(), s);
         } else {
             launch_likelihood(ts, st, nW, s);
         }
@@ -968,7 +1038,10 @@ int fbp_copy_residual(fbp_handle* h, float* out)
     if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
     if (!out) return set_err(h, FBP_E_INVALID, "null output");
     HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
-    HIP_TRY(hipMemcpyAsync(out, h->V.p, (size_t)h->C * h->N * 4, hipMemcpyDeviceToHost, h->stream), "copy");
+    HIP_TRY(h->Vtmp.ensure((size_t)h->C * h->N * 4), "alloc voltage staging");
+    launch_to_channel_major(h->V.as<float>(), h->Vtmp.as<float>(), h->C, h->N, h->stream);
+    HIP_TRY(hipGetLastError(), "residual layout");
+    HIP_TRY(hipMemcpyAsync(out, h->Vtmp.p, (size_t)h->C * h->N * 4, hipMemcpyDeviceToHost, h->stream), "copy");
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     return FBP_OK;
 }

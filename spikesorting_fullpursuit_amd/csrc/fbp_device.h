@@ -60,8 +60,8 @@ __device__ __forceinline__ bool recheck_item_skipped(const float* __restrict__ L
 }
 
 // Exact FP32 value of item (u, f, s), binary_pursuit.cl:578-632.  vw points at
-// the window's voltage rows: vw[c * vstride + j] = V[c, b + j].
-__device__ __forceinline__ float recheck_item_value(const TemplateSet& ts, const float* vw, int vstride,
+// the window's voltage: vw[c * cstride + j * jstride] = V[c, b + j].
+__device__ __forceinline__ float recheck_item_value(const TemplateSet& ts, const float* vw, int cstride, int jstride,
                                                     uint32_t lab, uint32_t u, int64_t f, int64_t s, int64_t n)
 {
     const int C = ts.C, L = ts.L;
@@ -74,14 +74,14 @@ __device__ __forceinline__ float recheck_item_value(const TemplateSet& ts, const
         if (!nzf && !nzs) continue;
         const float* tf = ts.T + ((size_t)lab * C + c) * L;
         const float* tsr = ts.T + ((size_t)u * C + c) * L;
-        const float* vr = vw + (size_t)c * vstride;
+        const float* vr = vw + (size_t)c * cstride;
         float ss = 0.0f;
         for (int j = 0; j < L; j++) {
             const float fv = (j >= ffj && fa + j < L) ? tf[fa + j] : 0.0f;
             const float sv = (j >= sfj && sa + j < L) ? tsr[sa + j] : 0.0f;
             const float sum = fv + sv;
             ss = fmaf(sum, sum, ss);
-            lik = fmaf(sum, vr[j], lik);
+            lik = fmaf(sum, vr[(size_t)j * jstride], lik);
         }
         if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
     }
@@ -123,12 +123,10 @@ __device__ inline void apply_parse(const TemplateSet& ts, const PassState& st, i
 // acc[q] = fma chain over taps i of trow[i] * vrow[i + q], taps ascending.
 constexpr int kPos = 8;  // positions per work item
 
+// Taps ascending over registers r[i + q] (r holds LT + kPos - 1 samples).
 template <int LT>
-__device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, const float* vrow, float* acc)
+__device__ __forceinline__ void lik_row_core(const float* __restrict__ trow, const float* r, float* acc)
 {
-    float r[LT + kPos - 1];
-#pragma unroll
-    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
 #pragma unroll
     for (int i = 0; i < LT; i += 4) {
         const float4 tv = *reinterpret_cast<const float4*>(trow + i);
@@ -143,6 +141,26 @@ __device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, co
     }
 }
 
+template <int LT>
+__device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, const float* vrow, float* acc)
+{
+    float r[LT + kPos - 1];
+#pragma unroll
+    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
+    lik_row_core<LT>(trow, r, acc);
+}
+
+// Same, reading only vrow[0 .. lim) (zeros past it).
+template <int LT>
+__device__ __forceinline__ void lik_row_fixed_bounded(const float* __restrict__ trow, const float* __restrict__ vrow,
+                                                      int64_t lim, float* acc)
+{
+    float r[LT + kPos - 1];
+#pragma unroll
+    for (int j = 0; j < LT + kPos - 1; j++) r[j] = (j < lim) ? vrow[j] : 0.0f;
+    lik_row_core<LT>(trow, r, acc);
+}
+
 __device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, const float* vrow, int L, float* acc)
 {
     float r[kPos];
@@ -155,6 +173,22 @@ __device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, 
 #pragma unroll
         for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
         r[kPos - 1] = vrow[i + kPos];
+    }
+}
+
+__device__ __forceinline__ void lik_row_generic_bounded(const float* __restrict__ trow, const float* __restrict__ vrow,
+                                                        int L, int64_t lim, float* acc)
+{
+    float r[kPos];
+#pragma unroll
+    for (int q = 0; q < kPos; q++) r[q] = (q < lim) ? vrow[q] : 0.0f;
+    for (int i = 0; i < L; i++) {
+        const float tv = trow[i];
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
+        r[kPos - 1] = (i + kPos < lim) ? vrow[i + kPos] : 0.0f;
     }
 }
 

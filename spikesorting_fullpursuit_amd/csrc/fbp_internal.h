@@ -22,11 +22,12 @@ struct TemplateSet {
     const float* bias;     // [U]
     const float* thr;      // [U]
     int U, C, L, Lp;
+    int R;                 // rows of the compact layout
 };
 
 // Per-chunk pursuit state (all device pointers).
 struct PassState {
-    float* V;              // [C][N] residual, row stride N
+    float* V;              // [N][C] residual, sample-major (V[x * C + c])
     float* LIK;            // [U][N]
     float* wmax_v;         // [nwin][U] maximum of LIK over each window
     uint16_t* wmax_p;      // [nwin][U] first position of that maximum in the window

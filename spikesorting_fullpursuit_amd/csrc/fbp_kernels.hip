@@ -5,6 +5,8 @@
 // floating-point reduction that feeds a decision uses the same operation order
 // as the CPU restatement (explicit fmaf, ascending channel then sample), so
 // decisions are bit-identical to it.  Built with -ffp-contract=off.
+#include <algorithm>
+
 #include "fbp_device.h"
 
 namespace fbp {
@@ -16,13 +18,19 @@ namespace fbp {
 // register window over the voltage row staged in LDS.
 // ---------------------------------------------------------------------------
 
+// With fmask (one byte per listed window and template, bit p: positions
+// 16p .. 16p+15) only the groups of flagged tiles are computed, and the window
+// maximum of a pair covers its flagged groups; pairs without a flag are left
+// as they are.
 template <int LT>
 __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
                                                        const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
                                                        float* __restrict__ LIK, float* __restrict__ wmax_v,
-                                                       uint16_t* __restrict__ wmax_p)
+                                                       uint16_t* __restrict__ wmax_p,
+                                                       const uint8_t* __restrict__ fmask, uint32_t* __restrict__ npos)
 {
-    extern __shared__ float sV[];  // [wb][C][sv], then per-item maxima [wb][U][ngr] (value, position)
+    extern __shared__ float sV[];  // [wb][C][sv], per-item maxima [wb][U][ngr] (value, position), item list
+    __shared__ uint32_t nl_s, np_s;
     const int C = ts.C, U = ts.U;
     const int L = (LT > 0) ? LT : ts.L;
     const int ngr = (L + kPos - 1) / kPos;
@@ -30,19 +38,43 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     const uint32_t nbatch = (nW + wb - 1) / wb;
     float* pv = sV + wb * C * sv;
     uint32_t* pp = reinterpret_cast<uint32_t*>(pv + wb * per_win);
+    uint32_t* ilist = pp + wb * per_win;
     for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
         const uint32_t w0 = bt * wb;
         const int nb = (int)min((uint32_t)wb, nW - w0);
+        int nitems = nb * per_win;
+        if (fmask) {
+            __syncthreads();
+            if (threadIdx.x == 0) { nl_s = 0; np_s = 0; }
+            __syncthreads();
+            for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
+                const int b = it / per_win;
+                const int rem = it - b * per_win;
+                const int t = rem / ngr, g = rem - t * ngr;
+                pv[it] = -INFINITY;
+                pp[it] = 0xffffu;
+                if ((fmask[(size_t)(w0 + b) * U + t] >> ((g * kPos) / 16)) & 1u) {
+                    ilist[atomicAdd(&nl_s, 1u)] = it;
+                    atomicAdd(&np_s, (uint32_t)min(kPos, L - g * kPos));
+                }
+            }
+            __syncthreads();
+            nitems = (int)nl_s;
+            if (threadIdx.x == 0 && np_s) atomicAdd(npos, np_s);
+            if (nitems == 0) continue;  // uniform over the block
+        }
         __syncthreads();
+        // V is sample-major ([N][C]): read along channels, store channel rows
         for (int e = threadIdx.x; e < nb * C * sv; e += blockDim.x) {
             const int b = e / (C * sv);
             const int rem = e - b * C * sv;
-            const int c = rem / sv, j = rem - c * sv;
+            const int j = rem / C, c = rem - j * C;
             const int64_t x = (int64_t)W[w0 + b] * L + j;
-            sV[e] = (x < N) ? V[(size_t)c * N + x] : 0.0f;
+            sV[(b * C + c) * sv + j] = (x < N) ? V[(size_t)x * C + c] : 0.0f;
         }
         __syncthreads();
-        for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
+        for (int ii = threadIdx.x; ii < nitems; ii += blockDim.x) {
+            const int it = fmask ? (int)ilist[ii] : ii;
             const int b = it / per_win;
             const int rem = it - b * per_win;
             const int t = rem / ngr, g = rem - t * ngr;
@@ -79,6 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
         // per (window, template): first position of the window maximum
         for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
             const int b = pr / U, t = pr - b * U;
+            if (fmask && fmask[(size_t)(w0 + b) * U + t] == 0) continue;
             float mv = -INFINITY;
             uint32_t mp = 0xffffu;
             for (int g = 0; g < ngr; g++) {
@@ -268,18 +301,18 @@ __global__ __launch_bounds__(kBlock) void k_recheck_literal(TemplateSet ts, Pass
         const float lb = st.LIK[(size_t)lab * N + b];
         const bool wskip = (lb < ts.thr[lab]) || (b >= N - L);
         const float* vw;
-        int vstride;
+        int cstride, jstride;
         __syncthreads();
         if (use_lds) {
             if (!wskip)
                 for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-                    const int c = e / L, j = e - c * L;
-                    sW[e] = st.V[(size_t)c * N + b + j];
+                    const int j = e / C, c = e - j * C;
+                    sW[c * L + j] = st.V[(size_t)(b + j) * C + c];
                 }
             __syncthreads();
-            vw = sW; vstride = L;
+            vw = sW; cstride = L; jstride = 1;
         } else {
-            vw = st.V + b; vstride = (int)N;
+            vw = st.V + (size_t)b * C; cstride = 1; jstride = C;
         }
         float bv = 0.0f;
         uint32_t bk = kNoItem;
@@ -291,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_recheck_literal(TemplateSet ts, Pass
                 const uint32_t u = (uint32_t)(k % U);
                 const int64_t f = (int64_t)((k / U) % S), s = (int64_t)(k / (S * U));
                 if (recheck_item_skipped(st.LIK, N, L, lab, u, b, f, s, n_max)) continue;
-                const float v = recheck_item_value(ts, vw, vstride, lab, u, f, s, n_max);
+                const float v = recheck_item_value(ts, vw, cstride, jstride, lab, u, f, s, n_max);
                 argmax_merge(bv, bk, v, (uint32_t)k);
             }
         }
@@ -430,7 +463,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __re
         if (slab[k] >= (uint32_t)ts.U) continue;
         const uint32_t jlo = (k > (uint32_t)reach) ? k - reach : 0u;
         const uint32_t jhi = min(nsp - 1, k + (uint32_t)reach);
-        for (int i = threadIdx.x; i < L; i += blockDim.x) {
+        // one thread per (sample, channel), channels fastest: V is sample-major
+        for (int e = threadIdx.x; e < L * C; e += blockDim.x) {
+            const int i = e / C, c = e - i * C;
             const int64_t x = pk + i;
             if (x >= N) continue;
             bool owner = true;
@@ -439,16 +474,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __re
                 if (slab[j] < (uint32_t)ts.U && x >= pj && x < pj + L) { owner = false; break; }
             }
             if (!owner) continue;
-            for (int c = 0; c < C; c++) {
-                float v = V[(size_t)c * N + x];
-                for (uint32_t j = k; j <= jhi; j++) {
-                    const int64_t pj = sidx[j];
-                    const uint32_t lj = slab[j];
-                    if (lj >= (uint32_t)ts.U || x < pj || x >= pj + L) continue;
-                    v = v - ts.T[((size_t)lj * C + c) * L + (x - pj)];
-                }
-                V[(size_t)c * N + x] = v;
+            float v = V[(size_t)x * C + c];
+            for (uint32_t j = k; j <= jhi; j++) {
+                const int64_t pj = sidx[j];
+                const uint32_t lj = slab[j];
+                if (lj >= (uint32_t)ts.U || x < pj || x >= pj + L) continue;
+                v = v - ts.T[((size_t)lj * C + c) * L + (x - pj)];
             }
+            V[(size_t)x * C + c] = v;
         }
     }
 }
@@ -469,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_adjusted_clips(TemplateSet ts, const
         const int64_t x = (int64_t)sidx[k] + i;
         float out = 0.0f;
         if (slab[k] < (uint32_t)ts.U && x < N)
-            out = V[(size_t)c * N + x] + ts.T[((size_t)slab[k] * C + c) * L + i];
+            out = V[(size_t)x * C + c] + ts.T[((size_t)slab[k] * C + c) * L + i];
         clips[e] = out;
     }
 }
@@ -487,13 +520,49 @@ __global__ __launch_bounds__(kBlock) void k_iota(uint32_t* p, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = (uint32_t)i;
 }
-__global__ __launch_bounds__(kBlock) void k_copy_rows(const float* __restrict__ src, int64_t ld, float* __restrict__ dst,
-                                                      int64_t C, int64_t N)
+// dst[x][c] = src[c][x] (row stride ld), 64 x 64 tiles through LDS.
+__global__ __launch_bounds__(kBlock) void k_to_sample_major(const float* __restrict__ src, int64_t ld,
+                                                            float* __restrict__ dst, int64_t C, int64_t N)
 {
-    const int64_t total = C * N;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = e / N, x = e - c * N;
-        dst[e] = src[c * ld + x];
+    __shared__ float tile[64][65];
+    const int64_t tx = (N + 63) / 64, tc = (C + 63) / 64;
+    for (int64_t tI = blockIdx.x; tI < tx * tc; tI += gridDim.x) {
+        const int64_t x0 = (tI % tx) * 64, c0 = (tI / tx) * 64;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+            const int r = e / 64, q = e - r * 64;  // r: channel, q: sample
+            const int64_t c = c0 + r, x = x0 + q;
+            tile[r][q] = (c < C && x < N) ? src[c * ld + x] : 0.0f;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+            const int q = e / 64, r = e - q * 64;
+            const int64_t c = c0 + r, x = x0 + q;
+            if (c < C && x < N) dst[x * C + c] = tile[r][q];
+        }
+    }
+}
+
+// dst[c][x] = src[x][c].
+__global__ __launch_bounds__(kBlock) void k_to_channel_major(const float* __restrict__ src, float* __restrict__ dst,
+                                                             int64_t C, int64_t N)
+{
+    __shared__ float tile[64][65];
+    const int64_t tx = (N + 63) / 64, tc = (C + 63) / 64;
+    for (int64_t tI = blockIdx.x; tI < tx * tc; tI += gridDim.x) {
+        const int64_t x0 = (tI % tx) * 64, c0 = (tI / tx) * 64;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+            const int q = e / 64, r = e - q * 64;
+            const int64_t c = c0 + r, x = x0 + q;
+            tile[r][q] = (c < C && x < N) ? src[x * C + c] : 0.0f;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) {
+            const int r = e / 64, q = e - r * 64;
+            const int64_t c = c0 + r, x = x0 + q;
+            if (c < C && x < N) dst[c * N + x] = tile[r][q];
+        }
     }
 }
 
@@ -520,16 +589,24 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_iota, dim3(grid_for(n, kBlock * 4)), dim3(kBlock), 0, s, p, n);
 }
-void launch_copy_rows(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s)
+void launch_to_sample_major(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(C * N, kBlock * 8)), dim3(kBlock), 0, s, src, ld, dst, C, N);
+    const int64_t tiles = ((N + 63) / 64) * ((C + 63) / 64);
+    hipLaunchKernelGGL(k_to_sample_major, dim3((unsigned)std::min<int64_t>(tiles, 1 << 20)), dim3(kBlock), 0, s, src,
+                       ld, dst, C, N);
+}
+void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N, hipStream_t s)
+{
+    const int64_t tiles = ((N + 63) / 64) * ((C + 63) / 64);
+    hipLaunchKernelGGL(k_to_channel_major, dim3((unsigned)std::min<int64_t>(tiles, 1 << 20)), dim3(kBlock), 0, s, src,
+                       dst, C, N);
 }
 
 // Windows per likelihood block and LDS row length.
 static int64_t likelihood_lds_per_window(const TemplateSet& ts, int sv)
 {
     const int ngr = (ts.L + kPos - 1) / kPos;
-    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 8;
+    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 12;
 }
 
 void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
@@ -552,7 +629,8 @@ bool likelihood_fits(const TemplateSet& ts)
     return wb * likelihood_lds_per_window(ts, sv) <= 64 * 1024;
 }
 
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
+static void likelihood_launch(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
+                              const uint8_t* fmask, hipStream_t s)
 {
     if (nW == 0) return;
     int wb, sv;
@@ -563,13 +641,30 @@ void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, 
     const float* V = st.V;
     const int64_t N = st.N;
     if (ts.L == 48)
-        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
+                           st.counters + CNT_LIKX);
     else if (ts.L == 64)
-        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
+                           st.counters + CNT_LIKX);
     else if (ts.L == 32)
-        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
+                           st.counters + CNT_LIKX);
     else
-        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, st.W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
+                           st.counters + CNT_LIKX);
+}
+
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
+{
+    likelihood_launch(ts, st, st.W, nW, nullptr, s);
+}
+
+// Exact FP32 likelihood of the 16-position tiles flagged in fmask (one byte per
+// window W[0 .. nW) and template).
+void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
+                       const uint8_t* fmask, hipStream_t s)
+{
+    likelihood_launch(ts, st, W, nW, fmask, s);
 }
 
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
@@ -639,7 +734,7 @@ void launch_residual(const TemplateSet& ts, float* V, int64_t N, const uint32_t*
                      int reach, hipStream_t s)
 {
     if (nsp == 0) return;
-    hipLaunchKernelGGL(k_residual, dim3(grid_for(nsp, 1, 1u << 20)), dim3(ts.L <= 64 ? 64 : kBlock), 0, s, ts, V, N,
+    hipLaunchKernelGGL(k_residual, dim3(grid_for(nsp, 1, 1u << 20)), dim3(kBlock), 0, s, ts, V, N,
                        sidx, slab, nsp, reach);
 }
 

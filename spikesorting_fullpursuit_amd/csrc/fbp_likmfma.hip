@@ -1,0 +1,359 @@
+// Mixed-precision likelihood on the gfx950 matrix cores.
+//
+// compute_full_likelihood (SF/kernels/binary_pursuit.cl:216-300, SF =
+// spikesorting_fullpursuit/) gives, for every position x of a listed window
+// and every template t,
+//     LIK(t, x) = bias_t + sum_{c, j} T[t][c][j] * V[c][x + j].
+// Here the sum runs as an fp16 MFMA product (v_mfma_f32_16x16x32_f16, FP32
+// accumulation): rows of the A operand are 16 templates (a tile), columns of
+// the B operand are 16 positions, and the reduction index k = (j, channel)
+// covers the 8-channel blocks that hold the tile's non-zero channels.  A is
+// constant and stored in fragment order; B is read straight from the window's
+// voltage staged in LDS as [sample][channel] fp16, where the 8 channels of a
+// block at one sample are 16 contiguous, 16-byte aligned bytes.
+//
+// |approximation - exact FP32 chain| <= delta(t, window), a bound built from
+// the template L1 norms and the window's channel maxima.  Only likelihoods
+// above 0, or their sign, ever decide anything (window maxima start at 0,
+// flags and skips test > 0 or <= 0, the parse cascade rejects values <= 0), so
+// a 16-position tile of a (window, template) pair whose every approximation is
+// <= -delta keeps the approximations (all negative, as are the exact values),
+// and every other tile is recomputed with the exact FP32 chain by the masked
+// pass of k_likelihood.
+#include <math.h>
+
+#include <algorithm>
+
+#include "fbp_device.h"
+
+namespace fbp {
+
+void launch_fill_u32(uint32_t* p, uint32_t v, int64_t n, hipStream_t s);
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfWin = 4;      // windows per block iteration
+constexpr int kMfMaxBlk = 32;  // 8-channel blocks per tile at most (C <= 256)
+constexpr int kMfThreads = 256;
+
+struct MfTables {
+    const half8* frag;  // A fragments: tile, k-block, lane
+    const int* units;   // [ntiles * 16] unit of each tile row (-1: padding)
+    const int* nkb;     // [ntiles] k-blocks of 32
+    const int* nblk;    // [ntiles] 8-channel blocks
+    const int* aoff;    // [ntiles] first k-block of the tile in frag
+    const int* chb;     // [ntiles] first 8-channel block (the tile's blocks are contiguous)
+    const float* T1;    // [U] sum of |T| over the template (rounded up)
+    int ntiles;
+    float inv_tscale;   // 1 / template scale (a power of two)
+    float kl, kb, ka;   // delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts)
+};
+
+// Largest |V| of the chunk as float bits (non-negative floats order as integers).
+__global__ __launch_bounds__(kBlock) void k_absmax(const float* __restrict__ V, int64_t n, uint32_t* __restrict__ out)
+{
+    float m = 0.0f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(V[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane_id() == 0 && m > 0.0f) atomicMax(out, __float_as_uint(m));
+}
+
+// Voltage scale: the power of two that maps the largest |V| into [2^13, 2^14).
+__global__ void k_vscale(const uint32_t* __restrict__ mbits, float* __restrict__ vs)
+{
+    const float m = __uint_as_float(*mbits);
+    int e = 0;
+    if (m > 0.0f && isfinite(m)) frexpf(m, &e);
+    *vs = (m > 0.0f && isfinite(m)) ? ldexpf(1.0f, 14 - e) : 1.0f;
+}
+
+template <int NP>
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_likmfma(TemplateSet ts, PassState st, MfTables mt,
+                                                        const float* __restrict__ rowL1, const float* __restrict__ vsp,
+                                                        const uint32_t* __restrict__ mbits, uint32_t w0, uint32_t nb,
+                                                        uint8_t* __restrict__ fmask)
+{
+    extern __shared__ __align__(16) char smem[];
+    __shared__ int64_t wid_s[kMfWin];
+    constexpr int KW = kMfWin * NP;  // accumulator tiles per wave
+    const int C = ts.C, U = ts.U, L = ts.L, R = ts.R;
+    const int Cp = (C + 7) & ~7, RS = Cp + 8;  // halfs per staged sample (8 spare: bank spread)
+    const int NS = NP * 16 + L - 1;            // samples read per window
+    const int NSP = (NS + 15) & ~15;           // staged samples per window (16-sample blocks)
+    const int NB = NSP / 16;
+    const int64_t N = st.N;
+    _Float16* Vh = reinterpret_cast<_Float16*>(smem);                        // [kMfWin][NSP][RS]
+    float* bmx = reinterpret_cast<float*>(Vh + kMfWin * NSP * RS);          // [kMfWin][NB][Cp] block max |v|
+    float* Qs = bmx + kMfWin * NB * Cp;                                      // [kMfWin][U][NP] sum rowL1 * vmax
+    uint32_t* flag = reinterpret_cast<uint32_t*>(Qs + kMfWin * U * NP);     // [kMfWin][U] bit p: tile p rescored
+    int* rch = reinterpret_cast<int*>(flag + kMfWin * U);                   // [R] channel of each compact row
+    float* rl1 = reinterpret_cast<float*>(rch + R);                          // [R] its L1 norm
+    const int lane = lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int q4 = lane >> 4, col = lane & 15;
+    const float vs = *vsp;
+    const float inv = mt.inv_tscale / vs;
+    const float ivs = 1.0f / vs;
+    const float gmax = __uint_as_float(*mbits) * (1.0f + 0x1p-20f);  // largest |V| of the chunk
+    for (int e = threadIdx.x; e < R; e += blockDim.x) {
+        rch[e] = ts.chan[e];
+        rl1[e] = rowL1[e];
+    }
+    const uint32_t ngroups = (nb + kMfWin - 1) / kMfWin;
+    for (uint32_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        __syncthreads();
+        if (threadIdx.x < kMfWin) {
+            const uint32_t w = gi * kMfWin + threadIdx.x;
+            wid_s[threadIdx.x] = w < nb ? (int64_t)st.W[w0 + w] : -1;
+        }
+        for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) flag[e] = 0u;
+        __syncthreads();
+        // stage: one thread per (window, sample): the sample's channels are
+        // contiguous in V ([N][C]), 8 channels per 16-byte LDS store; 16
+        // consecutive lanes hold one 16-sample block, so the per-channel block
+        // maxima are 4 shuffle steps
+        for (int col2 = threadIdx.x; col2 < kMfWin * NSP; col2 += blockDim.x) {
+            const int k = col2 / NSP, j = col2 - k * NSP;
+            const int64_t wd = wid_s[k];
+            const int64_t x = wd * L + j;
+            const bool in = wd >= 0 && j < NS && x < N;
+            _Float16* dst = Vh + (size_t)col2 * RS;
+            const float* src = st.V + (size_t)(in ? x : 0) * C;
+            for (int c0 = 0; c0 < Cp; c0 += 32) {
+                float v[32];
+                if (C % 4 == 0) {
+#pragma unroll
+                    for (int i = 0; i < 32; i += 4) {
+                        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (in && c0 + i < C) q = *reinterpret_cast<const float4*>(src + c0 + i);
+                        v[i] = q.x; v[i + 1] = q.y; v[i + 2] = q.z; v[i + 3] = q.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 32; i++) v[i] = (in && c0 + i < C) ? src[c0 + i] : 0.0f;
+                }
+                __builtin_amdgcn_sched_barrier(0);  // the loads go out together
+#pragma unroll
+                for (int h8 = 0; h8 < 4; h8++) {
+                    if (c0 + h8 * 8 >= Cp) break;
+                    half8 h;
+#pragma unroll
+                    for (int i = 0; i < 8; i++) h[i] = (_Float16)(v[h8 * 8 + i] * vs);
+                    *reinterpret_cast<half8*>(dst + c0 + h8 * 8) = h;
+                }
+#pragma unroll
+                for (int i = 0; i < 32; i++) {
+                    float m = fabsf(v[i]);
+                    m = fmaxf(m, __shfl_xor(m, 1, 64));
+                    m = fmaxf(m, __shfl_xor(m, 2, 64));
+                    m = fmaxf(m, __shfl_xor(m, 4, 64));
+                    m = fmaxf(m, __shfl_xor(m, 8, 64));
+                    if (col == 0 && c0 + i < Cp) bmx[((size_t)k * NB + j / 16) * Cp + c0 + i] = m;
+                }
+            }
+        }
+        __syncthreads();
+        // per (window, template): Q[p] = sum rowL1 * max|v| over the samples
+        // that 16-position tile p reads (blocks p .. p + (15 + L - 1) / 16)
+        for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
+            const int k = e / U, t = e - k * U;
+            const int r0 = ts.off[t], nr = ts.nch[t];
+            float q[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) q[p] = 0.0f;
+            const float* bk = bmx + (size_t)k * NB * Cp;
+            for (int i = 0; i < nr; i++) {
+                const int c = rch[r0 + i];
+                const float w1 = rl1[r0 + i];
+                float bm[NP + 4];
+#pragma unroll
+                for (int bq = 0; bq < NP + 4; bq++) bm[bq] = bq < NB ? bk[bq * Cp + c] : 0.0f;
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const int bhi = min(NB - 1, (p * 16 + 15 + L - 1) / 16);
+                    float vm = 0.0f;
+#pragma unroll
+                    for (int bq = 0; bq < NP + 4; bq++)
+                        if (bq >= p && bq <= bhi) vm = fmaxf(vm, bm[bq]);
+                    q[p] += w1 * vm;
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) Qs[e * NP + p] = q[p];
+        }
+        __syncthreads();
+        for (int tile = wv; tile < mt.ntiles; tile += nwv) {
+            const int nkb = mt.nkb[tile], nbt = mt.nblk[tile];
+            const half8* Af = mt.frag + (size_t)mt.aoff[tile] * 64 + lane;
+            const _Float16* vbase = Vh + col * RS + mt.chb[tile] * 8;
+            floatx4 acc[KW];
+#pragma unroll
+            for (int i = 0; i < KW; i++) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+            // run of this lane's 8 k-values in k-block kb: r = 4 kb + q4 = (j, b)
+            // with r = j * nbt + b; blocks are contiguous from chb
+            int rj = q4 / nbt, rb = q4 - rj * nbt;
+            auto boff = [&](int j, int b) -> int { return j < L ? j * RS + b * 8 : 0; };
+            auto step = [&]() {
+                rb += 4;
+                while (rb >= nbt) { rb -= nbt; rj++; }
+            };
+            // B fragments of the current k-block loaded, the next in flight;
+            // A fragments four k-blocks ahead
+            half8 bc[KW], bn[KW];
+            {
+                const int o = boff(rj, rb);
+#pragma unroll
+                for (int k = 0; k < kMfWin; k++)
+#pragma unroll
+                    for (int p = 0; p < NP; p++)
+                        bc[k * NP + p] = *reinterpret_cast<const half8*>(vbase + o + (k * NSP + p * 16) * RS);
+            }
+            half8 acur[4], anxt[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) acur[q] = Af[(size_t)min(q, nkb - 1) * 64];
+            for (int kb0 = 0; kb0 < nkb; kb0 += 4) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) anxt[q] = Af[(size_t)min(kb0 + 4 + q, nkb - 1) * 64];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (kb0 + q >= nkb) break;
+                    step();
+                    const int o = boff(rj, rb);
+#pragma unroll
+                    for (int k = 0; k < kMfWin; k++)
+#pragma unroll
+                        for (int p = 0; p < NP; p++)
+                            bn[k * NP + p] = *reinterpret_cast<const half8*>(vbase + o + (k * NSP + p * 16) * RS);
+                    __builtin_amdgcn_sched_barrier(0);  // next fragments in flight during these MFMAs
+#pragma unroll
+                    for (int i = 0; i < KW; i++)
+                        acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(acur[q], bc[i], acc[i], 0, 0, 0);
+#pragma unroll
+                    for (int i = 0; i < KW; i++) bc[i] = bn[i];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) acur[q] = anxt[q];
+            }
+        // lane holds rows 4*q4 + i (templates), column col (position)
+            int unit[4];
+            float bt[4], ka_t[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                unit[i] = mt.units[tile * 16 + q4 * 4 + i];
+                bt[i] = unit[i] >= 0 ? ts.bias[unit[i]] : 0.0f;
+                // sub-normal fp16 residue: sum |T| / vs + L * (channels * max|V|) / ts
+                ka_t[i] = unit[i] >= 0 ? mt.ka * (mt.T1[unit[i]] * ivs +
+                                                  (float)L * (float)ts.nch[unit[i]] * gmax * mt.inv_tscale)
+                                       : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kMfWin; k++) {
+                const int64_t wd = wid_s[k];
+                if (wd < 0) continue;  // uniform over the block
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const int pos = p * 16 + col;
+                    const int64_t x = wd * L + pos;
+                    const bool inpos = pos < L && x < N;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int t = unit[i];
+                        const bool live = inpos && t >= 0;
+                        float v = 0.0f;
+                        bool need = false;
+                        if (live && x + L < N) {
+                            v = bt[i] + acc[k * NP + p][i] * inv;
+                            const int qi = (k * U + t) * NP + p;
+                            const float d = mt.kl * Qs[qi] + mt.kb * fabsf(bt[i]) + ka_t[i];
+                            need = v > -d;
+                        }
+                        if (live) st.LIK[(size_t)t * N + x] = v;
+                        // the 16 lanes of a row group share (template, tile): one LDS OR per group
+                        const uint64_t bm = __ballot(need);
+                        if (col == 0 && ((bm >> (16 * q4)) & 0xffffull) != 0ull)
+                            atomicOr(&flag[k * U + t], 1u << p);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // tile masks for the masked exact pass; pairs with no flagged tile keep
+        // negative window maxima, which the window maximum treats as none
+        for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
+            const int k = e / U, t = e - k * U;
+            const int64_t wd = wid_s[k];
+            if (wd < 0) continue;
+            const uint32_t f = flag[e];
+            fmask[(size_t)(gi * kMfWin + k) * U + t] = (uint8_t)f;
+            if (f == 0u) {
+                st.wmax_v[(size_t)wd * U + t] = -INFINITY;
+                st.wmax_p[(size_t)wd * U + t] = 0xffffu;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+size_t likmfma_lds(const TemplateSet& ts, int ntiles)
+{
+    const int Cp = (ts.C + 7) & ~7, RS = Cp + 8;
+    const int NP = (ts.L + 15) / 16;
+    const int NS = NP * 16 + ts.L - 1;
+    const int NSP = (NS + 15) & ~15, NB = NSP / 16;
+    (void)ntiles;
+    return (size_t)kMfWin * NSP * RS * 2 + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
+           (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8;
+}
+
+void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s)
+{
+    launch_fill_u32(mbits, 0u, 1, s);
+    int64_t g = (n + kBlock * 8 - 1) / (kBlock * 8);
+    g = std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+    hipLaunchKernelGGL(k_absmax, dim3((unsigned)g), dim3(kBlock), 0, s, V, n, mbits);
+    hipLaunchKernelGGL(k_vscale, dim3(1), dim3(1), 0, s, mbits, vs);
+}
+
+template <class K>
+static void allow_lds_mf(K kernel, size_t bytes)
+{
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes);
+}
+
+void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
+                       const uint8_t* fmask, hipStream_t s);
+
+// MFMA likelihood of the listed windows W[w0 .. w0+nb), then the exact FP32
+// chain for the 16-position tiles it flags (fmask, one byte per window and
+// template of the batch).
+void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
+                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s)
+{
+    if (nb == 0) return;
+    const size_t lds = likmfma_lds(ts, mt.ntiles);
+    const uint32_t ngroups = (nb + kMfWin - 1) / kMfWin;
+    const dim3 g(std::min<uint32_t>(ngroups, 4096u)), b(kMfThreads);
+    const int NP = (ts.L + 15) / 16;
+    if (NP == 3) {
+        allow_lds_mf(k_likmfma<3>, lds);
+        hipLaunchKernelGGL(k_likmfma<3>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
+    } else if (NP == 4) {
+        allow_lds_mf(k_likmfma<4>, lds);
+        hipLaunchKernelGGL(k_likmfma<4>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
+    } else if (NP == 2) {
+        allow_lds_mf(k_likmfma<2>, lds);
+        hipLaunchKernelGGL(k_likmfma<2>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
+    } else {
+        allow_lds_mf(k_likmfma<1>, lds);
+        hipLaunchKernelGGL(k_likmfma<1>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
+    }
+    likelihood_masked(ts, st, st.W + w0, nb, fmask, s);
+}
+
+}  // namespace fbp

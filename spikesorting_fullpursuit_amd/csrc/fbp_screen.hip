@@ -138,9 +138,9 @@ __global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassStat
         __syncthreads();
         if (!skip)
             for (int e = threadIdx.x; e < C * rp; e += blockDim.x) {
-                const int c = e / rp, p = e - c * rp;
+                const int p = e / C, c = e - p * C;
                 const int j = p - n;
-                sP[e] = (j >= 0 && j < L) ? st.V[(size_t)c * N + b + j] : 0.0f;
+                sP[c * rp + p] = (j >= 0 && j < L) ? st.V[(size_t)(b + j) * C + c] : 0.0f;
             }
         __syncthreads();
         if (skip) continue;
@@ -317,8 +317,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         BView Bv{Bbuf + (size_t)wi * US, tb.brow, tb.Ntr, tb.gemm ? tb.inv_tscale / tb.vscale[wi] : 1.0f, S,
                  tb.gemm != 0};
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-            const int c = e / L, j = e - c * L;
-            vW[e] = st.V[(size_t)c * N + b + j];
+            const int j = e / C, c = e - j * C;
+            vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
         }
         const int nl = ts.nch[lab];
         for (int e = threadIdx.x; e < nl * L; e += blockDim.x) {
@@ -565,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
                     if (eh < theta) continue;
-                    const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
+                    const float v = recheck_item_value(ts, vW, L, 1, lab, (uint32_t)u, f, s, n);
                     argmax_merge(bv, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
                 }
             }
@@ -712,8 +712,8 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             return gemm ? fmaf(-0.5f, tb.Ntr[e], Bw[browl[u] + s] * sc) : Bw[e];
         };
         for (int e = threadIdx.x; e < C * L; e += bd) {
-            const int c = e / L, j = e - c * L;
-            vW[e] = st.V[(size_t)c * N + b + j];
+            const int j = e / C, c = e - j * C;
+            vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
         }
         const int nl = ts.nch[lab];
         for (int e = threadIdx.x; e < nl * L; e += bd) {
@@ -1049,7 +1049,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
                     if (eh < theta) continue;
-                    const float v = recheck_item_value(ts, vW, L, lab, (uint32_t)u, f, s, n);
+                    const float v = recheck_item_value(ts, vW, L, 1, lab, (uint32_t)u, f, s, n);
                     argmax_merge(bv_best, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
                 }
             }
@@ -1199,9 +1199,9 @@ __global__ __launch_bounds__(kBlock) void k_gemm_pack(TemplateSet ts, PassState 
                                                       _Float16* __restrict__ Vh, _Float16* __restrict__ Vl,
                                                       float* __restrict__ vscale)
 {
+    extern __shared__ float sw[];  // [C][L] the window's voltage, channel rows
     __shared__ float red[kBlock / 64];
     const int C = ts.C, L = ts.L;
-    const int64_t N = st.N;
     const uint32_t nF = st.counters[CNT_F];
     for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
         const uint32_t w = f0 + wi;
@@ -1210,11 +1210,13 @@ __global__ __launch_bounds__(kBlock) void k_gemm_pack(TemplateSet ts, PassState 
         const bool live = w < nF && !window_skipped(ts, st, st.F[w]);
         const int64_t b = live ? (int64_t)st.best_idx[st.F[w]] : 0;
         float m = 0.0f;
-        if (live)
-            for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-                const int c = e / L, j = e - c * L;
-                m = fmaxf(m, fabsf(st.V[(size_t)c * N + b + j]));
-            }
+        __syncthreads();
+        for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+            const int j = e / C, c = e - j * C;
+            const float v = live ? st.V[(size_t)(b + j) * C + c] : 0.0f;
+            sw[c * L + j] = v;
+            m = fmaxf(m, fabsf(v));
+        }
         m = block_max(m, red);
         float sc = 1.0f;
         if (m > 0.0f) {
@@ -1223,17 +1225,12 @@ __global__ __launch_bounds__(kBlock) void k_gemm_pack(TemplateSet ts, PassState 
             sc = ldexpf(1.0f, 14 - ex);  // m * sc < 2^14
         }
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-            float v = 0.0f;
-            if (live) {
-                const int c = e / L, j = e - c * L;
-                v = st.V[(size_t)c * N + b + j] * sc;
-            }
+            const float v = sw[e] * sc;
             const _Float16 hi = (_Float16)v;
             vh[e] = hi;
             vl[e] = (_Float16)(v - (float)hi);
         }
         if (threadIdx.x == 0) vscale[wi] = sc;
-        __syncthreads();
     }
 }
 
@@ -1249,7 +1246,8 @@ void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, u
                       _Float16* Vl, float* vscale, hipStream_t s)
 {
     if (nbatch == 0) return;
-    hipLaunchKernelGGL(k_gemm_pack, dim3(sgrid(nbatch, 1, 1u << 20)), dim3(kBlock), 0, s, ts, st, f0, nbatch, Vh, Vl,
+    hipLaunchKernelGGL(k_gemm_pack, dim3(sgrid(nbatch, 1, 1u << 20)), dim3(kBlock), (size_t)ts.C * ts.L * 4, s, ts, st,
+                       f0, nbatch, Vh, Vl,
                        vscale);
 }
 
@@ -1307,207 +1305,6 @@ void launch_screen_corr(const TemplateSet& ts, const PassState& st, const Screen
     else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
     else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-}
-
-}  // namespace fbp
-
-// ===========================================================================
-// Mixed-precision likelihood.  LIK(t, x0 + x) - bias_t = sum_{c, j} Tt[(t, x)][(c, j)] * seg[c][j]
-// with Tt the template of t placed at offset x inside a segment of P + L - 1
-// samples starting at the window start x0 (Toeplitz expansion, constant) and
-// seg the window's voltage segment: one fp16 GEMM per unit group gives an
-// approximation with |error| <= delta_t = kappa_l * sum_c |T_t[c]|_1 max|seg_c|.
-// Only likelihoods above 0, or their sign, ever decide anything (window maxima
-// start at 0, flags and skips test > 0 or <= 0, the cascade rejects values
-// <= 0), so every position whose approximation exceeds -delta_t is rescored
-// with the exact FP32 chain of k_likelihood and the others keep the
-// approximation, which is negative as their exact value is <= 0.
-// ===========================================================================
-namespace fbp {
-
-__global__ __launch_bounds__(kBlock) void k_likmix_A(TemplateSet ts, const int* __restrict__ units, int nunits, int c_lo,
-                                                     int nch, int P, float tscale, _Float16* __restrict__ Ah)
-{
-    const int L = ts.L, C = ts.C, R = P + L - 1;
-    const int K = nch * R;
-    const int64_t total = (int64_t)nunits * P * K;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = e / K;
-        const int k = (int)(e - r * K);
-        const int u = units[r / P], x = (int)(r % P);
-        const int c = c_lo + k / R, j = k % R;
-        const int i = j - x;
-        float v = 0.0f;
-        if (c < C && i >= 0 && i < L) v = ts.T[((size_t)u * C + c) * L + i] * tscale;
-        Ah[e] = (_Float16)v;
-    }
-}
-
-// Voltage segments [w][C][P+L-1] (fp16, power-of-two scaled per window) of the
-// listed windows W[w0 .. w0+nb).
-__global__ __launch_bounds__(kBlock) void k_likmix_pack(TemplateSet ts, PassState st, uint32_t w0, uint32_t nb, int P,
-                                                        _Float16* __restrict__ seg, float* __restrict__ vscale)
-{
-    __shared__ float red[kBlock / 64];
-    const int C = ts.C, L = ts.L, R = P + L - 1;
-    const int64_t N = st.N;
-    for (uint32_t wi = blockIdx.x; wi < nb; wi += gridDim.x) {
-        const int64_t x0 = (int64_t)st.W[w0 + wi] * L;
-        float m = 0.0f;
-        for (int e = threadIdx.x; e < C * R; e += blockDim.x) {
-            const int c = e / R, j = e - c * R;
-            const int64_t x = x0 + j;
-            if (x < N) m = fmaxf(m, fabsf(st.V[(size_t)c * N + x]));
-        }
-        m = block_max(m, red);
-        float sc = 1.0f;
-        if (m > 0.0f) {
-            int ex;
-            frexpf(m, &ex);
-            sc = ldexpf(1.0f, 14 - ex);
-        }
-        _Float16* sg = seg + (size_t)wi * C * R;
-        for (int e = threadIdx.x; e < C * R; e += blockDim.x) {
-            const int c = e / R, j = e - c * R;
-            const int64_t x = x0 + j;
-            sg[e] = (_Float16)(x < N ? st.V[(size_t)c * N + x] * sc : 0.0f);
-        }
-        if (threadIdx.x == 0) vscale[wi] = sc;
-        __syncthreads();
-    }
-}
-
-// Approximations -> LIK with exact rescoring near and above zero, plus the
-// per-window maxima of k_likelihood.  One block per listed window.
-template <int LT>
-__global__ __launch_bounds__(kBlock) void k_likmix_post(TemplateSet ts, PassState st, uint32_t w0, uint32_t nb, int P,
-                                                        const float* __restrict__ Out, const int* __restrict__ orow,
-                                                        const float* __restrict__ vscale, const float* __restrict__ rowL1,
-                                                        float inv_tscale, float kappa_l, float kappa_b)
-{
-    extern __shared__ float sm[];
-    __shared__ float red[kBlock / 64];
-    __shared__ uint32_t nrescore;
-    const int C = ts.C, U = ts.U;
-    const int L = LT > 0 ? LT : ts.L;
-    const int R = P + L - 1;
-    const int64_t N = st.N;
-    float* seg = sm;             // [C][R] fp32 voltage of the segment
-    float* vmx = seg + C * R;    // [C]
-    float* dl = vmx + C;         // [U] delta_t
-    for (uint32_t wi = blockIdx.x; wi < nb; wi += gridDim.x) {
-        const uint32_t id = st.W[w0 + wi];
-        const int64_t x0 = (int64_t)id * L;
-        __syncthreads();
-        for (int e = threadIdx.x; e < C * R; e += blockDim.x) {
-            const int c = e / R, j = e - c * R;
-            const int64_t x = x0 + j;
-            seg[e] = x < N ? st.V[(size_t)c * N + x] : 0.0f;
-        }
-        if (threadIdx.x == 0) nrescore = 0;
-        __syncthreads();
-        for (int c = threadIdx.x; c < C; c += blockDim.x) {
-            float m = 0.0f;
-            for (int j = 0; j < R; j++) m = fmaxf(m, fabsf(seg[c * R + j]));
-            vmx[c] = m;
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < U; t += blockDim.x) {
-            float q = 0.0f;
-            const int r0 = ts.off[t], nr = ts.nch[t];
-            for (int k = 0; k < nr; k++) q += rowL1[r0 + k] * vmx[ts.chan[r0 + k]];
-            dl[t] = kappa_l * q + kappa_b * fabsf(ts.bias[t]);
-        }
-        __syncthreads();
-        const float sc = inv_tscale / vscale[wi];
-        const float* ow = Out + (size_t)wi * U * P;
-        uint32_t nloc = 0;
-        // one wave per template: lanes over positions, first maximum per window
-        for (int t = threadIdx.x >> 6; t < U; t += kBlock / 64) {
-            float mv = -INFINITY;
-            uint32_t mp = 0xffffu;
-            const float bt = ts.bias[t];
-            const float dt = dl[t];
-            for (int x = lane_id(); x < L; x += 64) {
-                const int64_t xx = x0 + x;
-                if (xx >= N) continue;
-                float v;
-                if (xx + L >= N) {
-                    v = 0.0f;
-                } else {
-                    v = bt + ow[orow[t] + x] * sc;
-                    if (v > -dt) {
-                        // exact FP32 chain, operation order of k_likelihood
-                        float lik = bt;
-                        const int r0 = ts.off[t], nr = ts.nch[t];
-                        for (int k = 0; k < nr; k++) {
-                            const float* trow = ts.rows + (size_t)(r0 + k) * ts.Lp;
-                            const float* vr = seg + ts.chan[r0 + k] * R + x;
-                            for (int i = 0; i < L; i++) lik = fmaf(trow[i], vr[i], lik);
-                        }
-                        v = lik;
-                        nloc++;
-                    }
-                }
-                st.LIK[(size_t)t * N + xx] = v;
-                if (v > mv) { mv = v; mp = (uint32_t)x; }
-            }
-            float m2 = mv;
-            uint32_t p2 = mp;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const float vo = __shfl_xor(m2, off, 64);
-                const uint32_t po = __shfl_xor(p2, off, 64);
-                if (vo > m2 || (vo == m2 && po < p2)) { m2 = vo; p2 = po; }
-            }
-            if (lane_id() == 0) {
-                st.wmax_v[(size_t)id * U + t] = m2;
-                st.wmax_p[(size_t)id * U + t] = (uint16_t)p2;
-            }
-        }
-        if (nloc) atomicAdd(&nrescore, nloc);
-        __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(st.counters + CNT_LIKX, nrescore);
-    }
-}
-
-void launch_likmix_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int nch, int P, float tscale,
-                     _Float16* Ah, hipStream_t s)
-{
-    const int64_t total = (int64_t)nunits * P * nch * (P + ts.L - 1);
-    hipLaunchKernelGGL(k_likmix_A, dim3(sgrid(total, kBlock, 1u << 20)), dim3(kBlock), 0, s, ts, units, nunits, c_lo, nch,
-                       P, tscale, Ah);
-}
-
-void launch_likmix_pack(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, _Float16* seg,
-                        float* vscale, hipStream_t s)
-{
-    if (nb == 0) return;
-    hipLaunchKernelGGL(k_likmix_pack, dim3(sgrid(nb, 1, 1u << 20)), dim3(kBlock), 0, s, ts, st, w0, nb, P, seg, vscale);
-}
-
-size_t likmix_post_lds(const TemplateSet& ts, int P) { return (size_t)(ts.C * (P + ts.L - 1) + ts.C + ts.U) * 4; }
-
-void launch_likmix_post(const TemplateSet& ts, const PassState& st, uint32_t w0, uint32_t nb, int P, const float* Out,
-                        const int* orow, const float* vscale, const float* rowL1, float inv_tscale, float kappa_l,
-                        float kappa_b, hipStream_t s)
-{
-    if (nb == 0) return;
-    const dim3 g(sgrid(nb, 1, 1u << 20)), b(kBlock);
-    const size_t lds = likmix_post_lds(ts, P);
-    if (ts.L == 48) {
-        allow_lds(k_likmix_post<48>, lds);
-        hipLaunchKernelGGL(k_likmix_post<48>, g, b, lds, s, ts, st, w0, nb, P, Out, orow, vscale, rowL1, inv_tscale,
-                           kappa_l, kappa_b);
-    } else if (ts.L == 64) {
-        allow_lds(k_likmix_post<64>, lds);
-        hipLaunchKernelGGL(k_likmix_post<64>, g, b, lds, s, ts, st, w0, nb, P, Out, orow, vscale, rowL1, inv_tscale,
-                           kappa_l, kappa_b);
-    } else {
-        allow_lds(k_likmix_post<0>, lds);
-        hipLaunchKernelGGL(k_likmix_post<0>, g, b, lds, s, ts, st, w0, nb, P, Out, orow, vscale, rowL1, inv_tscale,
-                           kappa_l, kappa_b);
-    }
 }
 
 }  // namespace fbp

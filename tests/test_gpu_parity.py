@@ -110,6 +110,18 @@ def test_mixed_precision_workloads_match_oracle(handle, name, dur):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
+@pytest.mark.parametrize("C,L,U,N", [(2, 8, 3, 600), (5, 20, 17, 1500), (9, 33, 40, 2000), (12, 48, 16, 3000),
+                                     (17, 64, 33, 2500), (3, 16, 1, 400), (64, 48, 20, 1200)])
+def test_mixed_precision_shapes_match_oracle(handle, C, L, U, N):
+    """Tile padding (U not a multiple of 16), channel padding (C not a multiple
+    of 8), positions past L within the last 16-column tile, dense templates and
+    windows at the end of the chunk."""
+    V, T, bias, thr, on, lab = cases.noisy_case(7 + C, C=C, L=L, U=U, N=N, nsp=max(4, N // 25))
+    n = L - 1
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
 def test_mixed_precision_rescores_few_positions(handle):
     d = synthetic.generate("probe64", duration_s=0.5)
     T, bias, thr = synthetic.kernel_inputs(d)

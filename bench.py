@@ -91,7 +91,8 @@ def main():
     T3, bias, thr = synthetic.kernel_inputs(d)
     h.set_templates(T3, bias, thr)
     base = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_TIMING | (_native.FLAG_RECHECK_LITERAL if args.literal else 0)
-    h.run(None, d["n_max_shift_inds"], base, ld=V.stride(0), n_samples=N, device_ptr=V.data_ptr())
+    h.run(None, d["n_max_shift_inds"], base | _native.FLAG_SCREEN_FP32, ld=V.stride(0), n_samples=N,
+          device_ptr=V.data_ptr())
     st = h.stats()
     log("kernel stats fp32:", json.dumps(st))
     h.run(None, d["n_max_shift_inds"], base | _native.FLAG_PRECISION_MIXED, ld=V.stride(0), n_samples=N,
@@ -100,21 +101,38 @@ def main():
     log("kernel stats mixed:", json.dumps(stm))
     classes = {"likelihood": stm["ms_likelihood"], "recheck": stm["ms_recheck"], "window_max": stm["ms_window_max"],
                "pursuit": stm["ms_pursuit"], "residual": stm["ms_residual"], "other": stm["ms_other"]}
-    dominant = max(classes, key=classes.get)
+    S = 2 * int(d["n_max_shift_inds"]) + 1
+    # dominant kernel of the mixed path: the overlap screen selection.  Its
+    # algorithmic traffic per flagged window is B(u, s) and the sign of
+    # LIK[u][b+s-n] for every (unit, shift) pair plus the window's voltage.
+    sel_bytes_per_window = U * S * 8 + C * L * 4
+    sel_launch_ms = stm["ms_select"] / max(1, stm["n_select_launches"])
+    sel_bytes_per_launch = stm["windows_flagged"] * sel_bytes_per_window / max(1, stm["n_select_launches"])
+    sel_gbs = sel_bytes_per_launch / (sel_launch_ms * 1e-3) / 1e9 if sel_launch_ms > 0 else 0.0
+    mf_ms = stm["ms_likelihood"] - stm["ms_likelihood_exact"]
+    mf_tflops = stm["likelihood_flops"] / (mf_ms * 1e-3) / 1e12 if mf_ms > 0 else 0.0
     lik_avg_ms = st["ms_likelihood"] / max(1, st["n_likelihood_launches"])
     lik_tflops = st["likelihood_flops"] / (st["ms_likelihood"] * 1e-3) / 1e12 if st["ms_likelihood"] > 0 else 0.0
     roofline = {
-        "kernel": "k_likelihood (FP32 path)",
-        "bound": "mfma",
-        "achieved": round(lik_tflops, 3),
-        "peak": FP32_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(lik_tflops / FP32_PEAK_TFLOPS, 4),
+        "kernel": "k_screen_select_reg (overlap screen selection)",
+        "bound": "hbm",
+        "achieved": round(sel_gbs, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(sel_gbs / HBM_PEAK_GBS, 5),
         "traffic": None,
-        "avg_launch_ms": round(lik_avg_ms, 4),
-        "flops_per_launch": st["likelihood_flops"] / max(1, st["n_likelihood_launches"]),
-        "dominant_class_mixed": dominant,
+        "avg_launch_ms": round(sel_launch_ms, 4),
+        "algorithmic_bytes_per_window": sel_bytes_per_window,
+        "bytes_per_launch": round(sel_bytes_per_launch),
         "ms_by_class_mixed": {k: round(v, 3) for k, v in classes.items()},
+        "ms_select_mixed": round(stm["ms_select"], 3),
+        "likelihood_mfma": {"kernel": "k_likmfma (fp16 MFMA)", "achieved_tflops_equiv": round(mf_tflops, 2),
+                            "peak": 2500.0, "ms": round(mf_ms, 3),
+                            "exact_pass_ms": round(stm["ms_likelihood_exact"], 3),
+                            "note": "algorithmic FP32 flops of the likelihood / MFMA kernel time"},
+        "likelihood_fp32": {"kernel": "k_likelihood (FP32 path)", "achieved_tflops": round(lik_tflops, 3),
+                            "peak": FP32_PEAK_TFLOPS, "frac": round(lik_tflops / FP32_PEAK_TFLOPS, 4),
+                            "avg_launch_ms": round(lik_avg_ms, 4)},
         "passes": stm["passes"],
     }
 
@@ -149,7 +167,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f16 GEMM + f32 (decisions in exact f32)",
+        "dtype": "f16 MFMA + f32 (decisions in exact f32)",
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: {C} ch, {w.fs / 1000:g} kHz, {N / w.fs:g} s, {U} units, L={L}, "

@@ -68,6 +68,8 @@ struct MfTables {
 void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s);
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
                     const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s);
+void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
+                       const uint8_t* fmask, hipStream_t s);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -247,7 +249,7 @@ static PassState pass_view(const fbp_handle* h)
 struct Timer {
     hipStream_t s;
     bool on;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[6];
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[8];
     Timer(hipStream_t st, bool enable) : s(st), on(enable) {}
     void begin(int cls)
     {
@@ -282,7 +284,7 @@ struct Timer {
             }
     }
 };
-enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER };
+enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT, TM_LIKX };
 
 static ScreenTables tables_view(const fbp_handle* h)
 {
@@ -892,9 +894,14 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         tm.begin(TM_LIK);
         if (mixed) {
             const MfTables mt = mf_view(h);
-            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch)
+            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
+                const uint32_t nb = std::min(lik_batch, nW - w0);
                 launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(), w0,
-                               std::min(lik_batch, nW - w0), h->mf_mask.as<uint8_t>(), s);
+                               nb, h->mf_mask.as<uint8_t>(), s);
+                tm.begin(TM_LIKX);
+                likelihood_masked(ts, st, st.W + w0, nb, h->mf_mask.as<uint8_t>(), s);
+                tm.end(TM_LIKX);
+            }
         } else {
             launch_likelihood(ts, st, nW, s);
         }
@@ -926,8 +933,11 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                         } else {
                             launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
                         }
+                        tm.begin(TM_SELECT);
                         launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(),
                                              (flags & FBP_FLAG_SELECT_STREAMING) != 0, s);
+                        tm.end(TM_SELECT);
+                        stats.n_select_launches++;
                     }
                 } else {
                     HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
@@ -983,6 +993,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.ms_pursuit = tm.total(TM_PURSUIT);
     stats.ms_residual = tm.total(TM_RESID);
     stats.ms_other = tm.total(TM_OTHER);
+    stats.ms_select = tm.total(TM_SELECT);
+    stats.ms_likelihood_exact = tm.total(TM_LIKX);
     HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
     stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.recheck_overflow = h->h_counters[CNT_CAND];

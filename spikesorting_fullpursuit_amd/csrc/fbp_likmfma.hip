@@ -326,12 +326,9 @@ static void allow_lds_mf(K kernel, size_t bytes)
                                   (int)bytes);
 }
 
-void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
-                       const uint8_t* fmask, hipStream_t s);
-
-// MFMA likelihood of the listed windows W[w0 .. w0+nb), then the exact FP32
-// chain for the 16-position tiles it flags (fmask, one byte per window and
-// template of the batch).
+// MFMA likelihood of the listed windows W[w0 .. w0+nb); fmask receives, per
+// window of the batch and template, the 16-position tiles that need the exact
+// FP32 chain (likelihood_masked).
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
                     const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s)
 {
@@ -353,7 +350,6 @@ void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& 
         allow_lds_mf(k_likmfma<1>, lds);
         hipLaunchKernelGGL(k_likmfma<1>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
     }
-    likelihood_masked(ts, st, st.W + w0, nb, fmask, s);
 }
 
 }  // namespace fbp

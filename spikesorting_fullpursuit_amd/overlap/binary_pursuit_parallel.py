@@ -89,10 +89,11 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     (there is no OpenCL source).  Keyword-only extras: `return_residual` appends
     the list of per-chunk residual voltages; `stats` (a list) receives the
     device statistics of every chunk; `precision` "mixed" (default) computes the
-    likelihood with an fp16 GEMM and rescores in exact FP32 every value that
-    can influence a decision, "fp32" computes every value in FP32 -- both give
-    the same spikes and residual; `recheck_literal` evaluates every overlap
-    item instead of screening them.
+    likelihood and the overlap screen's correlations with fp16 matrix-core
+    products and recomputes in exact FP32 every value that can influence a
+    decision, "fp32" computes both on the FP32 vector units -- the two give the
+    same spikes and residual; `recheck_literal` evaluates every overlap item
+    instead of screening them.
     """
     if precision not in ("mixed", "fp32"):
         raise ValueError("precision must be 'mixed' or 'fp32'")
@@ -136,6 +137,8 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
         flags |= _native.FLAG_RECHECK_LITERAL
     if precision == "mixed":
         flags |= _native.FLAG_PRECISION_MIXED
+    else:
+        flags |= _native.FLAG_SCREEN_FP32
 
     on_device = _is_device_tensor(voltage)
     if on_device:

@@ -42,6 +42,7 @@ void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, con
 void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s);
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
+int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
                         float* Bbuf, hipStream_t s);
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
@@ -318,6 +319,7 @@ static ScreenTables tables_view(const fbp_handle* h)
     tb.brow = h->brow.as<int>();
     tb.vscale = h->vscale.as<float>();
     tb.inv_tscale = 1.0f / h->tscale;
+    tb.sel_region = screen_select_region(template_view(h), tb.S, tb.cmax, tb.npl_max);
     return tb;
 }
 

@@ -60,6 +60,7 @@ struct ScreenTables {
     float* minGb;         // [npairs][S][nbg] min over each block of kGB consecutive f
     int nbg;              // blocks of f: ceil(S / kGB)
     int npl_max;          // most units sharing a channel with one unit
+    int sel_region;       // floats of the register-resident selection's phase-dependent LDS region
     float* Ntr;           // [U][S]          energy of the window-truncated template
     const float* rowL1;   // [R] L1 norm of each compact template row (rounded up)
     const float* SS;      // [U] sum of squares of each template (rounded up)

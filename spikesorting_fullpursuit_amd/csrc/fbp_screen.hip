@@ -238,6 +238,53 @@ __device__ __forceinline__ float exact_item_rows(const float* vW, int Lrt, const
     return lik;
 }
 
+// exact_item_rows over zero-padded rows (row length LP = L + 2n, P[i] = T[i - n]):
+// the label's term at sample j is TlP[row][2n - f + j], the unit's
+// Tu[row][2n - s + j], an absent channel reads the zero row.  Same values and
+// operation order as exact_item_rows, without per-sample range tests.
+template <int LT>
+__device__ __forceinline__ float exact_item_padded(const float* vW, int Lrt, const float* TlP, const int* cl, int nl,
+                                                   const float* Tu, const int* cu, int nu, const float* zrow, int LP,
+                                                   int f, int s, int n)
+{
+    const int L = LT > 0 ? LT : Lrt;
+    const int of = 2 * n - f, os = 2 * n - s;
+    float lik = 0.0f;
+    int il = 0, iu = 0;
+    while (il < nl || iu < nu) {
+        const int c1 = il < nl ? cl[il] : 0x7fffffff;
+        const int c2 = iu < nu ? cu[iu] : 0x7fffffff;
+        const int c = min(c1, c2);
+        const bool hf = c1 == c, hs = c2 == c;
+        const float* tf = (hf ? TlP + il * LP : zrow) + of;
+        const float* tsr = (hs ? Tu + iu * LP : zrow) + os;
+        il += hf;
+        iu += hs;
+        const float* vr = vW + c * L;
+        float ss = 0.0f;
+        for (int j0 = 0; j0 < L; j0 += 8) {
+            float a[8], b[8], vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int j = (LT > 0 || j0 + q < L) ? j0 + q : 0;
+                a[q] = tf[j];
+                b[q] = tsr[j];
+                vv[q] = vr[j];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // keep the 24 loads ahead of the chains
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (LT == 0 && j0 + q >= L) break;
+                const float sum = a[q] + b[q];
+                ss = fmaf(sum, sum, ss);
+                lik = fmaf(sum, vv[q], lik);
+            }
+        }
+        if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
+    }
+    return lik;
+}
+
 constexpr int kCandCap = 1024;
 
 // e = q * S + r for 0 <= e < 2^24 with a float reciprocal and one correction.
@@ -610,7 +657,7 @@ constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
 static_assert(kSelR <= 32, "pair masks are 32-bit");
 constexpr int kSelMaxBlock = 1024;  // threads per block at most (U * S <= kSelR * kSelMaxBlock)
 constexpr size_t kSelMaxLds = 96 * 1024;  // dynamic LDS of the register-resident selection at most
-constexpr int kStageRows = 128;     // template rows of candidate units staged in LDS for rescoring
+constexpr int kStageRegion = 8192;  // floats of the selection's phase-dependent LDS region at least
 
 __device__ __forceinline__ float block_max_n(float v, float* red, int nw)
 {
@@ -671,17 +718,22 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     float* SMp = SMa + S;
     float* Q = SMp + S;                   // [U]
     float* D = Q + U;                     // [U] per-unit bound delta
-    float* Tl = D + U;                    // [CM][L] rows of the window's label
-    int* cl = reinterpret_cast<int*>(Tl + CM * L);  // [CM]
+    const int LP = L + 2 * n;             // zero-padded row: P[i] = T[i - n], i - n outside [0, L) -> 0
+    const int RG = tb.sel_region;         // floats of the phase-dependent region
+    float* TlP = D + U;                   // [CM][LP] padded rows of the window's label
+    float* zrow = TlP + CM * LP;          // [LP] zeros (a template channel that is absent)
+    int* cl = reinterpret_cast<int*>(zrow + LP);  // [CM]
     int* pidw = cl + CM;                  // [U] pair id of (label, u); -1: no shared channel
     int* browl = pidw + U;                // [U] GEMM row of (u, s = 0)
-    float* gmB = reinterpret_cast<float*>(browl + U);  // [npl][S] least G over the blocks of f with LIK[l] > 0
-    float* stg = gmB + tb.npl_max * S;    // [kStageRows][L] candidate units' rows; survivor list before that
-    int* stg_ch = reinterpret_cast<int*>(stg + kStageRows * L);  // [kStageRows] channel of each staged row
-    int* slot_of = stg_ch + kStageRows;   // [U] first staged row of unit u (-1: not staged)
-    uint32_t* surv_e = reinterpret_cast<uint32_t*>(stg);  // [survcap] pair of each listed survivor
-    float* surv_b = stg + kStageRows * L / 2;             // [survcap] its B
-    const uint32_t survcap = (uint32_t)(kStageRows * L / 2);
+    float* reg = reinterpret_cast<float*>(browl + U);  // [RG], in turn:
+    float* gmB = reg;                     //   [npl][S] least G over the blocks of f with LIK[l] > 0
+    uint32_t* surv_e = reinterpret_cast<uint32_t*>(reg);  //   survivor lists: pair ...
+    float* surv_b = reg + RG / 2;                          //   ... and its B
+    const uint32_t survcap = (uint32_t)(RG / 2);
+    float* stgP = reg;                    //   [srows][LP] padded rows of the candidates' units
+    const int srows = RG / LP;
+    int* stg_ch = reinterpret_cast<int*>(reg + RG);  // [RG / LP] channel of each staged row
+    int* slot_of = stg_ch + RG / LP;      // [U] order of unit u among the staged units (-1: none)
     uint8_t* fin = reinterpret_cast<uint8_t*>(slot_of + U);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
     const int wv = threadIdx.x >> 6, lane = lane_id();
@@ -716,10 +768,11 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
         }
         const int nl = ts.nch[lab];
-        for (int e = threadIdx.x; e < nl * L; e += bd) {
-            const int k = e / L, j = e - k * L;
-            Tl[e] = ts.rows[(size_t)(ts.off[lab] + k) * ts.Lp + j];
+        for (int e = threadIdx.x; e < nl * LP; e += bd) {
+            const int k = e / LP, i = e - k * LP;
+            TlP[e] = (i >= n && i < n + L) ? ts.rows[(size_t)(ts.off[lab] + k) * ts.Lp + (i - n)] : 0.0f;
         }
+        for (int i = threadIdx.x; i < LP; i += bd) zrow[i] = 0.0f;
         for (int k = threadIdx.x; k < nl; k += bd) cl[k] = ts.chan[ts.off[lab] + k];
         if (threadIdx.x == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
         __syncthreads();
@@ -1055,47 +1108,45 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             }
         } else {
             // candidates reaching theta_u, each rescored exactly by one thread;
-            // the rows of their units are staged in LDS first (each unit once,
-            // while kStageRows rows last)
+            // the zero-padded rows of their units are staged in LDS, upr units
+            // per round
             for (int u = threadIdx.x; u < U; u += bd) slot_of[u] = -1;
             if (threadIdx.x == 0) nrow_s = 0;
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < nc; i += bd) {
                 const int u = (int)(cand_k[i] % (uint32_t)U);
                 if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
-                if (atomicCAS(&slot_of[u], -1, -2) == -1) {
-                    const int nr = ts.nch[u];
-                    const int r = (int)atomicAdd(&nrow_s, (uint32_t)nr);
-                    slot_of[u] = (r + nr <= kStageRows) ? r : -3;
-                }
+                if (atomicCAS(&slot_of[u], -1, -2) == -1) slot_of[u] = (int)atomicAdd(&nrow_s, 1u);
             }
             __syncthreads();
-            for (int u = wv; u < U; u += nw) {
-                const int sl = slot_of[u];
-                if (sl < 0) continue;
-                const int r0 = ts.off[u], nr = ts.nch[u];
-                for (int e = lane; e < nr * L; e += 64) {
-                    const int k = e / L, j = e - k * L;
-                    stg[(sl + k) * L + j] = ts.rows[(size_t)(r0 + k) * ts.Lp + j];
+            const int nun = (int)nrow_s;
+            const int upr = max(1, srows / CM);
+            for (int r0u = 0; r0u < nun; r0u += upr) {
+                if (r0u > 0) __syncthreads();  // the previous round's chains are done
+                for (int u = wv; u < U; u += nw) {
+                    const int o = slot_of[u];
+                    if (o < r0u || o >= r0u + upr) continue;
+                    const int rb = (o - r0u) * CM, r0 = ts.off[u], nr = ts.nch[u];
+                    for (int e = lane; e < nr * LP; e += 64) {
+                        const int k = e / LP, i = e - k * LP;
+                        stgP[(rb + k) * LP + i] =
+                            (i >= n && i < n + L) ? ts.rows[(size_t)(r0 + k) * ts.Lp + (i - n)] : 0.0f;
+                    }
+                    for (int k = lane; k < nr; k += 64) stg_ch[rb + k] = ts.chan[r0 + k];
                 }
-                for (int k = lane; k < nr; k += 64) stg_ch[sl + k] = ts.chan[r0 + k];
-            }
-            __syncthreads();
-            for (uint32_t i = threadIdx.x; i < nc; i += bd) {
-                const uint32_t k = cand_k[i];
-                const int u = (int)(k % (uint32_t)U);
-                const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
-                if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
-                const int sl = slot_of[u];
-                float v;
-                if (sl >= 0) {
-                    v = exact_item_rows<LT>(vW, L, Tl, cl, nl, stg + sl * L, L, stg_ch + sl, ts.nch[u], f, s, n);
-                } else {
-                    const int r0 = ts.off[u];
-                    v = exact_item_rows<LT>(vW, L, Tl, cl, nl, ts.rows + (size_t)r0 * ts.Lp, ts.Lp, ts.chan + r0,
-                                            ts.nch[u], f, s, n);
+                __syncthreads();
+                for (uint32_t i = threadIdx.x; i < nc; i += bd) {
+                    const uint32_t k = cand_k[i];
+                    const int u = (int)(k % (uint32_t)U);
+                    const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
+                    if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                    const int o = slot_of[u];
+                    if (o < r0u || o >= r0u + upr) continue;
+                    const int rb = (o - r0u) * CM;
+                    const float v = exact_item_padded<LT>(vW, L, TlP, cl, nl, stgP + rb * LP, stg_ch + rb, ts.nch[u],
+                                                          zrow, LP, f, s, n);
+                    argmax_merge(bv_best, bk, v, k);
                 }
-                argmax_merge(bv_best, bk, v, k);
             }
         }
         wave_argmax(bv_best, bk);
@@ -1147,10 +1198,21 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
            2 * (size_t)S + 16;
 }
 
+// Floats of the selection's phase-dependent region: the blocked Gram minima of
+// one label, the survivor lists, or the staged padded rows.
+int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max)
+{
+    const int LP = ts.L + S - 1;
+    int rg = std::max(npl_max * S, kStageRegion);
+    rg = std::max(rg, cmax * LP);  // one unit's rows at least
+    return (rg + 3) & ~3;
+}
+
 size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max)
 {
-    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 5 * ts.U + cmax * ts.L + cmax + (size_t)npl_max * S +
-                    (size_t)kStageRows * (ts.L + 1)) * 4 +
+    const int LP = ts.L + S - 1;
+    const int rg = screen_select_region(ts, S, cmax, npl_max);
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 5 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
            2 * (size_t)S + 16;
 }
 

@@ -41,7 +41,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the time-sharded multi-GPU design is described in DESIGN.md)"}))
+                                   "(the path would shard along time with one all-gather of the spike lists)"}))
         sys.exit(2)
 
     import torch

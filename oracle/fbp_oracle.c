@@ -26,7 +26,7 @@
  *   * residual  V[c,p+i] = V[c,p+i] - T[u,c,i] in spike-list order (binary_pursuit.cl:93-97).
  *
  * Phase-separated meaning adopted where the reference races or is
- * scheduling dependent (each is listed in DESIGN.md):
+ * scheduling dependent:
  *   * overlap arg-max ties go to the lowest linear item id k (the reference's
  *     work-group tree reduction, binary_pursuit.cl:167-186, picks a
  *     work-group-size dependent winner among exact ties);

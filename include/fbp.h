@@ -67,7 +67,6 @@ typedef struct fbp_stats {
     int64_t screen_shared_pairs;  /* (unit, shift) pairs sharing a channel enumerated over f by the screen */
     int64_t screen_separable_pairs; /* (unit, shift) pairs sharing no channel enumerated over f */
     double ms_select;             /* overlap screen selection kernel (part of ms_recheck), with FBP_FLAG_TIMING */
-    double ms_likelihood_exact;   /* mixed precision: exact FP32 pass over flagged tiles (part of ms_likelihood) */
     int64_t n_select_launches;
 } fbp_stats;
 

@@ -58,7 +58,6 @@ class FbpStats(ctypes.Structure):
         ("screen_shared_pairs", ctypes.c_int64),
         ("screen_separable_pairs", ctypes.c_int64),
         ("ms_select", ctypes.c_double),
-        ("ms_likelihood_exact", ctypes.c_double),
         ("n_select_launches", ctypes.c_int64),
     ]
 

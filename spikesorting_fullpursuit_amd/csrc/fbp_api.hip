@@ -68,9 +68,7 @@ struct MfTables {
 };
 void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s);
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
-                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s);
-void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
-                       const uint8_t* fmask, hipStream_t s);
+                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, hipStream_t s);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -166,7 +164,7 @@ struct fbp_handle {
     // mixed-precision likelihood: Toeplitz template matrices per group
     bool mf_built = false;
     // MFMA likelihood: A fragments of 16-template tiles and the rescoring list
-    DevBuf mf_frag, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mask, mf_mbits, mf_vs;
+    DevBuf mf_frag, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs;
     int mf_ntiles = 0;
     float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
     bool mf_ok = false;
@@ -285,7 +283,7 @@ struct Timer {
             }
     }
 };
-enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT, TM_LIKX };
+enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT };
 
 static ScreenTables tables_view(const fbp_handle* h)
 {
@@ -672,7 +670,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale,
                       &h->mf_frag, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
-                      &h->mf_mask, &h->mf_mbits, &h->mf_vs};
+                      &h->mf_mbits, &h->mf_vs};
     for (DevBuf* b : bufs) b->release();
     if (h->rb) (void)rocblas_destroy_handle(h->rb);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
@@ -833,7 +831,6 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         }
         mixed = h->mf_ok;  // outside the MFMA tables' limits the FP32 likelihood runs (same results)
     }
-    if (mixed) HIP_TRY(h->mf_mask.ensure((size_t)lik_batch * U), "alloc likelihood masks");
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
     hipEvent_t ev0, ev1;
     HIP_TRY(hipEventCreate(&ev0), "event");
@@ -896,14 +893,9 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         tm.begin(TM_LIK);
         if (mixed) {
             const MfTables mt = mf_view(h);
-            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
-                const uint32_t nb = std::min(lik_batch, nW - w0);
+            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch)
                 launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(), w0,
-                               nb, h->mf_mask.as<uint8_t>(), s);
-                tm.begin(TM_LIKX);
-                likelihood_masked(ts, st, st.W + w0, nb, h->mf_mask.as<uint8_t>(), s);
-                tm.end(TM_LIKX);
-            }
+                               std::min(lik_batch, nW - w0), s);
         } else {
             launch_likelihood(ts, st, nW, s);
         }
@@ -996,7 +988,6 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.ms_residual = tm.total(TM_RESID);
     stats.ms_other = tm.total(TM_OTHER);
     stats.ms_select = tm.total(TM_SELECT);
-    stats.ms_likelihood_exact = tm.total(TM_LIKX);
     HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
     stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.recheck_overflow = h->h_counters[CNT_CAND];

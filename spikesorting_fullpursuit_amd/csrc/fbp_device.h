@@ -161,6 +161,34 @@ __device__ __forceinline__ void lik_row_fixed_bounded(const float* __restrict__ 
     lik_row_core<LT>(trow, r, acc);
 }
 
+// Same, the voltage row read from sample-major memory: sample j at vcol[j * stride],
+// zeros from sample lim on.
+template <int LT>
+__device__ __forceinline__ void lik_row_strided(const float* __restrict__ trow, const float* __restrict__ vcol,
+                                                int stride, int64_t lim, float* acc)
+{
+    float r[LT + kPos - 1];
+#pragma unroll
+    for (int j = 0; j < LT + kPos - 1; j++) r[j] = (j < lim) ? vcol[(size_t)j * stride] : 0.0f;
+    lik_row_core<LT>(trow, r, acc);
+}
+
+__device__ __forceinline__ void lik_row_generic_strided(const float* __restrict__ trow, const float* __restrict__ vcol,
+                                                        int stride, int L, int64_t lim, float* acc)
+{
+    float r[kPos];
+#pragma unroll
+    for (int q = 0; q < kPos; q++) r[q] = (q < lim) ? vcol[(size_t)q * stride] : 0.0f;
+    for (int i = 0; i < L; i++) {
+        const float tv = trow[i];
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
+        r[kPos - 1] = (i + kPos < lim) ? vcol[(size_t)(i + kPos) * stride] : 0.0f;
+    }
+}
+
 __device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, const float* vrow, int L, float* acc)
 {
     float r[kPos];

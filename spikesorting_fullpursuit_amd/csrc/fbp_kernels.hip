@@ -18,19 +18,13 @@ namespace fbp {
 // register window over the voltage row staged in LDS.
 // ---------------------------------------------------------------------------
 
-// With fmask (one byte per listed window and template, bit p: positions
-// 16p .. 16p+15) only the groups of flagged tiles are computed, and the window
-// maximum of a pair covers its flagged groups; pairs without a flag are left
-// as they are.
 template <int LT>
 __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
                                                        const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
                                                        float* __restrict__ LIK, float* __restrict__ wmax_v,
-                                                       uint16_t* __restrict__ wmax_p,
-                                                       const uint8_t* __restrict__ fmask, uint32_t* __restrict__ npos)
+                                                       uint16_t* __restrict__ wmax_p)
 {
-    extern __shared__ float sV[];  // [wb][C][sv], per-item maxima [wb][U][ngr] (value, position), item list
-    __shared__ uint32_t nl_s, np_s;
+    extern __shared__ float sV[];  // [wb][C][sv], then per-item maxima [wb][U][ngr] (value, position)
     const int C = ts.C, U = ts.U;
     const int L = (LT > 0) ? LT : ts.L;
     const int ngr = (L + kPos - 1) / kPos;
@@ -38,31 +32,9 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     const uint32_t nbatch = (nW + wb - 1) / wb;
     float* pv = sV + wb * C * sv;
     uint32_t* pp = reinterpret_cast<uint32_t*>(pv + wb * per_win);
-    uint32_t* ilist = pp + wb * per_win;
     for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
         const uint32_t w0 = bt * wb;
         const int nb = (int)min((uint32_t)wb, nW - w0);
-        int nitems = nb * per_win;
-        if (fmask) {
-            __syncthreads();
-            if (threadIdx.x == 0) { nl_s = 0; np_s = 0; }
-            __syncthreads();
-            for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
-                const int b = it / per_win;
-                const int rem = it - b * per_win;
-                const int t = rem / ngr, g = rem - t * ngr;
-                pv[it] = -INFINITY;
-                pp[it] = 0xffffu;
-                if ((fmask[(size_t)(w0 + b) * U + t] >> ((g * kPos) / 16)) & 1u) {
-                    ilist[atomicAdd(&nl_s, 1u)] = it;
-                    atomicAdd(&np_s, (uint32_t)min(kPos, L - g * kPos));
-                }
-            }
-            __syncthreads();
-            nitems = (int)nl_s;
-            if (threadIdx.x == 0 && np_s) atomicAdd(npos, np_s);
-            if (nitems == 0) continue;  // uniform over the block
-        }
         __syncthreads();
         // V is sample-major ([N][C]): read along channels, store channel rows
         for (int e = threadIdx.x; e < nb * C * sv; e += blockDim.x) {
@@ -73,8 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
             sV[(b * C + c) * sv + j] = (x < N) ? V[(size_t)x * C + c] : 0.0f;
         }
         __syncthreads();
-        for (int ii = threadIdx.x; ii < nitems; ii += blockDim.x) {
-            const int it = fmask ? (int)ilist[ii] : ii;
+        for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
             const int b = it / per_win;
             const int rem = it - b * per_win;
             const int t = rem / ngr, g = rem - t * ngr;
@@ -111,7 +82,6 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
         // per (window, template): first position of the window maximum
         for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
             const int b = pr / U, t = pr - b * U;
-            if (fmask && fmask[(size_t)(w0 + b) * U + t] == 0) continue;
             float mv = -INFINITY;
             uint32_t mp = 0xffffu;
             for (int g = 0; g < ngr; g++) {
@@ -606,7 +576,7 @@ void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N,
 static int64_t likelihood_lds_per_window(const TemplateSet& ts, int sv)
 {
     const int ngr = (ts.L + kPos - 1) / kPos;
-    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 12;
+    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 8;
 }
 
 void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
@@ -629,8 +599,7 @@ bool likelihood_fits(const TemplateSet& ts)
     return wb * likelihood_lds_per_window(ts, sv) <= 64 * 1024;
 }
 
-static void likelihood_launch(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
-                              const uint8_t* fmask, hipStream_t s)
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
 {
     if (nW == 0) return;
     int wb, sv;
@@ -640,31 +609,15 @@ static void likelihood_launch(const TemplateSet& ts, const PassState& st, const 
     const dim3 g(grid_for(nbatch, 1, 1u << 20)), b(kBlock);
     const float* V = st.V;
     const int64_t N = st.N;
+    const uint32_t* W = st.W;
     if (ts.L == 48)
-        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
-                           st.counters + CNT_LIKX);
+        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
     else if (ts.L == 64)
-        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
-                           st.counters + CNT_LIKX);
+        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
     else if (ts.L == 32)
-        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
-                           st.counters + CNT_LIKX);
+        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
     else
-        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p, fmask,
-                           st.counters + CNT_LIKX);
-}
-
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
-{
-    likelihood_launch(ts, st, st.W, nW, nullptr, s);
-}
-
-// Exact FP32 likelihood of the 16-position tiles flagged in fmask (one byte per
-// window W[0 .. nW) and template).
-void likelihood_masked(const TemplateSet& ts, const PassState& st, const uint32_t* W, uint32_t nW,
-                       const uint8_t* fmask, hipStream_t s)
-{
-    likelihood_launch(ts, st, W, nW, fmask, s);
+        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
 }
 
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)

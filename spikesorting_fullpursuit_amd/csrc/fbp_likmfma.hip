@@ -18,8 +18,8 @@
 // flags and skips test > 0 or <= 0, the parse cascade rejects values <= 0), so
 // a 16-position tile of a (window, template) pair whose every approximation is
 // <= -delta keeps the approximations (all negative, as are the exact values),
-// and every other tile is recomputed with the exact FP32 chain by the masked
-// pass of k_likelihood.
+// and every other tile is recomputed by the same block with the exact FP32
+// chain of k_likelihood.
 #include <math.h>
 
 #include <algorithm>
@@ -70,13 +70,29 @@ __global__ void k_vscale(const uint32_t* __restrict__ mbits, float* __restrict__
     *vs = (m > 0.0f && isfinite(m)) ? ldexpf(1.0f, 14 - e) : 1.0f;
 }
 
-template <int NP>
+// (value, position) -> 64-bit key whose maximum is the largest value, then the
+// smallest position.
+__device__ __forceinline__ unsigned long long max_key(float v, uint32_t p)
+{
+    uint32_t b = __float_as_uint(v);
+    b = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ((unsigned long long)b << 32) | (0xffffffffu - p);
+}
+__device__ __forceinline__ void key_max(unsigned long long key, float& v, uint32_t& p)
+{
+    uint32_t b = (uint32_t)(key >> 32);
+    b = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+    v = __uint_as_float(b);
+    p = 0xffffffffu - (uint32_t)(key & 0xffffffffu);
+}
+
+template <int NP, int LT>
 __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_likmfma(TemplateSet ts, PassState st, MfTables mt,
                                                         const float* __restrict__ rowL1, const float* __restrict__ vsp,
-                                                        const uint32_t* __restrict__ mbits, uint32_t w0, uint32_t nb,
-                                                        uint8_t* __restrict__ fmask)
+                                                        const uint32_t* __restrict__ mbits, uint32_t w0, uint32_t nb)
 {
     extern __shared__ __align__(16) char smem[];
+    __shared__ uint32_t nitem_s;
     __shared__ int64_t wid_s[kMfWin];
     constexpr int KW = kMfWin * NP;  // accumulator tiles per wave
     const int C = ts.C, U = ts.U, L = ts.L, R = ts.R;
@@ -85,8 +101,10 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
     const int NSP = (NS + 15) & ~15;           // staged samples per window (16-sample blocks)
     const int NB = NSP / 16;
     const int64_t N = st.N;
+    constexpr int NGX = (NP * 16 + kPos - 1) / kPos;
     _Float16* Vh = reinterpret_cast<_Float16*>(smem);                        // [kMfWin][NSP][RS]
-    float* bmx = reinterpret_cast<float*>(Vh + kMfWin * NSP * RS);          // [kMfWin][NB][Cp] block max |v|
+    const size_t region = max((size_t)kMfWin * NSP * RS * 2, (size_t)kMfWin * U * (NGX * 4 + 8));
+    float* bmx = reinterpret_cast<float*>(smem + region);                   // [kMfWin][NB][Cp] block max |v|
     float* Qs = bmx + kMfWin * NB * Cp;                                      // [kMfWin][U][NP] sum rowL1 * vmax
     uint32_t* flag = reinterpret_cast<uint32_t*>(Qs + kMfWin * U * NP);     // [kMfWin][U] bit p: tile p rescored
     int* rch = reinterpret_cast<int*>(flag + kMfWin * U);                   // [R] channel of each compact row
@@ -279,18 +297,72 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
             }
         }
         __syncthreads();
-        // tile masks for the masked exact pass; pairs with no flagged tile keep
-        // negative window maxima, which the window maximum treats as none
+        // exact FP32 pass over the flagged tiles of this group's windows, with
+        // the operation order of k_likelihood (bias, the template's rows in
+        // channel order, taps ascending, 8 positions per item); the voltage is
+        // read from V, which the staging above just brought into the caches.
+        // Window maxima of flagged pairs cover their flagged groups; the other
+        // pairs' maxima are negative, which the window maximum treats as none.
+        constexpr int NG = (NP * 16 + kPos - 1) / kPos;  // 8-position groups per window (upper bound)
+        const int ngr = (L + kPos - 1) / kPos;
+        uint32_t* ilist = reinterpret_cast<uint32_t*>(Vh);  // [kMfWin * U * NG] items (the fp16 windows are dead)
+        unsigned long long* keys = reinterpret_cast<unsigned long long*>(ilist + kMfWin * U * NG);  // [kMfWin * U]
+        if (threadIdx.x == 0) nitem_s = 0;
+        for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) keys[e] = 0ull;
+        __syncthreads();
+        for (int e = threadIdx.x; e < kMfWin * U * ngr; e += blockDim.x) {
+            const int k = e / (U * ngr);
+            const int rem = e - k * U * ngr;
+            const int t = rem / ngr, g = rem - t * ngr;
+            if (wid_s[k] < 0) continue;
+            if ((flag[k * U + t] >> ((g * kPos) / 16)) & 1u) ilist[atomicAdd(&nitem_s, 1u)] = (uint32_t)e;
+        }
+        __syncthreads();
+        const int nitems = (int)nitem_s;
+        for (int ii = threadIdx.x; ii < nitems; ii += blockDim.x) {
+            const int e = (int)ilist[ii];
+            const int k = e / (U * ngr);
+            const int rem = e - k * U * ngr;
+            const int t = rem / ngr, g = rem - t * ngr;
+            const int p0 = g * kPos;
+            const int64_t x0 = wid_s[k] * L + p0;
+            float acc[kPos];
+            const float b0 = ts.bias[t];
+#pragma unroll
+            for (int q = 0; q < kPos; q++) acc[q] = b0;
+            const int r0 = ts.off[t], nr = ts.nch[t];
+            const int64_t lim = N - x0;
+            for (int r = 0; r < nr; r++) {
+                const float* trow = ts.rows + (size_t)(r0 + r) * ts.Lp;
+                const float* vcol = st.V + (size_t)x0 * C + rch[r0 + r];
+                if (LT > 0) lik_row_strided<(LT > 0 ? LT : 4)>(trow, vcol, C, lim, acc);
+                else lik_row_generic_strided(trow, vcol, C, L, lim, acc);
+            }
+            float* out = st.LIK + (size_t)t * N;
+            float mv = -INFINITY;
+            uint32_t mp = 0xffffu;
+#pragma unroll
+            for (int q = 0; q < kPos; q++) {
+                const int64_t x = x0 + q;
+                if (p0 + q < L && x < N) {
+                    const float v = (x + L >= N) ? 0.0f : acc[q];
+                    out[x] = v;
+                    if (v > mv) { mv = v; mp = (uint32_t)(p0 + q); }
+                }
+            }
+            if (mv != -INFINITY) atomicMax(&keys[k * U + t], max_key(mv, mp));
+        }
+        if (threadIdx.x == 0 && nitems) atomicAdd(st.counters + CNT_LIKX, (uint32_t)nitems * kPos);
+        __syncthreads();
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
             const int k = e / U, t = e - k * U;
             const int64_t wd = wid_s[k];
             if (wd < 0) continue;
-            const uint32_t f = flag[e];
-            fmask[(size_t)(gi * kMfWin + k) * U + t] = (uint8_t)f;
-            if (f == 0u) {
-                st.wmax_v[(size_t)wd * U + t] = -INFINITY;
-                st.wmax_p[(size_t)wd * U + t] = 0xffffu;
-            }
+            float mv = -INFINITY;
+            uint32_t mp = 0xffffu;
+            if (flag[e] != 0u && keys[e] != 0ull) key_max(keys[e], mv, mp);
+            st.wmax_v[(size_t)wd * U + t] = mv;
+            st.wmax_p[(size_t)wd * U + t] = (uint16_t)mp;
         }
     }
 }
@@ -305,7 +377,10 @@ size_t likmfma_lds(const TemplateSet& ts, int ntiles)
     const int NS = NP * 16 + ts.L - 1;
     const int NSP = (NS + 15) & ~15, NB = NSP / 16;
     (void)ntiles;
-    return (size_t)kMfWin * NSP * RS * 2 + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
+    // the staged windows' region later holds the exact pass's item list and keys
+    const size_t NG = (NP * 16 + kPos - 1) / kPos;
+    const size_t region = std::max((size_t)kMfWin * NSP * RS * 2, (size_t)kMfWin * ts.U * (NG * 4 + 8));
+    return region + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
            (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8;
 }
 
@@ -326,30 +401,31 @@ static void allow_lds_mf(K kernel, size_t bytes)
                                   (int)bytes);
 }
 
-// MFMA likelihood of the listed windows W[w0 .. w0+nb); fmask receives, per
-// window of the batch and template, the 16-position tiles that need the exact
-// FP32 chain (likelihood_masked).
+template <int NP, int LT>
+static void launch_mf(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
+                      const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, size_t lds, dim3 g, hipStream_t s)
+{
+    allow_lds_mf(k_likmfma<NP, LT>, lds);
+    hipLaunchKernelGGL((k_likmfma<NP, LT>), g, dim3(kMfThreads), lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb);
+}
+
+// MFMA likelihood of the listed windows W[w0 .. w0+nb), with the exact FP32
+// chain for the 16-position tiles the bound flags.
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
-                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, uint8_t* fmask, hipStream_t s)
+                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, hipStream_t s)
 {
     if (nb == 0) return;
     const size_t lds = likmfma_lds(ts, mt.ntiles);
     const uint32_t ngroups = (nb + kMfWin - 1) / kMfWin;
-    const dim3 g(std::min<uint32_t>(ngroups, 4096u)), b(kMfThreads);
+    const dim3 g(std::min<uint32_t>(ngroups, 4096u));
     const int NP = (ts.L + 15) / 16;
-    if (NP == 3) {
-        allow_lds_mf(k_likmfma<3>, lds);
-        hipLaunchKernelGGL(k_likmfma<3>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
-    } else if (NP == 4) {
-        allow_lds_mf(k_likmfma<4>, lds);
-        hipLaunchKernelGGL(k_likmfma<4>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
-    } else if (NP == 2) {
-        allow_lds_mf(k_likmfma<2>, lds);
-        hipLaunchKernelGGL(k_likmfma<2>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
-    } else {
-        allow_lds_mf(k_likmfma<1>, lds);
-        hipLaunchKernelGGL(k_likmfma<1>, g, b, lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb, fmask);
-    }
+    if (ts.L == 48) launch_mf<3, 48>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else if (ts.L == 64) launch_mf<4, 64>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else if (ts.L == 32) launch_mf<2, 32>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else if (NP == 4) launch_mf<4, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else if (NP == 3) launch_mf<3, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else if (NP == 2) launch_mf<2, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    else launch_mf<1, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
 }
 
 }  // namespace fbp

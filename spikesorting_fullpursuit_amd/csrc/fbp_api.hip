@@ -5,7 +5,6 @@
 // window lists on the device; the host reads two small counters per pass
 // (overlap windows, then spikes added and next list length).
 #include <hip/hip_runtime.h>
-#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <cstdio>
@@ -47,10 +46,6 @@ void launch_screen_corr(const TemplateSet& ts, const PassState& st, const Screen
                         float* Bbuf, hipStream_t s);
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
                           uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s);
-void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int K, int S, int n, float tscale,
-                   _Float16* Ah, _Float16* Al, hipStream_t s);
-void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
-                      _Float16* Vl, float* vscale, hipStream_t s);
 // MFMA likelihood tables (fbp_likmfma.hip).
 constexpr int kMfMaxBlk = 32;
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -69,6 +64,9 @@ struct MfTables {
 void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s);
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
                     const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, hipStream_t s);
+constexpr int kScreenMfmaMaxS = 128;  // shifts of the matrix-core screen (8 tiles of 16)
+void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const void* fragl,
+                        const float* Ntr, int S, int n, uint32_t f0, uint32_t nbatch, float* Bbuf, hipStream_t s);
 }  // namespace fbp
 
 // Limits of the screened recheck (beyond them the literal recheck runs).
@@ -148,25 +146,13 @@ struct fbp_handle {
     int64_t npairs = 0;
     bool screen_ok = false;
     DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, rowL1, SS, Bbuf;
-    // split-fp16 GEMM form of the screen correlations
-    struct Group {
-        int c_lo, K, M, row_off;
-        size_t a_off;
-        int units_off, nunits;
-    };
-    std::vector<Group> groups;
-    std::vector<int> h_gunits;
-    bool gemm_ok = false;
-    float tscale = 1.0f;
-    rocblas_handle rb = nullptr;
-    DevBuf Ah, Al, brow, gunits, Vh, Vl, vscale;
-    bool groups_ok = false;
-    // mixed-precision likelihood: Toeplitz template matrices per group
+    // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
+    // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
-    // MFMA likelihood: A fragments of 16-template tiles and the rescoring list
-    DevBuf mf_frag, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs;
+    DevBuf mf_frag, mf_fragl, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs;
     int mf_ntiles = 0;
     float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
+    float scr_k2 = 0.0f, scr_k3 = 0.0f, scr_k4 = 0.0f;  // screen bound terms of the split-fp16 B
     bool mf_ok = false;
     // chunk state
     int64_t N = 0, nwin = 0;
@@ -307,129 +293,22 @@ static ScreenTables tables_view(const fbp_handle* h)
     const double K = 2.0 * h->Cmax * (h->L + 1) + 2.0 * h->L + 16.0;
     const double u = std::ldexp(1.0, -24);
     tb.kappa = (float)(4.0 * K * u / (1.0 - K * u));
-    // split-fp16 GEMM: 3 GEMMs of at most Cmax*L non-zero products each
-    // (gamma_{3K2} with the unit roundoff doubled for an unknown accumulation
-    // order), the hi/lo split residue (3 * 2^-22) and the scaling, doubled.
-    const double K2 = 3.0 * h->Cmax * h->L + 8.0;
-    const double u2 = 2.0 * u;
-    tb.kappa2 = (float)(2.0 * (K2 * u2 / (1.0 - K2 * u2) + 4.0 * std::ldexp(1.0, -22)));
-    tb.gemm = 0;
-    tb.brow = h->brow.as<int>();
-    tb.vscale = h->vscale.as<float>();
-    tb.inv_tscale = 1.0f / h->tscale;
+    // FP32 B (the vector-unit correlation); the matrix-core B sets these
+    tb.kappa2 = 0.0f;
+    tb.kappa3 = 0.0f;
+    tb.kappa4 = 0.0f;
     tb.sel_region = screen_select_region(template_view(h), tb.S, tb.cmax, tb.npl_max);
     return tb;
 }
 
-// Groups units whose non-zero channels fit in a window of at most 16 channels
-// (or one unit's own span when wider); sets the template scale for fp16.
-static int build_groups(fbp_handle* h)
-{
-    if (h->groups_ok) return FBP_OK;
-    const int U = h->U, C = h->C, L = h->L;
-    std::vector<int> first(U, 0), last(U, 0), order(U);
-    for (int u = 0; u < U; u++) {
-        int f = -1, l = -1;
-        for (int c = 0; c < C; c++)
-            if (h->h_nz[(size_t)u * C + c]) {
-                if (f < 0) f = c;
-                l = c;
-            }
-        first[u] = f < 0 ? 0 : f;
-        last[u] = l < 0 ? 0 : l;
-        order[u] = u;
-    }
-    std::vector<float> hT((size_t)U * C * L);
-    HIP_TRY(hipMemcpy(hT.data(), h->T.p, hT.size() * 4, hipMemcpyDeviceToHost), "copy templates");
-    float tmax = 0.0f;
-    for (float v : hT) tmax = std::max(tmax, std::fabs(v));
-    int ex = 0;
-    std::frexp(tmax > 0.0f ? tmax : 1.0f, &ex);
-    h->tscale = std::ldexp(1.0f, 14 - ex);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-        return first[a] != first[b] ? first[a] < first[b] : last[a] < last[b];
-    });
-    const int Wmax = 16;
-    h->groups.clear();
-    std::vector<int> gunits;
-    size_t i = 0;
-    while (i < order.size()) {
-        const int lo = first[order[i]];
-        int hi = last[order[i]];
-        size_t j = i + 1;
-        while (j < order.size() && std::max(hi, last[order[j]]) - lo + 1 <= std::max(Wmax, last[order[i]] - lo + 1)) {
-            hi = std::max(hi, last[order[j]]);
-            j++;
-        }
-        fbp_handle::Group g;
-        g.c_lo = lo;
-        g.K = (hi - lo + 1) * L;  // channel count * L; per use the sample extent differs
-        g.nunits = (int)(j - i);
-        g.M = 0;
-        g.row_off = 0;
-        g.a_off = 0;
-        g.units_off = (int)gunits.size();
-        for (size_t k = i; k < j; k++) gunits.push_back(order[k]);
-        h->groups.push_back(g);
-        i = j;
-    }
-    HIP_TRY(h->gunits.ensure(gunits.size() * 4), "alloc gemm rows");
-    HIP_TRY(hipMemcpy(h->gunits.p, gunits.data(), gunits.size() * 4, hipMemcpyHostToDevice), "copy");
-    h->h_gunits = gunits;
-    if (!h->rb) {
-        if (rocblas_create_handle(&h->rb) != rocblas_status_success) {
-            h->rb = nullptr;
-            return set_err(h, FBP_E_HIP, "rocblas_create_handle failed");
-        }
-    }
-    if (rocblas_set_stream(h->rb, h->stream) != rocblas_status_success)
-        return set_err(h, FBP_E_HIP, "rocblas_set_stream failed");
-    h->groups_ok = true;
-    return FBP_OK;
-}
-
-static int group_channels(const fbp_handle* h, const fbp_handle::Group& g) { return g.K / h->L; }
-
-// Split-fp16 matrices of the window-truncated shifted templates (screen B).
-static int build_gemm_tables(fbp_handle* h, int64_t n)
-{
-    h->gemm_ok = false;
-    if (build_groups(h) != FBP_OK) return FBP_OK;
-    const int U = h->U, L = h->L;
-    const int S = (int)(2 * n + 1);
-    std::vector<int> brow(U, 0);
-    int row = 0;
-    size_t aoff = 0;
-    for (auto& g : h->groups) {
-        g.M = g.nunits * S;
-        g.row_off = row;
-        g.a_off = aoff;
-        for (int k = 0; k < g.nunits; k++) brow[h->h_gunits[g.units_off + k]] = row + k * S;
-        row += g.M;
-        aoff += (size_t)g.M * g.K;
-    }
-    if (aoff * 2 * sizeof(_Float16) > kMaxGramBytes) return FBP_OK;
-    HIP_TRY(h->Ah.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
-    HIP_TRY(h->Al.ensure(aoff * sizeof(_Float16)), "alloc gemm A");
-    HIP_TRY(h->brow.ensure((size_t)U * 4), "alloc gemm rows");
-    HIP_TRY(hipMemcpy(h->brow.p, brow.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
-    TemplateSet ts = template_view(h);
-    for (const auto& g : h->groups)
-        launch_gemm_A(ts, h->gunits.as<int>() + g.units_off, g.nunits, g.c_lo, g.K, S, (int)n, h->tscale,
-                      h->Ah.as<_Float16>() + g.a_off, h->Al.as<_Float16>() + g.a_off, h->stream);
-    HIP_TRY(hipGetLastError(), "gemm tables");
-    HIP_TRY(hipStreamSynchronize(h->stream), "gemm tables");
-    (void)L;
-    h->gemm_ok = true;
-    return FBP_OK;
-}
-
-// Tables of the MFMA likelihood: templates ordered by their first non-zero
-// channel, cut into tiles of 16; each tile covers the contiguous 8-channel
-// blocks from its first to its last non-zero channel; the A operand in
-// fragment order (tile, k-block of 32, lane, 8 values), k = (sample j, block
-// b, channel in block) with run 4 * kb + lane / 16 = j * blocks + b.  mf_ok stays false (FP32 likelihood) when
-// L > 64 or the probe has more than 256 channels.
+// Matrix-core tables (likelihood and overlap screen): templates ordered by
+// their first non-zero channel, cut into tiles of 16; each tile covers the
+// contiguous 8-channel blocks from its first to its last non-zero channel; the
+// A operand in fragment order (tile, k-block of 32, lane, 8 values), k =
+// (sample j, block b, channel in block) with run 4 * kb + lane / 16 =
+// j * blocks + b, as fp16 hi parts and the lo parts of the remainder.  mf_ok
+// stays false (FP32 kernels) when L > 64 or the probe has more than 256
+// channels.
 static int build_mf_tables(fbp_handle* h)
 {
     h->mf_built = true;
@@ -473,7 +352,7 @@ static int build_mf_tables(fbp_handle* h)
     int ex = 0;
     std::frexp(tmax > 0.0f ? tmax : 1.0f, &ex);
     h->mf_tscale = std::ldexp(1.0f, 14 - ex);
-    std::vector<_Float16> frag((size_t)total_kb * 64 * 8);
+    std::vector<_Float16> frag((size_t)total_kb * 64 * 8), fragl(frag.size());
     for (int t = 0; t < ntiles; t++)
         for (int kb = 0; kb < nkb[t]; kb++)
             for (int lane = 0; lane < 64; lane++)
@@ -486,7 +365,9 @@ static int build_mf_tables(fbp_handle* h)
                         const int c = (chb[t] + b) * 8 + i;
                         if (c < C) v = hT[((size_t)u * C + c) * L + j] * h->mf_tscale;
                     }
-                    frag[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = (_Float16)v;
+                    const _Float16 hi = (_Float16)v;
+                    frag[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = hi;
+                    fragl[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = (_Float16)(v - (float)hi);
                 }
     std::vector<float> T1(U);
     for (int u = 0; u < U; u++) {
@@ -495,6 +376,7 @@ static int build_mf_tables(fbp_handle* h)
         T1[u] = (float)(a * (1.0 + 1e-6));
     }
     HIP_TRY(h->mf_frag.ensure(frag.size() * sizeof(_Float16)), "alloc likelihood tables");
+    HIP_TRY(h->mf_fragl.ensure(fragl.size() * sizeof(_Float16)), "alloc likelihood tables");
     HIP_TRY(h->mf_units.ensure(units.size() * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_nkb.ensure((size_t)ntiles * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_nblk.ensure((size_t)ntiles * 4), "alloc likelihood tables");
@@ -504,6 +386,7 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(h->mf_mbits.ensure(4), "alloc likelihood tables");
     HIP_TRY(h->mf_vs.ensure(4), "alloc likelihood tables");
     HIP_TRY(hipMemcpy(h->mf_frag.p, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_fragl.p, fragl.data(), fragl.size() * sizeof(_Float16), hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_units.p, units.data(), units.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_nkb.p, nkb.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_nblk.p, nblk.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
@@ -511,18 +394,38 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(hipMemcpy(h->mf_chb.p, chb.data(), chb.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_T1.p, T1.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
     h->mf_ntiles = ntiles;
-    // delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts): fp16 rounding
-    // of both operands (2^-11 relative each, 2^-25 absolute below the normal
-    // range), FP32 accumulation of K products (gamma_K, doubled for an unknown
-    // order), the exact chain's own error (gamma_K2 on the products and the
-    // bias), the final scaling and bias add; all doubled
+    // Likelihood: delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts).
+    // An operand x (scaled) enters as fp16 x' with |x - x'| <= 2^-11 |x| + 2^-14
+    // (2^-14 covers a sub-normal part, kept or flushed by the matrix core), so a
+    // product errs by at most (1 + 2^-10) (2^-10 |t v| + 2^-14 (|t| + |v|)); add
+    // FP32 accumulation of K products (gamma_K, doubled for an unknown order),
+    // the exact chain's own error (gamma_K2 on the products and the bias) and
+    // the final scaling and bias add; all doubled.
     const double u = std::ldexp(1.0, -24);
     const double K = (double)L * 8.0 * maxnb, K2 = (double)h->Cmax * L + 1.0;
     const double gK = K * u / (1.0 - K * u), gK2 = K2 * u / (1.0 - K2 * u);
-    h->mf_kl = (float)(2.0 * (std::ldexp(1.0, -10) * (1.0 + std::ldexp(1.0, -11)) + 2.0 * gK + gK2 +
+    h->mf_kl = (float)(2.0 * (std::ldexp(1.0, -10) * (1.0 + std::ldexp(1.0, -10)) + 2.0 * gK + gK2 +
                               std::ldexp(1.0, -23)));
     h->mf_kb = (float)(2.0 * (gK2 + std::ldexp(1.0, -24)));
-    h->mf_ka = (float)(2.0 * std::ldexp(1.0, -25) * (1.0 + std::ldexp(1.0, -10)));
+    h->mf_ka = (float)(2.0 * std::ldexp(1.0, -14) * (1.0 + std::ldexp(1.0, -10)));
+    // Overlap screen, B from split operands x = hi + lo + e with
+    // |e| <= 2^-22 |x| + 2^-14 and |e| <= |x| (scaled): three products
+    // hi*hi + hi*lo + lo*hi err by at most 4 * 2^-22 |t v| + 2^-14 (2 |t| + |v|)
+    // each, i.e. 4 * 2^-22 |t v| + 2^-27 (2 maxw |t| + Tmax |v|) unscaled (both
+    // scales put the operand maximum in [2^13, 2^14)).  FP32 accumulation in
+    // two chains (unit roundoff doubled for an unknown order inside the matrix
+    // core): hi*hi of at most Cmax * L non-zero products, the cross products
+    // (at most 2^-10 of |t v| together) of twice as many, and their sum.  The
+    // relative part rides on Q (kappa2), the absolute part on sum |T| and the
+    // window maxima (kappa3, kappa4); all doubled.
+    {
+        const double K3 = (double)h->Cmax * L + 8.0;
+        const double u2 = 2.0 * u;
+        const double g1 = K3 * u2 / (1.0 - K3 * u2), g2 = 2.0 * K3 * u2 / (1.0 - 2.0 * K3 * u2);
+        h->scr_k2 = (float)(2.0 * ((1.0 + u) * (g1 + std::ldexp(1.0, -10) * g2) + u + 4.0 * std::ldexp(1.0, -22)));
+        h->scr_k3 = (float)(2.0 * 2.0 * std::ldexp(1.0, -27) * (1.0 + std::ldexp(1.0, -10)));
+        h->scr_k4 = (float)(2.0 * std::ldexp(1.0, -27) * (1.0 + std::ldexp(1.0, -10)) * (double)tmax * (double)L);
+    }
     h->mf_ok = true;
     return FBP_OK;
 }
@@ -543,37 +446,6 @@ static MfTables mf_view(const fbp_handle* h)
     mt.kb = h->mf_kb;
     mt.ka = h->mf_ka;
     return mt;
-}
-
-// B for the flagged windows F[f0 .. f0+nb) into Bbuf[w][row] by three
-// split-fp16 GEMMs per unit group.
-static int run_screen_gemm(fbp_handle* h, const PassState& st, uint32_t f0, uint32_t nb)
-{
-    TemplateSet ts = template_view(h);
-    launch_gemm_pack(ts, st, f0, nb, h->Vh.as<_Float16>(), h->Vl.as<_Float16>(), h->vscale.as<float>(), h->stream);
-    const int S = (int)(2 * h->tab_n + 1);
-    const int US = h->U * S;
-    const int CL = h->C * h->L;
-    const float one = 1.0f, zero = 0.0f;
-    const _Float16* Vh = h->Vh.as<_Float16>();
-    const _Float16* Vl = h->Vl.as<_Float16>();
-    for (const auto& g : h->groups) {
-        const _Float16* Ah = h->Ah.as<_Float16>() + g.a_off;
-        const _Float16* Al = h->Al.as<_Float16>() + g.a_off;
-        float* Cg = h->Bbuf.as<float>() + g.row_off;
-        const _Float16* Bs[3] = {Vh, Vl, Vh};
-        const _Float16* As[3] = {Ah, Ah, Al};
-        for (int k = 0; k < 3; k++) {
-            rocblas_status rs = rocblas_gemm_ex(
-                h->rb, rocblas_operation_transpose, rocblas_operation_none, g.M, (int)nb, g.K, &one, As[k],
-                rocblas_datatype_f16_r, g.K, Bs[k] + (size_t)g.c_lo * h->L, rocblas_datatype_f16_r, CL,
-                k == 0 ? &zero : &one, Cg, rocblas_datatype_f32_r, US, Cg, rocblas_datatype_f32_r, US,
-                rocblas_datatype_f32_r, rocblas_gemm_algo_standard, 0, 0);
-            if (rs != rocblas_status_success)
-                return set_err(h, FBP_E_HIP, std::string("rocblas_gemm_ex failed: ") + rocblas_status_to_string(rs));
-        }
-    }
-    return FBP_OK;
 }
 
 // Builds the voltage-independent tables of the screened recheck for shift
@@ -627,10 +499,6 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     launch_screen_tables(ts, tables_view(h), np, h->stream);
     HIP_TRY(hipGetLastError(), "screen tables");
     HIP_TRY(hipStreamSynchronize(h->stream), "screen tables");
-    {
-        int rc = build_gemm_tables(h, n);
-        if (rc != FBP_OK) return rc;
-    }
     h->screen_ok = true;
     *ok = true;
     return FBP_OK;
@@ -668,11 +536,9 @@ int fbp_destroy(fbp_handle* h)
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
-                      &h->Ah, &h->Al, &h->brow, &h->gunits, &h->Vh, &h->Vl, &h->vscale,
-                      &h->mf_frag, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
+                      &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs};
     for (DevBuf* b : bufs) b->release();
-    if (h->rb) (void)rocblas_destroy_handle(h->rb);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -768,7 +634,6 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
     HIP_TRY(hipMemcpy(h->rowL1.p, h->h_rowL1.data(), h->h_rowL1.size() * 4, hipMemcpyHostToDevice), "copy norms");
     HIP_TRY(hipMemcpy(h->SS.p, h->h_SS.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy norms");
     h->tab_n = -1;
-    h->groups_ok = false;
     h->mf_built = false;
     h->have_templates = true;
     h->have_result = false;
@@ -785,8 +650,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     if (N >= (int64_t)0xffffffffLL) return set_err(h, FBP_E_UNSUPPORTED, "chunk longer than 2^32-1 samples");
     if (L > 65535) return set_err(h, FBP_E_UNSUPPORTED, "template length above 65535 samples");
     const uint64_t S = 2ull * n_max_shift + 1ull;
-    const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
-    if (recheck && S * S * (uint64_t)U >= 0xffffffffull)
+    if ((flags & FBP_FLAG_OVERLAP_RECHECK) && S * S * (uint64_t)U >= 0xffffffffull)
         return set_err(h, FBP_E_UNSUPPORTED, "(2*n_max_shift_inds+1)^2 * n_templates must be < 2^32-1");
     HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
     hipStream_t s = h->stream;
@@ -824,12 +688,18 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
 
     bool mixed = (flags & FBP_FLAG_PRECISION_MIXED) != 0;
     const uint32_t lik_batch = 65536;
-    if (mixed) {
+    const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
+    // split-fp16 overlap screen on the matrix cores unless FP32 B is asked for
+    bool screen_mf = recheck && !(flags & (FBP_FLAG_RECHECK_LITERAL | FBP_FLAG_SCREEN_FP32)) &&
+                     S <= (uint64_t)kScreenMfmaMaxS;
+    if (mixed || screen_mf) {
         if (!h->mf_built) {
             int rc = build_mf_tables(h);
             if (rc != FBP_OK) return rc;
         }
-        mixed = h->mf_ok;  // outside the MFMA tables' limits the FP32 likelihood runs (same results)
+        // outside the matrix-core tables' limits the FP32 kernels run (same results)
+        mixed = mixed && h->mf_ok;
+        screen_mf = screen_mf && h->mf_ok;
     }
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
     hipEvent_t ev0, ev1;
@@ -876,13 +746,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     }
     const uint32_t screen_batch =
         use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
-    const bool use_gemm = use_screen && h->gemm_ok && !(flags & FBP_FLAG_SCREEN_FP32);
+    screen_mf = screen_mf && use_screen;
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
-    if (use_gemm) {
-        HIP_TRY(h->Vh.ensure((size_t)screen_batch * C * L * sizeof(_Float16)), "alloc screen pack");
-        HIP_TRY(h->Vl.ensure((size_t)screen_batch * C * L * sizeof(_Float16)), "alloc screen pack");
-        HIP_TRY(h->vscale.ensure((size_t)screen_batch * 4), "alloc screen pack");
-    }
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -917,13 +782,16 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                 tm.begin(TM_RECHECK);
                 if (use_screen) {
                     ScreenTables tb = tables_view(h);
-                    if (use_gemm) tb.gemm = 1;
-                    else tb.kappa2 = 0.0f;
+                    if (screen_mf) {
+                        tb.kappa2 = h->scr_k2;
+                        tb.kappa3 = h->scr_k3;
+                        tb.kappa4 = h->scr_k4;
+                    }
                     for (uint32_t f0 = 0; f0 < nF; f0 += screen_batch) {
                         const uint32_t nb = std::min(screen_batch, nF - f0);
-                        if (use_gemm) {
-                            int rc = run_screen_gemm(h, st, f0, nb);
-                            if (rc != FBP_OK) return rc;
+                        if (screen_mf) {
+                            launch_screen_mfma(ts, st, mf_view(h), h->mf_fragl.p, tb.Ntr, tb.S, tb.n, f0, nb,
+                                               h->Bbuf.as<float>(), s);
                         } else {
                             launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
                         }

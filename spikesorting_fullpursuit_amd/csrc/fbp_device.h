@@ -220,4 +220,14 @@ __device__ __forceinline__ void lik_row_generic_bounded(const float* __restrict_
     }
 }
 
+// A flagged window whose best spike fell below its threshold or too near the
+// chunk end takes no overlap recheck.
+__device__ __forceinline__ bool window_skipped(const TemplateSet& ts, const PassState& st, uint32_t id)
+{
+    const uint32_t lab = st.best_lab[id];
+    const int64_t b = st.best_idx[id];
+    const float lb = st.LIK[(size_t)lab * st.N + b];
+    return (lb < ts.thr[lab]) || (b >= st.N - ts.L);  // binary_pursuit.cl:518-527
+}
+
 }  // namespace fbp

@@ -67,14 +67,13 @@ struct ScreenTables {
     int S;
     int n;
     int cmax;             // most non-zero channels of any template
-    float kappa;          // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u)) + kappa2 * (Q_l + Q_u)
-    float kappa2;         // 0 when B is the FP32 VALU correlation; > 0 for the split-fp16 GEMM
-    // split-fp16 GEMM layout of B (gemm != 0): row of (u, s) is brow[u] + s,
-    // column scale of window w is 1 / (vscale[w] * tscale)
-    int gemm;
-    const int* brow;      // [U]
-    const float* vscale;  // [batch]
-    float inv_tscale;
+    // delta(l, u) = P_l + P_u bounds |E-hat - E_fp32| of a pair, with
+    // P_x = (kappa + kappa2) Q_x + 2 kappa SS_x + kappa3 maxw T1_x + kappa4 VS_x,
+    // Q_x = sum over x's channels of rowL1 * window max, T1_x = sum of rowL1,
+    // VS_x = sum of the window maxima over x's channels, maxw = window maximum
+    float kappa;          // FP32 chains of B, A and G
+    float kappa2;         // 0 when B is the FP32 VALU correlation; > 0 for split-fp16 matrix-core B
+    float kappa3, kappa4; // absolute residue of the fp16 split (0 for the FP32 B)
 };
 
 enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7, CNT_N = 8 };

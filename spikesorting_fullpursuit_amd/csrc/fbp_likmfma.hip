@@ -368,6 +368,154 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
 }
 
 // ---------------------------------------------------------------------------
+// Overlap-screen correlations on the matrix cores:
+//     B(u, s) = sum_{c, i} T[u][c][i] * w_c[s + i - n] - Ntr(u, s) / 2
+// for every unit u and shift s < S of the flagged windows F[f0 .. f0+nbatch),
+// where w is the window's voltage (samples b .. b+L-1, zero elsewhere) and Ntr
+// the energy of the window-truncated template (k_screen_corr computes the same
+// values on the vector units).  Both operands are split into fp16 hi + lo parts
+// (templates scaled by ts, each window by a power of two vs_w with
+// max |v| vs_w in [2^13, 2^14)); the products hi*hi and hi*lo + lo*hi per
+// k-block accumulate in FP32 in two chains, which leaves about 2^-22 relative
+// of the operands' rounding plus the accumulation error (the screen bound's
+// kappa2) and an absolute residue of the sub-normal parts (kappa3, kappa4).
+// Rows of a window absent from it read a zero row of LDS; a k-block whose
+// 16-shift tile reads only absent rows is skipped.
+// ---------------------------------------------------------------------------
+constexpr int kSwWin = 2;  // flagged windows per block iteration
+
+template <int NPS>
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_screen_mfma(
+    TemplateSet ts, PassState st, MfTables mt, const half8* __restrict__ fragl, const float* __restrict__ Ntr, int S,
+    int n, uint32_t f0, uint32_t nbatch, float* __restrict__ Bbuf)
+{
+    extern __shared__ __align__(16) char smem[];
+    __shared__ int64_t b_s[kSwWin];
+    __shared__ float vs_s[kSwWin];
+    __shared__ float red[kMfThreads / 64];
+    const int C = ts.C, U = ts.U, L = ts.L;
+    const int Cp = (C + 7) & ~7, RS = Cp + 8;
+    const int LR = L + 1;  // window rows + one zero row
+    const int US = U * S;
+    _Float16* Wh = reinterpret_cast<_Float16*>(smem);  // [kSwWin][LR][RS]
+    _Float16* Wl = Wh + kSwWin * LR * RS;              // [kSwWin][LR][RS]
+    const int lane = lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    const int q4 = lane >> 4, col = lane & 15;
+    const uint32_t nF = st.counters[CNT_F];
+    const uint32_t ngroups = (nbatch + kSwWin - 1) / kSwWin;
+    for (uint32_t gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        __syncthreads();
+        if (threadIdx.x < kSwWin) {
+            const uint32_t wi = gi * kSwWin + threadIdx.x, w = f0 + wi;
+            int64_t b = -1;
+            if (wi < nbatch && w < nF) {
+                const uint32_t id = st.F[w];
+                if (!window_skipped(ts, st, id)) b = st.best_idx[id];
+            }
+            b_s[threadIdx.x] = b;
+        }
+        __syncthreads();
+        // window maxima -> per-window scale
+        for (int k = 0; k < kSwWin; k++) {
+            const int64_t b = b_s[k];
+            float m = 0.0f;
+            if (b >= 0)
+                for (int e = threadIdx.x; e < L * C; e += blockDim.x) m = fmaxf(m, fabsf(st.V[(size_t)b * C + e]));
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+            if (lane == 0) red[wv] = m;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float mm = red[0];
+                for (int i = 1; i < nwv; i++) mm = fmaxf(mm, red[i]);
+                int ex = 0;
+                if (mm > 0.0f) frexpf(mm, &ex);
+                vs_s[k] = mm > 0.0f ? ldexpf(1.0f, 14 - ex) : 1.0f;
+            }
+            __syncthreads();
+        }
+        // stage the windows as split fp16 rows [sample][channel]; row L is zero
+        for (int e = threadIdx.x; e < kSwWin * LR * Cp; e += blockDim.x) {
+            const int k = e / (LR * Cp);
+            const int rem = e - k * LR * Cp;
+            const int j = rem / Cp, c = rem - j * Cp;
+            const int64_t b = b_s[k];
+            float v = 0.0f;
+            if (b >= 0 && j < L && c < C) v = st.V[(size_t)(b + j) * C + c] * vs_s[k];
+            const _Float16 hi = (_Float16)v;
+            Wh[(k * LR + j) * RS + c] = hi;
+            Wl[(k * LR + j) * RS + c] = (_Float16)(v - (float)hi);
+        }
+        __syncthreads();
+        for (int tile = wv; tile < mt.ntiles; tile += nwv) {
+            const int nkb = mt.nkb[tile], nbt = mt.nblk[tile];
+            const half8* Ah = mt.frag + (size_t)mt.aoff[tile] * 64 + lane;
+            const half8* Al = fragl + (size_t)mt.aoff[tile] * 64 + lane;
+            const int cb = mt.chb[tile] * 8;
+            // hi*hi and the two cross products accumulate apart: the long chain
+            // then holds one product per template value, not three
+            floatx4 acc[kSwWin][NPS], acs[kSwWin][NPS];
+#pragma unroll
+            for (int k = 0; k < kSwWin; k++)
+#pragma unroll
+                for (int p = 0; p < NPS; p++) {
+                    acc[k][p] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                    acs[k][p] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+                }
+            half8 ahn = Ah[0], aln = Al[0];
+            for (int kb = 0; kb < nkb; kb++) {
+                const half8 ah = ahn, al = aln;
+                if (kb + 1 < nkb) {
+                    ahn = Ah[(size_t)(kb + 1) * 64];
+                    aln = Al[(size_t)(kb + 1) * 64];
+                }
+                const int r = 4 * kb + q4;
+                int i = r / nbt, bk = r - i * nbt;
+                const bool pad = i >= L;  // padding run: its A values are zero
+                if (pad) { i = 0; bk = 0; }
+                // template samples of the k-block (uniform): runs 4 kb .. 4 kb + 3
+                const int ilo = (4 * kb) / nbt, ihi = min(L - 1, (4 * kb + 3) / nbt);
+#pragma unroll
+                for (int p = 0; p < NPS; p++) {
+                    // window samples j = s + i - n over the tile's shifts s
+                    const int jlo = 16 * p + ilo - n, jhi = 16 * p + 15 + ihi - n;
+                    if (jhi < 0 || jlo >= L) continue;  // every row absent: all-zero B
+                    const int j = 16 * p + col + i - n;
+                    const int row = (j >= 0 && j < L && !pad) ? j : L;
+#pragma unroll
+                    for (int k = 0; k < kSwWin; k++) {
+                        const int o = (k * LR + row) * RS + cb + bk * 8;
+                        const half8 bh = *reinterpret_cast<const half8*>(Wh + o);
+                        const half8 bl = *reinterpret_cast<const half8*>(Wl + o);
+                        acc[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[k][p], 0, 0, 0);
+                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acs[k][p], 0, 0, 0);
+                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acs[k][p], 0, 0, 0);
+                    }
+                }
+            }
+            // lane holds rows 4*q4 + i (units), column col (shift within the tile)
+#pragma unroll
+            for (int k = 0; k < kSwWin; k++) {
+                if (b_s[k] < 0) continue;  // uniform
+                const float sc = mt.inv_tscale / vs_s[k];
+                float* Bw = Bbuf + (size_t)(gi * kSwWin + k) * US;
+#pragma unroll
+                for (int p = 0; p < NPS; p++) {
+                    const int sft = 16 * p + col;
+#pragma unroll
+                    for (int ii = 0; ii < 4; ii++) {
+                        const int u = mt.units[tile * 16 + q4 * 4 + ii];
+                        if (u < 0 || sft >= S) continue;
+                        const int e = u * S + sft;
+                        Bw[e] = fmaf(-0.5f, Ntr[e], (acc[k][p][ii] + acs[k][p][ii]) * sc);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 size_t likmfma_lds(const TemplateSet& ts, int ntiles)
@@ -426,6 +574,40 @@ void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& 
     else if (NP == 3) launch_mf<3, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
     else if (NP == 2) launch_mf<2, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
     else launch_mf<1, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+}
+
+size_t screen_mfma_lds(const TemplateSet& ts)
+{
+    const int Cp = (ts.C + 7) & ~7, RS = Cp + 8;
+    return (size_t)2 * kSwWin * (ts.L + 1) * RS * 2;
+}
+
+// B(u, s) of the flagged windows F[f0 .. f0+nbatch) into Bbuf[w][u * S + s].
+void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const void* fragl,
+                        const float* Ntr, int S, int n, uint32_t f0, uint32_t nbatch, float* Bbuf, hipStream_t s)
+{
+    if (nbatch == 0) return;
+    const size_t lds = screen_mfma_lds(ts);
+    const uint32_t ngroups = (nbatch + kSwWin - 1) / kSwWin;
+    const dim3 g(std::min<uint32_t>(ngroups, 8192u)), b(kMfThreads);
+    const half8* fl = reinterpret_cast<const half8*>(fragl);
+    const int NPS = (S + 15) / 16;
+#define FBP_SCREEN_MFMA(NPV)                                                                                      \
+    do {                                                                                                        \
+        allow_lds_mf(k_screen_mfma<NPV>, lds);                                                                  \
+        hipLaunchKernelGGL(k_screen_mfma<NPV>, g, b, lds, s, ts, st, mt, fl, Ntr, S, n, f0, nbatch, Bbuf);      \
+    } while (0)
+    switch (NPS) {
+    case 1: FBP_SCREEN_MFMA(1); break;
+    case 2: FBP_SCREEN_MFMA(2); break;
+    case 3: FBP_SCREEN_MFMA(3); break;
+    case 4: FBP_SCREEN_MFMA(4); break;
+    case 5: FBP_SCREEN_MFMA(5); break;
+    case 6: FBP_SCREEN_MFMA(6); break;
+    case 7: FBP_SCREEN_MFMA(7); break;
+    default: FBP_SCREEN_MFMA(8); break;
+    }
+#undef FBP_SCREEN_MFMA
 }
 
 }  // namespace fbp

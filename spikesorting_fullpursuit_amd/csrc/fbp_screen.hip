@@ -107,14 +107,6 @@ __device__ __forceinline__ void corr_row_range(const float* __restrict__ trow, c
     }
 }
 
-__device__ __forceinline__ bool window_skipped(const TemplateSet& ts, const PassState& st, uint32_t id)
-{
-    const uint32_t lab = st.best_lab[id];
-    const int64_t b = st.best_idx[id];
-    const float lb = st.LIK[(size_t)lab * st.N + b];
-    return (lb < ts.thr[lab]) || (b >= st.N - ts.L);  // binary_pursuit.cl:518-527
-}
-
 // B(u, s) for every unit and shift of the flagged windows F[f0 .. f0+nbatch).
 // One block per window; the window's voltage (zero outside [b, b+L)) is staged
 // in LDS as rows of rp samples, sample p <-> v[p - n].
@@ -300,15 +292,7 @@ constexpr int kListChunk = 2048;  // (unit, shift) pairs examined per work-list 
 // B(u, s) of the current window: element e = u*S + s.
 struct BView {
     const float* Bg;
-    const int* brow;
-    const float* Ntr;
-    float sc;
-    int S;
-    bool gemm;
-    __device__ __forceinline__ float operator()(int e, int u, int s) const
-    {
-        return gemm ? fmaf(-0.5f, Ntr[e], Bg[brow[u] + s] * sc) : Bg[e];
-    }
+    __device__ __forceinline__ float operator()(int e, int, int) const { return Bg[e]; }
 };
 
 // Selection, exact rescoring and the parse cascade for one window per block.
@@ -361,8 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
             if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
             continue;
         }
-        BView Bv{Bbuf + (size_t)wi * US, tb.brow, tb.Ntr, tb.gemm ? tb.inv_tscale / tb.vscale[wi] : 1.0f, S,
-                 tb.gemm != 0};
+        BView Bv{Bbuf + (size_t)wi * US};
         for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
             const int j = e / C, c = e - j * C;
             vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
@@ -461,14 +444,25 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
         }
         __syncthreads();
         for (int u = threadIdx.x; u < U; u += blockDim.x) {
-            float q = 0.0f;
+            float q = 0.0f, t1 = 0.0f, vsum = 0.0f;
             const int r0 = ts.off[u], nr = ts.nch[u];
-            for (int k = 0; k < nr; k++) q += tb.rowL1[r0 + k] * vmax[ts.chan[r0 + k]];
-            Q[u] = q;
+            for (int k = 0; k < nr; k++) {
+                const float vm = vmax[ts.chan[r0 + k]];
+                q += tb.rowL1[r0 + k] * vm;
+                t1 += tb.rowL1[r0 + k];
+                vsum += vm;
+            }
+            float z = 0.0f;
+            if (tb.kappa3 != 0.0f) {  // fp16 operands: absolute residue of the scaled split
+                float mw = 0.0f;
+                for (int c = 0; c < C; c++) mw = fmaxf(mw, vmax[c]);
+                z = tb.kappa3 * mw * t1 + tb.kappa4 * vsum;
+            }
+            Q[u] = (tb.kappa + tb.kappa2) * q + 2.0f * tb.kappa * tb.SS[u] + z;
         }
         __syncthreads();
-        for (int u = threadIdx.x; u < U; u += blockDim.x)
-            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])) + tb.kappa2 * (Q[lab] + Q[u]);
+        // delta(l, u) = P_l + P_u: the bound is a sum of one term per template
+        for (int u = threadIdx.x; u < U; u += blockDim.x) D[u] = Q[lab] + Q[u];
         float mA = -INFINITY;
         for (int f = threadIdx.x; f < S; f += blockDim.x)
             if (fin[f]) mA = fmaxf(mA, A[f]);
@@ -724,8 +718,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     float* zrow = TlP + CM * LP;          // [LP] zeros (a template channel that is absent)
     int* cl = reinterpret_cast<int*>(zrow + LP);  // [CM]
     int* pidw = cl + CM;                  // [U] pair id of (label, u); -1: no shared channel
-    int* browl = pidw + U;                // [U] GEMM row of (u, s = 0)
-    float* reg = reinterpret_cast<float*>(browl + U);  // [RG], in turn:
+    float* reg = reinterpret_cast<float*>(pidw + U);  // [RG], in turn:
     float* gmB = reg;                     //   [npl][S] least G over the blocks of f with LIK[l] > 0
     uint32_t* surv_e = reinterpret_cast<uint32_t*>(reg);  //   survivor lists: pair ...
     float* surv_b = reg + RG / 2;                          //   ... and its B
@@ -758,11 +751,6 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             return;
         }
         const float* Bw = Bbuf + (size_t)wi * US;
-        const float sc = tb.gemm ? tb.inv_tscale / tb.vscale[wi] : 1.0f;
-        const bool gemm = tb.gemm != 0;
-        auto bload = [&](int e, int u, int s) -> float {
-            return gemm ? fmaf(-0.5f, tb.Ntr[e], Bw[browl[u] + s] * sc) : Bw[e];
-        };
         for (int e = threadIdx.x; e < C * L; e += bd) {
             const int j = e / C, c = e - j * C;
             vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
@@ -779,7 +767,6 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         for (int u = threadIdx.x; u < U; u += bd) {
             const int p = tb.pid[(size_t)lab * U + u];
             pidw[u] = p;
-            browl[u] = gemm ? tb.brow[u] : 0;
             if (p >= 0) { atomicMin(&pfirst_sh, p); atomicAdd(&npl_sh, 1); }
         }
         __syncthreads();
@@ -790,7 +777,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             fin[f] = in;
             posl[f] = pl;
             if (pl) { atomicMin(&plo_sh, f); atomicMax(&phi_sh, f); }
-            A[f] = bload((int)lab * S + f, (int)lab, f);
+            A[f] = Bw[(int)lab * S + f];
         }
         // per-pair state: B, sign of LIK[u][b+s-n], shared-channel bit.  Loads
         // are issued 8 pairs at a time from clamped (always valid) addresses.
@@ -802,7 +789,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             asm volatile("" : "+v"(u), "+v"(s), "+v"(t0));  // recompute the pair walk, do not keep it live
 #pragma unroll
             for (int i0 = 0; i0 < kSelR; i0 += 8) {
-                float gb[8], gn[8], lv[8];
+                float gb[8], lv[8];
                 int pw[8], bi[8];
                 bool ok[8];
 #pragma unroll
@@ -810,10 +797,9 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     const int e = t0 + bd * (i0 + j);
                     const int64_t as = b + s - n;
                     ok[j] = e < US && as >= 0 && as < N - L;
-                    const int uc = ok[j] ? u : 0, scl = ok[j] ? s : 0, ec = ok[j] ? e : 0;
+                    const int uc = ok[j] ? u : 0, ec = ok[j] ? e : 0;
                     lv[j] = st.LIK[(size_t)uc * N + (ok[j] ? as : 0)];
-                    gn[j] = tb.Ntr[ec];
-                    bi[j] = gemm ? browl[uc] + scl : ec;
+                    bi[j] = ec;
                     pw[j] = pidw[uc];
                     pair_step(u, s, du, ds, S);
                 }
@@ -823,7 +809,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 __builtin_amdgcn_sched_barrier(0);  // issue the 8 pairs' loads together
 #pragma unroll
                 for (int j = 0; j < 8; j++) {
-                    bv[i0 + j] = ok[j] ? (gemm ? fmaf(-0.5f, gn[j], gb[j] * sc) : gb[j]) : -INFINITY;
+                    bv[i0 + j] = ok[j] ? gb[j] : -INFINITY;
                     if (ok[j] && lv[j] > 0.0f) pum |= 1u << (i0 + j);
                     if (ok[j] && pw[j] >= 0) shm |= 1u << (i0 + j);
                 }
@@ -901,14 +887,25 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         }
         __syncthreads();
         for (int u = threadIdx.x; u < U; u += bd) {
-            float q = 0.0f;
+            float q = 0.0f, t1 = 0.0f, vsum = 0.0f;
             const int r0 = ts.off[u], nr = ts.nch[u];
-            for (int k = 0; k < nr; k++) q += tb.rowL1[r0 + k] * vmax[ts.chan[r0 + k]];
-            Q[u] = q;
+            for (int k = 0; k < nr; k++) {
+                const float vm = vmax[ts.chan[r0 + k]];
+                q += tb.rowL1[r0 + k] * vm;
+                t1 += tb.rowL1[r0 + k];
+                vsum += vm;
+            }
+            float z = 0.0f;
+            if (tb.kappa3 != 0.0f) {  // fp16 operands: absolute residue of the scaled split
+                float mw = 0.0f;
+                for (int c = 0; c < C; c++) mw = fmaxf(mw, vmax[c]);
+                z = tb.kappa3 * mw * t1 + tb.kappa4 * vsum;
+            }
+            Q[u] = (tb.kappa + tb.kappa2) * q + 2.0f * tb.kappa * tb.SS[u] + z;
         }
         __syncthreads();
-        for (int u = threadIdx.x; u < U; u += bd)
-            D[u] = tb.kappa * (Q[lab] + Q[u] + 2.0f * (tb.SS[lab] + tb.SS[u])) + tb.kappa2 * (Q[lab] + Q[u]);
+        // delta(l, u) = P_l + P_u: the bound is a sum of one term per template
+        for (int u = threadIdx.x; u < U; u += bd) D[u] = Q[lab] + Q[u];
         __syncthreads();
         // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable;
         // M' = max(E-hat - delta_u) is a lower bound of the exact maximum
@@ -1068,7 +1065,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 const int e = (int)threadIdx.x + bd * i;
                 int u, s;
                 divmod_small(e, S, invS, u, s);
-                const float Bu = bload(e, u, s);
+                const float Bu = Bw[e];
                 if ((pum >> i) & 1u) {
                     for (int f = 0; f < S; f++) eval_all(e, Bu, f);
                 } else {
@@ -1093,7 +1090,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 divmod_small(e, S, invS, u, s);
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
-                const float Bu = bload(e, u, s);
+                const float Bu = Bw[e];
                 const int p = pidw[u];
                 const bool pu = st.LIK[(size_t)u * N + as] > 0.0f;
                 const float theta = fmaxf(Mp - D[u], bl - D[u]);
@@ -1212,7 +1209,7 @@ size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max
 {
     const int LP = ts.L + S - 1;
     const int rg = screen_select_region(ts, S, cmax, npl_max);
-    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 5 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 4 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
            2 * (size_t)S + 16;
 }
 
@@ -1225,93 +1222,6 @@ static void allow_lds(K kernel, size_t bytes)
                                   (int)bytes);
 }
 
-// ---------------------------------------------------------------------------
-// Split-fp16 GEMM form of B: B(u, s) = sum_{c, j} Tsh[(u, s)][(c, j)] * v[c][j]
-// with Tsh the window-truncated template of u shifted by s - n.  For a group
-// of units whose channels lie in [c_lo, c_lo + K/L), the rows (u, s) of Tsh
-// form a constant matrix A_g [M_g][K]; the flagged windows' voltage forms the
-// other operand [K][windows].  Both are split into fp16 hi + lo parts with
-// power-of-two scaling; three GEMMs (hi*hi + hi*lo + lo*hi) with fp32
-// accumulation give B to about 2^-22 relative plus the accumulation error,
-// which the select kernel's bound covers (kappa2).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_gemm_A(TemplateSet ts, const int* __restrict__ units, int nunits, int c_lo,
-                                                   int K, int S, int n, float tscale, _Float16* __restrict__ Ah,
-                                                   _Float16* __restrict__ Al)
-{
-    const int L = ts.L, C = ts.C;
-    const int64_t total = (int64_t)nunits * S * K;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = e / K;
-        const int k = (int)(e - r * K);
-        const int u = units[r / S], s = (int)(r % S);
-        const int c = c_lo + k / L, j = k % L;
-        const int i = j - (s - n);
-        float v = 0.0f;
-        if (c < C && i >= 0 && i < L) v = ts.T[((size_t)u * C + c) * L + i] * tscale;
-        const _Float16 hi = (_Float16)v;
-        Ah[e] = hi;
-        Al[e] = (_Float16)(v - (float)hi);
-    }
-}
-
-// Window voltage of the flagged windows F[f0 + w] as split fp16 [w][C][L],
-// scaled by a power of two per window so that |v| * scale <= 2^14.
-__global__ __launch_bounds__(kBlock) void k_gemm_pack(TemplateSet ts, PassState st, uint32_t f0, uint32_t nbatch,
-                                                      _Float16* __restrict__ Vh, _Float16* __restrict__ Vl,
-                                                      float* __restrict__ vscale)
-{
-    extern __shared__ float sw[];  // [C][L] the window's voltage, channel rows
-    __shared__ float red[kBlock / 64];
-    const int C = ts.C, L = ts.L;
-    const uint32_t nF = st.counters[CNT_F];
-    for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
-        const uint32_t w = f0 + wi;
-        _Float16* vh = Vh + (size_t)wi * C * L;
-        _Float16* vl = Vl + (size_t)wi * C * L;
-        const bool live = w < nF && !window_skipped(ts, st, st.F[w]);
-        const int64_t b = live ? (int64_t)st.best_idx[st.F[w]] : 0;
-        float m = 0.0f;
-        __syncthreads();
-        for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-            const int j = e / C, c = e - j * C;
-            const float v = live ? st.V[(size_t)(b + j) * C + c] : 0.0f;
-            sw[c * L + j] = v;
-            m = fmaxf(m, fabsf(v));
-        }
-        m = block_max(m, red);
-        float sc = 1.0f;
-        if (m > 0.0f) {
-            int ex;
-            frexpf(m, &ex);             // m < 2^ex
-            sc = ldexpf(1.0f, 14 - ex);  // m * sc < 2^14
-        }
-        for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
-            const float v = sw[e] * sc;
-            const _Float16 hi = (_Float16)v;
-            vh[e] = hi;
-            vl[e] = (_Float16)(v - (float)hi);
-        }
-        if (threadIdx.x == 0) vscale[wi] = sc;
-    }
-}
-
-void launch_gemm_A(const TemplateSet& ts, const int* units, int nunits, int c_lo, int K, int S, int n, float tscale,
-                   _Float16* Ah, _Float16* Al, hipStream_t s)
-{
-    const int64_t total = (int64_t)nunits * S * K;
-    hipLaunchKernelGGL(k_gemm_A, dim3(sgrid(total, kBlock, 1u << 20)), dim3(kBlock), 0, s, ts, units, nunits, c_lo, K, S,
-                       n, tscale, Ah, Al);
-}
-
-void launch_gemm_pack(const TemplateSet& ts, const PassState& st, uint32_t f0, uint32_t nbatch, _Float16* Vh,
-                      _Float16* Vl, float* vscale, hipStream_t s)
-{
-    if (nbatch == 0) return;
-    hipLaunchKernelGGL(k_gemm_pack, dim3(sgrid(nbatch, 1, 1u << 20)), dim3(kBlock), (size_t)ts.C * ts.L * 4, s, ts, st,
-                       f0, nbatch, Vh, Vl,
-                       vscale);
-}
 
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
                           uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s)

@@ -22,17 +22,18 @@ def handle():
     h.close()
 
 
-MODES = ["literal", "gemm", "valu", "stream"]
+MODES = ["literal", "split", "valu", "stream"]
 
 
 def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="literal", mixed=False):
-    """literal: "literal" (every overlap item), "gemm" (screen with split-fp16
-    GEMM correlations, the default path), "valu" (screen with FP32 vector
-    correlations) or "stream" (GEMM screen with the streaming selection
-    kernel, the path for U*(2n+1) > 24576); booleans map to literal / gemm."""
+    """literal: "literal" (every overlap item), "split" (screen with split-fp16
+    matrix-core correlations, the default path), "valu" (screen with FP32
+    vector correlations) or "stream" (split screen with the streaming
+    selection kernel, the path for U*(2n+1) > 24576); booleans map to
+    literal / split."""
     h.set_templates(T, bias, thr)
     flags = (_native.FLAG_OVERLAP_RECHECK if recheck else 0) | (_native.FLAG_ADJUSTED_CLIPS if clips else 0)
-    mode = {True: "literal", False: "gemm"}.get(literal, literal)
+    mode = {True: "literal", False: "split"}.get(literal, literal)
     if mode == "literal":
         flags |= _native.FLAG_RECHECK_LITERAL
     elif mode == "valu":
@@ -82,7 +83,7 @@ def test_correlated_pairs_match_oracle(handle, seed, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
 
 
-@pytest.mark.parametrize("recheck,literal", [(True, "literal"), (True, "gemm"), (True, "valu"), (False, "gemm")])
+@pytest.mark.parametrize("recheck,literal", [(True, "literal"), (True, "split"), (True, "valu"), (False, "split")])
 def test_small_workload_3s_matches_oracle(handle, recheck, literal):
     d = synthetic.generate("small", duration_s=3.0)
     T, bias, thr = synthetic.kernel_inputs(d)
@@ -96,7 +97,7 @@ def test_small_workload_3s_matches_oracle(handle, recheck, literal):
 def test_mixed_precision_noisy_cases_match_oracle(handle, seed):
     V, T, bias, thr, on, lab = cases.noisy_case(seed)
     n = T.shape[2] - 1
-    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=True),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
@@ -106,7 +107,7 @@ def test_mixed_precision_workloads_match_oracle(handle, name, dur):
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=True),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
@@ -118,14 +119,34 @@ def test_mixed_precision_shapes_match_oracle(handle, C, L, U, N):
     windows at the end of the chunk."""
     V, T, bias, thr, on, lab = cases.noisy_case(7 + C, C=C, L=L, U=U, N=N, nsp=max(4, N // 25))
     n = L - 1
-    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="gemm", mixed=True),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=True),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
+@pytest.mark.parametrize("literal", ["split", "valu"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_channel_dynamic_range_matches_oracle(handle, seed, literal):
+    """Channels whose amplitudes span 10^-7 .. 10^3 of each other, with tiny
+    values next to large ones in one window: the fp16 operands of the matrix
+    cores lose the small channels entirely (sub-normal or flushed), which the
+    bounds must absorb so that the decisions stay those of FP32."""
+    rng = np.random.default_rng(40 + seed)
+    U, C, L, N = 6, 8, 24, 3000
+    scale = np.array([1e3, 1.0, 1e-3, 1e-7, 3e2, 1e-5, 0.5, 2e-2], np.float32)
+    T = cases.dog_templates(U, C, L, seed).astype(np.float32) * scale[None, :, None]
+    V = (rng.normal(size=(C, N)).astype(np.float32) * 0.05) * scale[:, None]
+    for p in range(30, N - 2 * L, 41):
+        V[:, p:p + L] += T[(p // 41) % U] * np.float32(0.8 + 0.4 * rng.random())
+    bias, thr = cases.bias_thr(T.astype(np.float64), 0.05)
+    n = L - 1
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal=literal, mixed=True),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
 def test_mixed_precision_rescores_few_positions(handle):
     d = synthetic.generate("probe64", duration_s=0.5)
     T, bias, thr = synthetic.kernel_inputs(d)
-    gpu_run(handle, d["voltage"], T, bias, thr, d["n_max_shift_inds"], literal="gemm", mixed=True)
+    gpu_run(handle, d["voltage"], T, bias, thr, d["n_max_shift_inds"], literal="split", mixed=True)
     st = handle.stats()
     assert 0 < st["likelihood_rescored"] < 0.2 * st["likelihood_positions"]
 
@@ -139,7 +160,7 @@ def test_adjusted_clips_match_oracle(handle):
 
 
 @pytest.mark.parametrize("literal", MODES)
-@pytest.mark.parametrize("n_max", [0, 1, 3, 20])
+@pytest.mark.parametrize("n_max", [0, 1, 3, 20, 70])
 def test_shift_range_variants(handle, n_max, literal):
     V, T, bias, thr, on, lab = cases.noisy_case(20 + n_max, U=3, L=8)
     assert_same(gpu_run(handle, V, T, bias, thr, n_max, literal=literal),
@@ -193,7 +214,7 @@ def test_probe16_slice_matches_oracle(handle, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
-@pytest.mark.parametrize("literal", ["gemm", "valu"])
+@pytest.mark.parametrize("literal", ["split", "valu"])
 def test_probe64_slice_screened_matches_oracle(handle, literal):
     d = synthetic.generate("probe64", duration_s=0.1)
     T, bias, thr = synthetic.kernel_inputs(d)

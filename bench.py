@@ -23,6 +23,22 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
+TRAFFIC_PROFILE = os.path.join("profiles", "pursuit_probe64_600s_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-kernel scaled); None when the summary does not cover this workload."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
+    if workload != "probe64" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    return None
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -38,6 +54,9 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=0.1)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="for hardware counters: three 2 GiB device-to-device copies (the copy kernel of known "
+                         "size that scales FETCH_SIZE / WRITE_SIZE), then one mixed-precision drop-in call; no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -61,6 +80,19 @@ def main():
         m = dict(d["separability_metrics"])
         return bpp.binary_pursuit(V, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"],
                                   recheck_literal=literal, precision=precision)
+
+    if args.profile_run:
+        nbytes = 2 << 30
+        src = torch.ones(nbytes // 4, device=V.device, dtype=torch.float32)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        out = step("mixed")
+        torch.cuda.synchronize()
+        log(f"profile run: {nbytes} B copied 3 times; one mixed call, spikes={len(out[0])}")
+        return
 
     def timed(precision, k):
         torch.cuda.synchronize()
@@ -109,6 +141,7 @@ def main():
     sel_launch_ms = stm["ms_select"] / max(1, stm["n_select_launches"])
     sel_bytes_per_launch = stm["windows_flagged"] * sel_bytes_per_window / max(1, stm["n_select_launches"])
     sel_gbs = sel_bytes_per_launch / (sel_launch_ms * 1e-3) / 1e9 if sel_launch_ms > 0 else 0.0
+    traffic = profiled_traffic("k_screen_select_reg", args.workload) if args.duration is None else None
     mf_ms = stm["ms_likelihood"]
     mf_tflops = stm["likelihood_flops"] / (mf_ms * 1e-3) / 1e12 if mf_ms > 0 else 0.0
     lik_avg_ms = st["ms_likelihood"] / max(1, st["n_likelihood_launches"])
@@ -120,7 +153,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(sel_gbs / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
         "avg_launch_ms": round(sel_launch_ms, 4),
         "algorithmic_bytes_per_window": sel_bytes_per_window,
         "bytes_per_launch": round(sel_bytes_per_launch),

@@ -102,6 +102,45 @@ int fbp_copy_clips(fbp_handle *h, float *clips);
 int fbp_copy_residual(fbp_handle *h, float *out);
 int fbp_get_stats(fbp_handle *h, fbp_stats *out);
 
+/*
+ * Template refinement at fixed spike times.  No counterpart in the reference:
+ * it subtracts each found template once (binary_pursuit.cl:57-102) and never
+ * differentiates.  These calls serve a refinement loop that moves templates
+ * and per-spike amplitudes a_k downhill on the residual energy
+ *
+ *   R[c][x] = V[c][x] - sum_k a_k T[lab_k][c][x - p_k]   (0 <= x - p_k < L, x < N)
+ *   E       = 0.5 * sum_{c,x} R[c][x]^2                  (FP64 sums)
+ *
+ * with p_k = idx[k].  Every pointer is a DEVICE pointer; the work is queued on
+ * `stream` (a hipStream_t, NULL = the default stream) and the calls neither
+ * allocate nor synchronise.  All sums have a fixed order (no floating-point
+ * atomics): results are identical run to run.  Handle-free: on failure the
+ * message is fbp_refine_last_error() of the calling thread.
+ */
+
+/* Size in bytes of the `work` buffer both calls below need. */
+int fbp_refine_workspace_bytes(int64_t N, int64_t n_spikes, int32_t U, int32_t C, int32_t L, int64_t *bytes);
+
+/* R (row stride ldr) and energy[0] (double).  Spikes must be sorted by onset
+ * (idx ascending) with labels below U; otherwise energy[0] is NaN.  Spikes
+ * covering a sample are subtracted in list order, r = fmaf(-a_k, T, r). */
+int fbp_refine_forward(const float *V, int64_t ldv, int64_t N, int32_t C, const float *T, int32_t U, int32_t L,
+                       const uint32_t *idx, const uint32_t *lab, const float *amp, int64_t n_spikes,
+                       float *R, int64_t ldr, double *energy, void *work, int64_t work_bytes, void *stream);
+
+/* Gradients of E from the residual R of fbp_refine_forward:
+ *   grad_T[u][c][i] = -sum_{k of unit u, p_k+i < N} a_k R[c][p_k+i]     ([U][C][L], may be NULL)
+ *   grad_amp[k]     = -sum_{c, i: p_k+i < N} T[lab_k][c][i] R[c][p_k+i] ([n_spikes], may be NULL)
+ * by_unit lists the spike ids grouped by label and unit_offsets[U+1] is its
+ * CSR index (unit u owns by_unit[unit_offsets[u] .. unit_offsets[u+1]));
+ * grad_T sums each unit's spikes in chunks of 256 in by_unit order. */
+int fbp_refine_backward(const float *R, int64_t ldr, int64_t N, int32_t C, const float *T, int32_t U, int32_t L,
+                        const uint32_t *idx, const uint32_t *lab, const float *amp, int64_t n_spikes,
+                        const uint32_t *by_unit, const uint32_t *unit_offsets, float *grad_T, float *grad_amp,
+                        void *work, int64_t work_bytes, void *stream);
+
+const char *fbp_refine_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,7 +30,8 @@ FLAG_SELECT_STREAMING = 64
 EXPORTS = (
     "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
     "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
-    "fbp_get_stats",
+    "fbp_get_stats", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_backward",
+    "fbp_refine_last_error",
 )
 
 
@@ -90,11 +91,24 @@ def load():
     L.fbp_copy_clips.argtypes = [vp, vp]
     L.fbp_copy_residual.argtypes = [vp, vp]
     L.fbp_get_stats.argtypes = [vp, ctypes.POINTER(FbpStats)]
+    L.fbp_refine_workspace_bytes.argtypes = [i64, i64, i32, i32, i32, ctypes.POINTER(i64)]
+    L.fbp_refine_forward.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+    L.fbp_refine_backward.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, i64,
+                                      vp]
+    L.fbp_refine_last_error.argtypes = []
+    L.fbp_refine_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name != "fbp_last_error":
+        if name not in ("fbp_last_error", "fbp_refine_last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
+
+
+def refine_check(rc):
+    """Raise for a failed handle-free refinement call."""
+    if rc != FBP_OK:
+        msg = load().fbp_refine_last_error()
+        _raise(rc, msg.decode() if msg else f"error {rc}")
 
 
 def _raise(code, msg):

@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=0.1)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", default="pursuit", choices=["pursuit", "refine"],
+                    help="refine: template refinement on the 32-channel 120 s recording (its own JSON line)")
     ap.add_argument("--profile-run", action="store_true",
                     help="for hardware counters: three 2 GiB device-to-device copies (the copy kernel of known "
                          "size that scales FETCH_SIZE / WRITE_SIZE), then one mixed-precision drop-in call; no JSON")
@@ -62,6 +64,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(the path would shard along time with one all-gather of the spike lists)"}))
         sys.exit(2)
+
+    if args.variant == "refine":
+        return run_refine(args)
 
     import torch
     from spikesorting_fullpursuit_amd import synthetic
@@ -217,6 +222,125 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_refine(args):
+    """Differentiable variant: 32 ch, 30 kHz, 120 s, 60 units, 64-sample templates perturbed from the
+    truth by 10 % Gaussian noise; spike times from one mixed-precision pursuit with the perturbed
+    templates; 50 Adam steps on the residual energy; gradient throughput of forward + backward."""
+    import torch
+    from oracle import refine_fp64
+    from spikesorting_fullpursuit_amd import refine, synthetic
+    from spikesorting_fullpursuit_amd.overlap import binary_pursuit_parallel as bpp
+
+    t0 = time.time()
+    d = synthetic.generate_on_device("probe32", duration_s=args.duration)
+    V = d["voltage"]
+    C, N = V.shape
+    w = d["workload"]
+    truth = d["templates"].astype(np.float32)
+    U, _, L = truth.shape
+    rng = np.random.default_rng(w.seed + 1)
+    Tp = (truth * (1.0 + 0.1 * rng.standard_normal(truth.shape))).astype(np.float32)
+    SS, SS_by_chan, thr = synthetic.separability_inputs(Tp.astype(np.float64), w.fs)
+    m = {"templates": Tp.reshape(U, C * L).astype(np.float64), "template_SS": SS,
+         "template_SS_by_chan": SS_by_chan, "neuron_lower_thresholds": thr}
+    ev, lab, _, _ = bpp.binary_pursuit(V, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"])
+    onsets = np.asarray(ev, np.int64) - d["clip_init_samples"]
+    lab = np.asarray(lab, np.int64)
+    log(f"generated probe32 and pursued with perturbed templates: {len(onsets)} spikes "
+        f"({len(d['truth_onsets'])} planted) in {time.time() - t0:.1f}s")
+
+    refine.refine_templates(V, Tp, onsets, lab, steps=2, truth=truth)  # warm-up
+    torch.cuda.synchronize()
+    t1 = time.time()
+    Tref, amp, hist = refine.refine_templates(V, Tp, onsets, lab, steps=50, truth=truth)
+    torch.cuda.synchronize()
+    adam_s = time.time() - t1
+    log(f"50 Adam steps in {adam_s:.2f}s: template error {hist[0]['template_error']:.4f} -> "
+        f"{hist[-1]['template_error']:.4f}, energy {hist[0]['energy']:.6g} -> {hist[-1]['energy']:.6g}")
+
+    Td = torch.tensor(Tp, device=V.device)
+    table = refine.SpikeTable(onsets, lab, U, N, device=V.device)
+    a = torch.ones(table.n, device=V.device, dtype=torch.float32)
+    for _ in range(max(1, args.warmup)):
+        refine.time_gradient(V, Td, a, table, reps=1)
+    sec = refine.time_gradient(V, Td, a, table, reps=max(1, args.steps) * 10)
+    # per-kernel times with events on the current stream (the stream the kernels run on)
+    ev0, ev1, ev2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    reps = 20
+    ev0.record()
+    for _ in range(reps):
+        R, E = refine.forward_residual(V, Td, a, table)
+    ev1.record()
+    for _ in range(reps):
+        refine.backward_grads(R, Td, a, table)
+    ev2.record()
+    torch.cuda.synchronize()
+    fwd_ms = ev0.elapsed_time(ev1) / reps
+    bwd_ms = ev1.elapsed_time(ev2) / reps
+    fwd_bytes = 2 * 4 * C * N                     # read V, write R
+    bwd_bytes = 2 * 4 * table.n * C * L           # R at every spike window, once per gradient
+    fwd_gbs = fwd_bytes / (fwd_ms * 1e-3) / 1e9
+
+    # gradient check against the FP64 oracle on the whole recording (also the CPU baseline)
+    ns = N
+    sel = (onsets < ns)
+    Vs = V[:, :ns].contiguous()
+    tab_s = refine.SpikeTable(onsets[sel], lab[sel], U, ns, device=V.device)
+    a_s = torch.ones(tab_s.n, device=V.device, dtype=torch.float32)
+    Rs, Es = refine.forward_residual(Vs, Td, a_s, tab_s)
+    gT, ga = refine.backward_grads(Rs, Td, a_s, tab_s)
+    Vh = Vs.cpu().numpy()
+    t2 = time.perf_counter()
+    E64, gT64, ga64, _ = refine_fp64.gradients(Vh, Tp, onsets[sel], lab[sel], np.ones(int(sel.sum())))
+    tc = time.perf_counter() - t2
+    gT_err = float(np.max(np.abs(gT.cpu().numpy() - gT64)) / max(1e-30, np.max(np.abs(gT64))))
+    ga_err = float(np.max(np.abs(tab_s.from_sorted(ga).cpu().numpy() - ga64)) / max(1e-30, np.max(np.abs(ga64))))
+    E_err = abs(float(Es) - E64) / E64
+
+    value = N * C / sec / 1e6
+    line = {
+        "metric": "template-refinement gradient throughput Msamples·ch/s at 32ch (residual energy forward + "
+                  "backward w.r.t. templates and amplitudes)",
+        "value": round(value, 3),
+        "unit": "Msamples·ch/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps) * 10,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": round(1000 * sec, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (energy sums in f64)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"probe32: {C} ch, {w.fs / 1000:g} kHz, {N / w.fs:g} s, {U} units, L={L}, templates = "
+                        f"truth x (1 + 0.1 N(0,1)), spike times from one mixed-precision pursuit",
+            "spikes": int(table.n),
+            "spikes_per_s": round(table.n / sec, 1),
+            "adam_steps": 50,
+            "adam_s_for_50_steps": round(adam_s, 3),
+            "template_error_before": round(hist[0]["template_error"], 6),
+            "template_error_after": round(hist[-1]["template_error"], 6),
+            "energy_before": hist[0]["energy"],
+            "energy_after": hist[-1]["energy"],
+            "vs_fp64": {"energy_rel": E_err, "grad_templates_rel_max": gT_err,
+                                 "grad_amplitudes_rel_max": ga_err},
+        },
+        "roofline": {
+            "kernel": "k_residual_energy (forward: residual and FP64 energy partials)",
+            "bound": "hbm", "achieved": round(fwd_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(fwd_gbs / HBM_PEAK_GBS, 5), "traffic": None,
+            "avg_launch_ms": round(fwd_ms, 4), "bytes_per_launch": fwd_bytes,
+            "backward_ms": round(bwd_ms, 4),
+            "backward_algorithmic_gbs": round(bwd_bytes / (bwd_ms * 1e-3) / 1e9, 2),
+        },
+        "cpu_baseline": {"value": round(ns * C / tc / 1e6, 6), "unit": "Msamples·ch/s", "cores": 1, "kind": "port",
+                         "sample": f"FP64 NumPy residual energy and gradients, whole recording ({ns / w.fs:g} s, "
+                                   f"{int(sel.sum())} spikes), {tc:.2f} s"},
     }
     print(json.dumps(line), flush=True)
 

@@ -28,6 +28,7 @@ constexpr int kTile = 256;   // samples per block of the residual kernel
 constexpr int kSpan = 512;   // spikes staged in LDS per round of the residual kernel
 constexpr int kChunk = 256;  // spikes per partial sum of the template gradient
 constexpr int kRed = 1024;   // threads of the final reduction
+constexpr int kCG = 16;      // channels held in registers per thread by the residual kernel
 
 __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
@@ -137,16 +138,30 @@ __global__ __launch_bounds__(kTile) void k_residual_energy(const float* __restri
         __syncthreads();
         if (x < N) {
             const bool last = (r == nround - 1);
-            for (int c = 0; c < C; c++) {
-                float v = (r == 0) ? V[(int64_t)c * ldv + x] : R[(int64_t)c * ldr + x];
+            // kCG channels at a time in registers: their loads are issued together and each
+            // covering spike is tested once per group
+            for (int c0 = 0; c0 < C; c0 += kCG) {
+                float v[kCG];
+#pragma unroll
+                for (int cc = 0; cc < kCG; cc++)
+                    if (c0 + cc < C) v[cc] = (r == 0) ? V[(int64_t)(c0 + cc) * ldv + x] : R[(int64_t)(c0 + cc) * ldr + x];
                 for (int j = 0; j < cnt; j++) {
                     const int64_t d = x - (int64_t)s_p[j];
                     const uint32_t u = s_l[j];
-                    if (d >= 0 && d < L && u < (uint32_t)U)
-                        v = fmaf(-s_a[j], T[((int64_t)u * C + c) * L + d], v);
+                    if (d < 0 || d >= L || u >= (uint32_t)U) continue;
+                    const float na = -s_a[j];
+                    const float* t = T + ((int64_t)u * C + c0) * L + d;
+#pragma unroll
+                    for (int cc = 0; cc < kCG; cc++)
+                        if (c0 + cc < C) v[cc] = fmaf(na, t[(int64_t)cc * L], v[cc]);
                 }
-                R[(int64_t)c * ldr + x] = v;
-                if (last) e += (double)v * (double)v;
+#pragma unroll
+                for (int cc = 0; cc < kCG; cc++) {
+                    if (c0 + cc < C) {
+                        R[(int64_t)(c0 + cc) * ldr + x] = v[cc];
+                        if (last) e += (double)v[cc] * (double)v[cc];
+                    }
+                }
             }
         }
     }

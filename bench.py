@@ -54,8 +54,10 @@ def main():
     ap.add_argument("--cpu-sample-s", type=float, default=0.1)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--variant", default="pursuit", choices=["pursuit", "refine"],
-                    help="refine: template refinement on the 32-channel 120 s recording (its own JSON line)")
+    ap.add_argument("--variant", default="pursuit", choices=["pursuit", "refine", "batch"],
+                    help="refine: template refinement on the 32-channel 120 s recording; batch: 256 independent "
+                         "segments (5-120 s, 1-32 ch, 4-60 units) on concurrent streams (each its own JSON line)")
+    ap.add_argument("--segments", type=int, default=256, help="segments of the batch variant")
     ap.add_argument("--profile-run", action="store_true",
                     help="for hardware counters: three 2 GiB device-to-device copies (the copy kernel of known "
                          "size that scales FETCH_SIZE / WRITE_SIZE), then one mixed-precision drop-in call; no JSON")
@@ -67,6 +69,8 @@ def main():
 
     if args.variant == "refine":
         return run_refine(args)
+    if args.variant == "batch":
+        return run_batch(args)
 
     import torch
     from spikesorting_fullpursuit_amd import synthetic
@@ -222,6 +226,74 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_batch(args):
+    """Batched variant: independent segments with lengths 5-120 s, channel counts from {1, 4, 8, 16, 32}
+    and 4-60 units (64-sample templates, 30 kHz), voltage resident in HBM.  Per-segment plain calls (one
+    stream, one segment after another) against the batch on concurrent streams with the worker count
+    picked by the autotuner; both must give the identical spike set."""
+    import torch
+    from spikesorting_fullpursuit_amd import batch, synthetic
+
+    t0 = time.time()
+    ws = synthetic.batch_workloads(args.segments)
+    segs = []
+    for i, w in enumerate(ws):
+        d = synthetic.generate_workload_on_device(w, pow2_fft=True)
+        segs.append({"voltage": d["voltage"], "sort_info": d["sort_info"],
+                     "separability_metrics": d["separability_metrics"], "n_max_shift_inds": d["n_max_shift_inds"]})
+        if i % 32 == 31:
+            log(f"generated {i + 1} segments in {time.time() - t0:.1f}s")
+    work = sum(w.n_samples * w.n_chans for w in ws)
+    log(f"generated {len(segs)} segments, {work / 1e6:.1f} Msamples*ch, in {time.time() - t0:.1f}s")
+
+    best, rates = batch.autotune_workers(segs, candidates=(1, 2, 4, 8), sample=16)
+    log(f"autotune: workers {best}, Msamples*ch/s by workers {rates}")
+    for _ in range(args.warmup):
+        batch.binary_pursuit_batch(segs[:8], n_workers=best)
+    st1, stb = {}, {}
+    torch.cuda.synchronize()
+    one = batch.binary_pursuit_batch(segs, n_workers=1, stats=st1)
+    torch.cuda.synchronize()
+    log(f"per-segment calls on one stream: {st1['seconds']:.2f}s")
+    many = batch.binary_pursuit_batch(segs, n_workers=best, stats=stb)
+    torch.cuda.synchronize()
+    log(f"batch on {best} streams: {stb['seconds']:.2f}s")
+    identical = all(np.array_equal(np.asarray(a[0]), np.asarray(b[0])) and np.array_equal(np.asarray(a[1]),
+                                                                                          np.asarray(b[1]))
+                    for a, b in zip(one, many))
+    n_spk = int(sum(len(r[0]) for r in many))
+    value = work / stb["seconds"] / 1e6
+    line = {
+        "metric": "batched pursuit Msamples·ch/s over 256 independent segments (concurrent streams) vs "
+                  "per-segment plain calls at identical spike set",
+        "value": round(value, 3),
+        "unit": "Msamples·ch/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * stb["seconds"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 MFMA + f32 (decisions in exact f32)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(segs)} segments, 5-120 s, channels {{1,4,8,16,32}}, 4-60 units, L=64, 30 kHz, "
+                        f"{work / 1e6:.1f} Msamples·ch in all, voltage resident in HBM",
+            "workers": best,
+            "autotune_msamples_ch_per_s": {str(k): round(v, 2) for k, v in rates.items()},
+            "per_segment_value": round(work / st1["seconds"] / 1e6, 3),
+            "per_segment_seconds": round(st1["seconds"], 3),
+            "batch_over_per_segment": round(st1["seconds"] / stb["seconds"], 3),
+            "identical_spike_set": bool(identical),
+            "spikes": n_spk,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
 

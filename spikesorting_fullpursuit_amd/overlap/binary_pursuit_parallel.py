@@ -79,7 +79,7 @@ def _is_device_tensor(x):
 
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
                    kernels_path=None, max_gpu_memory=None, *, device=0, return_residual=False,
-                   recheck_literal=False, precision="mixed", stats=None):
+                   recheck_literal=False, precision="mixed", stats=None, handle=None):
     """Binary pursuit over every chunk of `voltage` ([C][N], any real dtype;
     a float32 torch tensor already on the GPU is used in place).
 
@@ -93,7 +93,9 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     products and recomputes in exact FP32 every value that can influence a
     decision, "fp32" computes both on the FP32 vector units -- the two give the
     same spikes and residual; `recheck_literal` evaluates every overlap item
-    instead of screening them.
+    instead of screening them; `handle` runs on that library handle (its own
+    stream and buffers) instead of the cached one of `device`, so that several
+    threads can run pursuits concurrently.
     """
     if precision not in ("mixed", "fp32"):
         raise ValueError("precision must be 'mixed' or 'fp32'")
@@ -117,7 +119,7 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     do_overlap_recheck = sort_info["do_overlap_recheck"]
     get_adjusted_clips = sort_info["get_adjusted_clips"]
 
-    h = _handle(device)
+    h = handle if handle is not None else _handle(device)
     if n_max_shift_inds is None or n_max_shift_inds < 0:
         n_max_shift_inds = template_samples_per_chan // 2
     n_max_shift_inds = np.uint32(n_max_shift_inds)

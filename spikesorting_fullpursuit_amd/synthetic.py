@@ -17,6 +17,7 @@ are computed as SF/overlap/neuron_separability.py:113-195 and :319-331 do:
   thr         = max(-0.5*SS + sigma_bp_noise*sqrt(var), 0, 0.5*SS - sigma_bp_CI*sqrt(var))
 with sigma_bp_noise = 3.719 and sigma_bp_CI = inf (SF/parallel/spikesorting_parallel.py:76-77).
 """
+import functools
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -186,11 +187,37 @@ def clip_width_for(L, fs):
     return [-pre / fs, post / fs], pre
 
 
+BATCH_CHANNELS = (1, 4, 8, 16, 32)
+
+
+def batch_workloads(n_segments=256, seed=20261204, duration_range=(5.0, 120.0), unit_range=(4, 60),
+                    channels=BATCH_CHANNELS, fs=30000.0, L=64):
+    """The batched workload: independent segments with lengths drawn uniformly
+    from duration_range (s), channel counts from `channels` and unit counts
+    from unit_range (inclusive), 64-sample templates at 30 kHz.  Channels sit
+    on a line at 25 um pitch; units span all channels of a segment with at most
+    4 channels and 5 adjacent channels otherwise; rates 1-30 Hz."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_segments):
+        dur = float(rng.uniform(*duration_range))
+        C = int(rng.choice(channels))
+        U = int(rng.integers(unit_range[0], unit_range[1] + 1))
+        xy = np.stack([np.zeros(C), 25.0 * np.arange(C)], axis=1)
+        nb = 0 if C <= 4 else 5
+        out.append(Workload(f"segment{i}", C, fs, dur, U, L, xy, nb, (1.0, 30.0), 0.0, seed + 1 + i))
+    return out
+
+
 def generate(name, duration_s=None, seed=None):
     """Build a workload.  Returns a dict with voltage [C][N] float32, templates
     [U][C][L] float64, truth (onsets, labels), sort_info and separability_metrics
     in the form SF/overlap/binary_pursuit_parallel.py:175-183 consumes."""
-    w = workload(name, duration_s, seed)
+    return generate_workload(workload(name, duration_s, seed))
+
+
+def generate_workload(w):
+    """generate() for a Workload description."""
     rng = np.random.default_rng(w.seed)
     N = w.n_samples
     T = make_templates(w, rng)
@@ -269,26 +296,37 @@ def generate_on_device(name, duration_s=None, seed=None, device="cuda"):
     domain by the zero-phase filter's squared magnitude response and rescaled
     to sd 1.  Returns the dict of generate() with `voltage` a float32 device
     tensor [C][N] (the realisation differs from generate()'s)."""
+    return generate_workload_on_device(workload(name, duration_s, seed), device)
+
+
+@functools.lru_cache(maxsize=8)
+def _power_gain(fs, nfft):
+    """|H(f)|^2 of the zero-phase low-pass at the rfft frequencies of length nfft."""
+    freqs = np.fft.rfftfreq(nfft, d=1.0 / fs)
+    _, h = signal.sosfreqz(lowpass_sos(fs), worN=freqs, fs=fs)
+    return (np.abs(h) ** 2).astype(np.float32)
+
+
+def generate_workload_on_device(w, device="cuda", pow2_fft=False):
+    """generate_on_device() for a Workload description.  pow2_fft shapes the
+    noise on the next power-of-two length and keeps the first N samples, so a
+    batch of segments of many lengths needs only a few FFT plans."""
     import torch
 
-    w = workload(name, duration_s, seed)
     rng = np.random.default_rng(w.seed)
     N = w.n_samples
     T = make_templates(w, rng)
     onsets, labels, amp = spike_trains(w, rng, N)
     gen = torch.Generator(device=device)
     gen.manual_seed(int(w.seed))
-    sos = lowpass_sos(w.fs)
-    nfft = N
-    freqs = np.fft.rfftfreq(nfft, d=1.0 / w.fs)
-    _, h = signal.sosfreqz(sos, worN=freqs, fs=w.fs)
-    gain = torch.from_numpy((np.abs(h) ** 2).astype(np.float32)).to(device)
+    nfft = 1 << max(0, (N - 1).bit_length()) if pow2_fft else N
+    gain = torch.from_numpy(_power_gain(float(w.fs), nfft)).to(device)
 
     def shaped(k):
-        x = torch.randn((k, N), generator=gen, device=device, dtype=torch.float32)
+        x = torch.randn((k, nfft), generator=gen, device=device, dtype=torch.float32)
         X = torch.fft.rfft(x, dim=1)
         X *= gain
-        y = torch.fft.irfft(X, n=N, dim=1)
+        y = torch.fft.irfft(X, n=nfft, dim=1)[:, :N]
         y /= y.std(dim=1, keepdim=True)
         return y
 
@@ -308,16 +346,17 @@ def generate_on_device(name, duration_s=None, seed=None, device="cuda"):
     on_t = torch.from_numpy(onsets).to(device)
     lab_t = torch.from_numpy(labels).to(device)
     amp_t = torch.from_numpy(amp.astype(np.float32)).to(device)
-    for u in range(U):
-        sel = lab_t == u
-        if not bool(sel.any()):
+    nz = torch.from_numpy(np.any(T != 0, axis=2)).to(device)  # [U][C] channels each template touches
+    # every (spike, touched channel) row at once, in slices of spikes to bound the index tensors
+    step = max(1, (1 << 24) // max(1, C * L))
+    for k0 in range(0, len(onsets), step):
+        lab_k = lab_t[k0:k0 + step]
+        ks, cs = nz[lab_k].nonzero(as_tuple=True)
+        if ks.numel() == 0:
             continue
-        on = on_t[sel]
-        a = amp_t[sel]
-        chans = [c for c in range(C) if bool(torch.any(Tf[u, c] != 0))]
-        for c in chans:
-            idx = (c * N + on[:, None] + ar[None, :]).reshape(-1)
-            val = (a[:, None] * Tf[u, c][None, :]).reshape(-1)
-            flat.index_add_(0, idx, val)
-    torch.cuda.synchronize()
+        idx = ((cs * N + on_t[k0:k0 + step][ks])[:, None] + ar[None, :]).reshape(-1)
+        val = (amp_t[k0:k0 + step][ks][:, None] * Tf[lab_k[ks], cs]).reshape(-1)
+        flat.index_add_(0, idx, val)
+    if V.is_cuda:
+        torch.cuda.synchronize()
     return package(w, V, T, onsets, labels)

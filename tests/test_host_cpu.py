@@ -119,3 +119,49 @@ def test_missing_sort_info_key_raises_keyerror():
           "get_adjusted_clips": False}
     with pytest.raises(KeyError):
         bpp.binary_pursuit(V, np.float32, si, m)
+
+
+@pytest.mark.skipif(_has_gpu(), reason="checks the no-GPU error path")
+def test_batch_and_refine_have_no_cpu_fallback():
+    from spikesorting_fullpursuit_amd import batch, refine
+    V = np.zeros((2, 1000), np.float32)
+    m = {"templates": np.ones((1, 16)), "template_SS": np.array([16.0]),
+         "template_SS_by_chan": np.ones((1, 2)), "neuron_lower_thresholds": np.array([1.0])}
+    si = {"sampling_rate": 40000.0, "clip_width": [-4 / 40000, 3 / 40000], "n_samples_per_chan": 8,
+          "do_overlap_recheck": True, "get_adjusted_clips": False}
+    with pytest.raises(RuntimeError, match="No GPU available"):
+        batch.binary_pursuit_batch([{"voltage": V, "sort_info": si, "separability_metrics": m}])
+    import torch
+    table = type("T", (), {"N": 1000, "U": 1, "n": 0})()
+    with pytest.raises(ValueError, match="device tensors"):
+        refine.forward_residual(torch.zeros(2, 1000), torch.zeros(1, 2, 8), torch.zeros(0), table)
+
+
+def test_batch_workloads_cover_the_configured_mix():
+    from spikesorting_fullpursuit_amd import synthetic
+    ws = synthetic.batch_workloads()
+    assert len(ws) == 256
+    assert {w.n_chans for w in ws} == set(synthetic.BATCH_CHANNELS)
+    assert all(5.0 <= w.duration_s <= 120.0 and 4 <= w.n_units <= 60 and w.L == 64 for w in ws)
+    total = sum(w.n_samples * w.n_chans for w in ws) / 1e6
+    assert 4500 < total < 7200  # expected 256 * 62.5 s * 12.2 ch * 30 kHz = 5856 Msamples*ch
+    assert len({w.seed for w in ws}) == 256
+
+
+def test_refine_abi_checks_arguments_without_a_gpu():
+    import ctypes
+    lib = _native.load()
+    nb = ctypes.c_int64(0)
+    assert lib.fbp_refine_workspace_bytes(1000, 10, 3, 4, 32, ctypes.byref(nb)) == _native.FBP_OK
+    # forward: 4 + 1 (256-B aligned) partials; backward: chunk table + (ceil(10/256) + 3) chunks of C*L floats
+    assert nb.value == max(256 + 256, 256 + 4 * 4 * 32 * 4)
+    assert lib.fbp_refine_workspace_bytes(0, 10, 3, 4, 32, ctypes.byref(nb)) == _native.FBP_E_INVALID
+    assert b"positive" in lib.fbp_refine_last_error()
+    assert lib.fbp_refine_workspace_bytes(1000, 10, 3, 1 << 13, 1 << 12, ctypes.byref(nb)) == _native.FBP_E_UNSUPPORTED
+    vp = ctypes.c_void_p
+    rc = lib.fbp_refine_forward(None, 1000, 1000, 4, None, 3, 32, None, None, None, 10, None, 1000, None, None, 0,
+                                None)
+    assert rc == _native.FBP_E_INVALID and b"null" in lib.fbp_refine_last_error()
+    rc = lib.fbp_refine_backward(vp(8), 1000, 1000, 4, vp(8), 3, 32, vp(8), vp(8), vp(8), 10, vp(8), vp(8), None,
+                                 None, vp(8), 1 << 20, None)
+    assert rc == _native.FBP_E_INVALID and b"no gradient" in lib.fbp_refine_last_error()

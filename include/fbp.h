@@ -68,6 +68,9 @@ typedef struct fbp_stats {
     int64_t screen_separable_pairs; /* (unit, shift) pairs sharing no channel enumerated over f */
     double ms_select;             /* overlap screen selection kernel (part of ms_recheck), with FBP_FLAG_TIMING */
     int64_t n_select_launches;
+    int64_t likelihood_pairs_computed; /* (window, template) pairs whose likelihood was computed; the others
+                                          had no channel change in the window's samples since their last
+                                          computation and kept their (identical) values */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

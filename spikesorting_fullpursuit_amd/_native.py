@@ -60,6 +60,7 @@ class FbpStats(ctypes.Structure):
         ("screen_separable_pairs", ctypes.c_int64),
         ("ms_select", ctypes.c_double),
         ("n_select_launches", ctypes.c_int64),
+        ("likelihood_pairs_computed", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -23,7 +23,10 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t s);
 void launch_to_sample_major(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
 void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N, hipStream_t s);
 bool likelihood_fits(const TemplateSet& ts);
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s);
+void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_t* sidx, const uint32_t* slab,
+                       uint32_t nsp, hipStream_t s);
+void launch_clear_dirty(const PassState& st, uint32_t nW, hipStream_t s);
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
 void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
@@ -57,13 +60,16 @@ struct MfTables {
     const int* aoff;
     const int* chb;
     const float* T1;
+    const int* tileof;
     int ntiles;
     float inv_tscale;
     float kl, kb, ka;
 };
+size_t likmfma_lds(const TemplateSet& ts, int ntiles);
 void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s);
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
-                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, hipStream_t s);
+                    const float* vs, const uint32_t* mbits, const uint64_t* dirty, uint32_t w0, uint32_t nb,
+                    hipStream_t s);
 constexpr int kScreenMfmaMaxS = 128;  // shifts of the matrix-core screen (8 tiles of 16)
 void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const void* fragl,
                         const float* Ntr, int S, int n, uint32_t f0, uint32_t nbatch, float* Bbuf, hipStream_t s);
@@ -135,7 +141,7 @@ struct fbp_handle {
     // templates
     int U = 0, C = 0, L = 0, Lp = 0;
     bool have_templates = false;
-    DevBuf T, rows, nch, off, chan, nz, bias, thr;
+    DevBuf T, rows, nch, off, chan, nz, bias, thr, cmask;
     std::vector<int> h_nch;
     std::vector<uint8_t> h_nz;
     std::vector<float> h_rowL1, h_SS;
@@ -149,7 +155,7 @@ struct fbp_handle {
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
-    DevBuf mf_frag, mf_fragl, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs;
+    DevBuf mf_frag, mf_fragl, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs, mf_tileof;
     int mf_ntiles = 0;
     float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
     float scr_k2 = 0.0f, scr_k3 = 0.0f, scr_k4 = 0.0f;  // screen bound terms of the split-fp16 B
@@ -157,7 +163,7 @@ struct fbp_handle {
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, Vtmp, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
-    DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf;
+    DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf, dirty;
     DevBuf part_v, part_k;
     DevBuf sp_idx, sp_lab, clips;
     uint32_t* h_counters = nullptr;  // pinned
@@ -195,6 +201,7 @@ static TemplateSet template_view(const fbp_handle* h)
     ts.nz = h->nz.as<uint8_t>();
     ts.bias = h->bias.as<float>();
     ts.thr = h->thr.as<float>();
+    ts.cmask = h->cmask.as<uint64_t>();
     ts.U = h->U;
     ts.C = h->C;
     ts.L = h->L;
@@ -225,6 +232,8 @@ static PassState pass_view(const fbp_handle* h)
     st.cand_idx = h->cand_idx.as<uint32_t>();
     st.cand_lab = h->cand_lab.as<uint32_t>();
     st.counters = h->counters.as<uint32_t>();
+    // channel masks are 64-bit: with more channels every template is recomputed
+    st.dirty = h->C <= 64 ? h->dirty.as<uint64_t>() : nullptr;
     st.N = h->N;
     st.nwin = h->nwin;
     return st;
@@ -325,6 +334,7 @@ static int build_mf_tables(fbp_handle* h)
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
     const int ntiles = (U + 15) / 16;
+    if (likmfma_lds(template_view(h), ntiles) > kMaxLdsSelect) return FBP_OK;  // beyond 160 KiB: FP32 kernels
     std::vector<int> units((size_t)ntiles * 16, -1), nkb(ntiles), nblk(ntiles), aoff(ntiles), chb(ntiles, 0);
     int total_kb = 0, maxnb = 1;
     for (int t = 0; t < ntiles; t++) {
@@ -369,6 +379,10 @@ static int build_mf_tables(fbp_handle* h)
                     frag[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = hi;
                     fragl[(((size_t)aoff[t] + kb) * 64 + lane) * 8 + i] = (_Float16)(v - (float)hi);
                 }
+    std::vector<int> tileof(U, 0);
+    for (int t = 0; t < ntiles; t++)
+        for (int r = 0; r < 16; r++)
+            if (units[(size_t)t * 16 + r] >= 0) tileof[units[(size_t)t * 16 + r]] = t;
     std::vector<float> T1(U);
     for (int u = 0; u < U; u++) {
         double a = 0.0;
@@ -383,6 +397,7 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(h->mf_aoff.ensure((size_t)ntiles * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_chb.ensure(chb.size() * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_T1.ensure((size_t)U * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_tileof.ensure((size_t)U * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_mbits.ensure(4), "alloc likelihood tables");
     HIP_TRY(h->mf_vs.ensure(4), "alloc likelihood tables");
     HIP_TRY(hipMemcpy(h->mf_frag.p, frag.data(), frag.size() * sizeof(_Float16), hipMemcpyHostToDevice), "copy");
@@ -393,6 +408,7 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(hipMemcpy(h->mf_aoff.p, aoff.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_chb.p, chb.data(), chb.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_T1.p, T1.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_tileof.p, tileof.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
     h->mf_ntiles = ntiles;
     // Likelihood: delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts).
     // An operand x (scaled) enters as fp16 x' with |x - x'| <= 2^-11 |x| + 2^-14
@@ -440,6 +456,7 @@ static MfTables mf_view(const fbp_handle* h)
     mt.aoff = h->mf_aoff.as<int>();
     mt.chb = h->mf_chb.as<int>();
     mt.T1 = h->mf_T1.as<float>();
+    mt.tileof = h->mf_tileof.as<int>();
     mt.ntiles = h->mf_ntiles;
     mt.inv_tscale = 1.0f / h->mf_tscale;
     mt.kl = h->mf_kl;
@@ -605,6 +622,15 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
     HIP_TRY(hipMemcpy(h->nz.p, nz.data(), nz.size(), hipMemcpyHostToDevice), "copy nz");
     HIP_TRY(hipMemcpy(h->bias.p, bias, U * 4, hipMemcpyHostToDevice), "copy bias");
     HIP_TRY(hipMemcpy(h->thr.p, thresholds, U * 4, hipMemcpyHostToDevice), "copy thr");
+    {
+        // channel bit masks of the templates (used when C <= 64)
+        std::vector<uint64_t> cm(U, 0ull);
+        for (int u = 0; u < U; u++)
+            for (int c = 0; c < C && c < 64; c++)
+                if (nz[(size_t)u * C + c]) cm[u] |= 1ull << c;
+        HIP_TRY(h->cmask.ensure((size_t)U * 8), "alloc channel masks");
+        HIP_TRY(hipMemcpy(h->cmask.p, cm.data(), (size_t)U * 8, hipMemcpyHostToDevice), "copy channel masks");
+    }
     h->U = U;
     h->C = C;
     h->L = L;
@@ -683,6 +709,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(h->cand_lab.ensure(nwin * 4), "alloc");
     HIP_TRY(h->counters.ensure(CNT_N * 4), "alloc");
     HIP_TRY(h->blockbuf.ensure(((nwin + kCompactTile - 1) / kCompactTile + 1) * 4), "alloc");
+    HIP_TRY(h->dirty.ensure(nwin * 8), "alloc");
     HIP_TRY(grow_keep(h->sp_idx, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
     HIP_TRY(grow_keep(h->sp_lab, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
 
@@ -729,6 +756,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     launch_fill_u8(st.orc, 0, nwin, s);
     launch_fill_u8(st.giveup, 0, nwin, s);
     launch_fill_u32(st.counters, 0u, CNT_N, s);
+    if (st.dirty) launch_fill_u32(reinterpret_cast<uint32_t*>(st.dirty), 0u, 2 * nwin, s);
     launch_iota(st.W, nwin, s);
 
     fbp_stats stats{};
@@ -751,19 +779,29 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
+    int64_t pairs_total = 0;
     for (;;) {
+        launch_fill_u32(st.counters + CNT_PAIRS, 0u, 1, s);
         stats.windows_listed += nW;
         stats.likelihood_positions += (int64_t)nW * L * U;
         stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
         tm.begin(TM_LIK);
-        if (mixed) {
+        // the first pass computes every window; later passes only the templates
+        // whose channels changed in a window's samples since it was computed
+        const uint64_t* dirty = passes > 0 ? st.dirty : nullptr;
+        // The matrix cores take the dense first pass.  Later passes recompute a
+        // few templates per window, which the vector-unit kernel skips one by one
+        // (the matrix-core kernel only by 16-template tiles over 4 windows).
+        // Both kernels leave exact values wherever a decision can depend on them.
+        if (mixed && dirty == nullptr) {
             const MfTables mt = mf_view(h);
             for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch)
-                launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(), w0,
-                               std::min(lik_batch, nW - w0), s);
+                launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(),
+                               dirty, w0, std::min(lik_batch, nW - w0), s);
         } else {
-            launch_likelihood(ts, st, nW, s);
+            launch_likelihood(ts, st, nW, dirty, s);
         }
+        launch_clear_dirty(st, nW, s);
         tm.end(TM_LIK);
         stats.n_likelihood_launches++;
         tm.begin(TM_MAX);
@@ -827,10 +865,12 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         HIP_TRY(hipMemcpyAsync(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost, s), "read counters");
         HIP_TRY(hipStreamSynchronize(s), "sync");
         passes++;
+        pairs_total += h->h_counters[CNT_PAIRS];
         const uint32_t nadd = h->h_counters[CNT_SPIKES];
         if (nadd == 0) break;
         tm.begin(TM_RESID);
         launch_residual(ts, st.V, N, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, reach, s);
+        launch_mark_dirty(ts, st, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, s);
         tm.end(TM_RESID);
         total += nadd;
         nW = h->h_counters[CNT_W];
@@ -864,6 +904,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.screen_separable_pairs = h->h_counters[CNT_NSP];
     stats.passes = passes;
     stats.spikes = total;
+    stats.likelihood_pairs_computed = pairs_total;
     h->stats = stats;
     h->n_spikes = total;
     h->have_result = true;

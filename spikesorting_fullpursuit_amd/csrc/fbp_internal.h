@@ -21,6 +21,7 @@ struct TemplateSet {
     const uint8_t* nz;     // [U][C] 1 where the template row is not all zero
     const float* bias;     // [U]
     const float* thr;      // [U]
+    const uint64_t* cmask; // [U] bit c: template row c is not all zero (C <= 64)
     int U, C, L, Lp;
     int R;                 // rows of the compact layout
 };
@@ -46,6 +47,12 @@ struct PassState {
     uint32_t* cand_idx;    // [nwin] per listed window: candidate spike
     uint32_t* cand_lab;
     uint32_t* counters;    // [8] device counters (see Counter)
+    // [nwin] channels (bit c) whose residual changed, since the window's
+    // likelihood was last computed, somewhere in the samples its positions
+    // read ([wL, wL + 2L - 1)).  A template whose channels miss that mask would
+    // be recomputed from identical inputs, i.e. to identical values, so the
+    // likelihood kernels leave it as it is.  nullptr: compute every template.
+    uint64_t* dirty;
     int64_t N;
     int64_t nwin;
 };
@@ -76,6 +83,7 @@ struct ScreenTables {
     float kappa3, kappa4; // absolute residue of the fp16 split (0 for the FP32 B)
 };
 
-enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7, CNT_N = 8 };
+enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7,
+               CNT_PAIRS = 8, CNT_N = 9 };
 
 }  // namespace fbp

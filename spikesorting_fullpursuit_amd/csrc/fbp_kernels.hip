@@ -18,13 +18,23 @@ namespace fbp {
 // register window over the voltage row staged in LDS.
 // ---------------------------------------------------------------------------
 
+constexpr int kLikMaxWb = 16;  // windows per likelihood block at most
+
 template <int LT>
 __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
                                                        const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
                                                        float* __restrict__ LIK, float* __restrict__ wmax_v,
-                                                       uint16_t* __restrict__ wmax_p)
+                                                       uint16_t* __restrict__ wmax_p,
+                                                       const uint64_t* __restrict__ dirty,
+                                                       uint32_t* __restrict__ cnt_pairs)
 {
-    extern __shared__ float sV[];  // [wb][C][sv], then per-item maxima [wb][U][ngr] (value, position)
+    // [wb][C][sv] voltage rows, then per-item maxima [wb][U][ngr] (value, position),
+    // then the block's (window, template) pairs to compute [wb][U], then the
+    // channels each window stages [wb][C]
+    extern __shared__ float sV[];
+    __shared__ uint32_t npair_s;
+    __shared__ unsigned long long cun_s[kLikMaxWb];
+    __shared__ int ncl_s[kLikMaxWb];
     const int C = ts.C, U = ts.U;
     const int L = (LT > 0) ? LT : ts.L;
     const int ngr = (L + kPos - 1) / kPos;
@@ -32,23 +42,80 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     const uint32_t nbatch = (nW + wb - 1) / wb;
     float* pv = sV + wb * C * sv;
     uint32_t* pp = reinterpret_cast<uint32_t*>(pv + wb * per_win);
+    uint32_t* plist = pp + wb * per_win;
+    int* clist = reinterpret_cast<int*>(plist + wb * U);
     for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
         const uint32_t w0 = bt * wb;
         const int nb = (int)min((uint32_t)wb, nW - w0);
         __syncthreads();
-        // V is sample-major ([N][C]): read along channels, store channel rows
-        for (int e = threadIdx.x; e < nb * C * sv; e += blockDim.x) {
-            const int b = e / (C * sv);
-            const int rem = e - b * C * sv;
-            const int j = rem / C, c = rem - j * C;
-            const int64_t x = (int64_t)W[w0 + b] * L + j;
-            sV[(b * C + c) * sv + j] = (x < N) ? V[(size_t)x * C + c] : 0.0f;
+        if (threadIdx.x == 0) npair_s = 0;
+        if (threadIdx.x < kLikMaxWb) cun_s[threadIdx.x] = 0ull;
+        __syncthreads();
+        // pairs whose channels changed since the window was last computed
+        // (every pair without a dirty map); the others keep LIK and maxima
+        for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
+            const int b = pr / U, t = pr - b * U;
+            if (dirty == nullptr) {
+                plist[atomicAdd(&npair_s, 1u)] = (uint32_t)pr;
+            } else {
+                const uint64_t m = ts.cmask[t];
+                if ((m & dirty[W[w0 + b]]) != 0ull) {
+                    plist[atomicAdd(&npair_s, 1u)] = (uint32_t)pr;
+                    atomicOr(&cun_s[b], (unsigned long long)m);
+                }
+            }
         }
         __syncthreads();
-        for (int it = threadIdx.x; it < nb * per_win; it += blockDim.x) {
-            const int b = it / per_win;
-            const int rem = it - b * per_win;
-            const int t = rem / ngr, g = rem - t * ngr;
+        const int npair = (int)npair_s;
+        if (npair == 0) continue;  // uniform over the block
+        if (threadIdx.x == 0) atomicAdd(cnt_pairs, (uint32_t)npair);
+        // channels to stage per window: those of its computed templates
+        if ((int)threadIdx.x < nb) {
+            const int b = threadIdx.x;
+            int n = 0;
+            if (dirty == nullptr) {
+                for (int c = 0; c < C; c++) clist[b * C + n++] = c;
+            } else {
+                const uint64_t m = cun_s[b];
+                for (int c = 0; c < C; c++)
+                    if ((m >> c) & 1ull) clist[b * C + n++] = c;
+            }
+            ncl_s[b] = n;
+        }
+        __syncthreads();
+        // V is sample-major ([N][C]): read along the staged channels, store
+        // channel rows; eight loads in flight per thread
+        for (int b = 0; b < nb; b++) {
+            const int ncl = ncl_s[b];
+            const int tot = ncl * sv;
+            const int64_t xb = (int64_t)W[w0 + b] * L;
+            for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * blockDim.x) {
+                float v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int e = e0 + q * blockDim.x;
+                    v[q] = 0.0f;
+                    if (e < tot) {
+                        const int j = e / ncl, ci = e - j * ncl;
+                        const int64_t x = xb + j;
+                        if (x < N) v[q] = V[(size_t)x * C + clist[b * C + ci]];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int e = e0 + q * blockDim.x;
+                    if (e < tot) {
+                        const int j = e / ncl, ci = e - j * ncl;
+                        sV[(b * C + clist[b * C + ci]) * sv + j] = v[q];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        for (int it = threadIdx.x; it < npair * ngr; it += blockDim.x) {
+            const int j = it / ngr, g = it - j * ngr;
+            const int pr = (int)plist[j];
+            const int b = pr / U, t = pr - b * U;
             const int p0 = g * kPos;
             const int64_t x0 = (int64_t)W[w0 + b] * L + p0;
             float acc[kPos];
@@ -79,13 +146,14 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
             pp[it] = mp;
         }
         __syncthreads();
-        // per (window, template): first position of the window maximum
-        for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
+        // per computed (window, template): first position of the window maximum
+        for (int j = threadIdx.x; j < npair; j += blockDim.x) {
+            const int pr = (int)plist[j];
             const int b = pr / U, t = pr - b * U;
             float mv = -INFINITY;
             uint32_t mp = 0xffffu;
             for (int g = 0; g < ngr; g++) {
-                const int it = b * per_win + t * ngr + g;
+                const int it = j * ngr + g;
                 if (pv[it] > mv) { mv = pv[it]; mp = pp[it]; }
             }
             const size_t o = (size_t)W[w0 + b] * U + t;
@@ -93,6 +161,30 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
             wmax_p[o] = (uint16_t)mp;
         }
     }
+}
+
+// Marks, for every spike subtracted this pass, the windows whose likelihood
+// positions read any of its samples ([p, p + L) meets [wL, wL + 2L - 1) for
+// w in [p/L - 2, p/L + 1], a superset) with the spike's channels.  Integer OR
+// is order independent.
+__global__ __launch_bounds__(kBlock) void k_mark_dirty(TemplateSet ts, PassState st, const uint32_t* __restrict__ sidx,
+                                                       const uint32_t* __restrict__ slab, uint32_t nsp)
+{
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nsp; k += gridDim.x * blockDim.x) {
+        const uint32_t u = slab[k];
+        if (u >= (uint32_t)ts.U) continue;
+        const int64_t w = (int64_t)sidx[k] / ts.L;
+        const uint64_t m = ts.cmask[u];
+        const int64_t lo = w >= 2 ? w - 2 : 0, hi = min(st.nwin - 1, w + 1);
+        for (int64_t v = lo; v <= hi; v++) atomicOr((unsigned long long*)&st.dirty[v], (unsigned long long)m);
+    }
+}
+
+// The listed windows' likelihoods are current: clear their dirty channels.
+__global__ __launch_bounds__(kBlock) void k_clear_dirty(PassState st, uint32_t nW)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nW; i += gridDim.x * blockDim.x)
+        st.dirty[st.W[i]] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -576,7 +668,7 @@ void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N,
 static int64_t likelihood_lds_per_window(const TemplateSet& ts, int sv)
 {
     const int ngr = (ts.L + kPos - 1) / kPos;
-    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 8;
+    return (int64_t)ts.C * sv * 4 + (int64_t)ts.U * ngr * 8 + (int64_t)ts.U * 4 + (int64_t)ts.C * 4;
 }
 
 void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
@@ -587,6 +679,7 @@ void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
     const int per_win = ts.U * ngr;
     int w = (2 * kBlock + per_win - 1) / per_win;
     if (w < 1) w = 1;
+    if (w > kLikMaxWb) w = kLikMaxWb;
     const int64_t lds_per = likelihood_lds_per_window(ts, *sv);
     while (w > 1 && w * lds_per > 48 * 1024) w--;
     *wb = w;
@@ -599,7 +692,7 @@ bool likelihood_fits(const TemplateSet& ts)
     return wb * likelihood_lds_per_window(ts, sv) <= 64 * 1024;
 }
 
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s)
 {
     if (nW == 0) return;
     int wb, sv;
@@ -611,13 +704,30 @@ void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, 
     const int64_t N = st.N;
     const uint32_t* W = st.W;
     if (ts.L == 48)
-        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
+                           dirty, st.counters + CNT_PAIRS);
     else if (ts.L == 64)
-        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
+                           dirty, st.counters + CNT_PAIRS);
     else if (ts.L == 32)
-        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
+                           dirty, st.counters + CNT_PAIRS);
     else
-        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p);
+        hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
+                           dirty, st.counters + CNT_PAIRS);
+}
+
+void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_t* sidx, const uint32_t* slab,
+                       uint32_t nsp, hipStream_t s)
+{
+    if (nsp == 0 || st.dirty == nullptr) return;
+    hipLaunchKernelGGL(k_mark_dirty, dim3(grid_for(nsp, kBlock, 1u << 16)), dim3(kBlock), 0, s, ts, st, sidx, slab, nsp);
+}
+
+void launch_clear_dirty(const PassState& st, uint32_t nW, hipStream_t s)
+{
+    if (nW == 0 || st.dirty == nullptr) return;
+    hipLaunchKernelGGL(k_clear_dirty, dim3(grid_for(nW, kBlock, 1u << 16)), dim3(kBlock), 0, s, st, nW);
 }
 
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s)

@@ -45,6 +45,7 @@ struct MfTables {
     const int* aoff;    // [ntiles] first k-block of the tile in frag
     const int* chb;     // [ntiles] first 8-channel block (the tile's blocks are contiguous)
     const float* T1;    // [U] sum of |T| over the template (rounded up)
+    const int* tileof;  // [U] tile holding each template
     int ntiles;
     float inv_tscale;   // 1 / template scale (a power of two)
     float kl, kb, ka;   // delta = kl * Q + kb * |bias| + ka * (T1 / vs + L * nch * max|V| / ts)
@@ -89,10 +90,12 @@ __device__ __forceinline__ void key_max(unsigned long long key, float& v, uint32
 template <int NP, int LT>
 __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_likmfma(TemplateSet ts, PassState st, MfTables mt,
                                                         const float* __restrict__ rowL1, const float* __restrict__ vsp,
-                                                        const uint32_t* __restrict__ mbits, uint32_t w0, uint32_t nb)
+                                                        const uint32_t* __restrict__ mbits,
+                                                        const uint64_t* __restrict__ dirty, uint32_t w0, uint32_t nb)
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint32_t nitem_s;
+    __shared__ int ntact_s;
     __shared__ int64_t wid_s[kMfWin];
     constexpr int KW = kMfWin * NP;  // accumulator tiles per wave
     const int C = ts.C, U = ts.U, L = ts.L, R = ts.R;
@@ -109,6 +112,8 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
     uint32_t* flag = reinterpret_cast<uint32_t*>(Qs + kMfWin * U * NP);     // [kMfWin][U] bit p: tile p rescored
     int* rch = reinterpret_cast<int*>(flag + kMfWin * U);                   // [R] channel of each compact row
     float* rl1 = reinterpret_cast<float*>(rch + R);                          // [R] its L1 norm
+    int* tlist = reinterpret_cast<int*>(rl1 + R);                            // [ntiles] tiles computed this group
+    int* tact = tlist + mt.ntiles;                                           // [ntiles] 1: computed this group
     const int lane = lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
     const int q4 = lane >> 4, col = lane & 15;
     const float vs = *vsp;
@@ -128,6 +133,35 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
         }
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) flag[e] = 0u;
         __syncthreads();
+        // tiles to compute: those whose 8-channel blocks meet a channel that
+        // changed in one of the group's windows since it was last computed (all
+        // tiles without a dirty map).  The other tiles' templates would come out
+        // identical: their LIK, flags and window maxima are left as they are.
+        if (threadIdx.x == 0) {
+            uint64_t gm = 0ull;
+            for (int k = 0; k < kMfWin; k++)
+                if (wid_s[k] >= 0) gm |= dirty ? dirty[wid_s[k]] : ~0ull;
+            int na = 0;
+            for (int tl = 0; tl < mt.ntiles; tl++) {
+                int on = 1;
+                if (dirty != nullptr) {  // C <= 64 here
+                    const int c0 = mt.chb[tl] * 8, c1 = min(C, c0 + mt.nblk[tl] * 8);
+                    const uint64_t tm = (c1 - c0 >= 64) ? ~0ull : (((1ull << (c1 - c0)) - 1ull) << c0);
+                    on = (tm & gm) != 0ull ? 1 : 0;
+                }
+                tact[tl] = on;
+                if (on) tlist[na++] = tl;
+            }
+            ntact_s = na;
+            // (window, template) pairs computed, for the statistics
+            int nwv_ok = 0, units_on = 0;
+            for (int k = 0; k < kMfWin; k++) nwv_ok += wid_s[k] >= 0 ? 1 : 0;
+            for (int i = 0; i < na; i++) units_on += min(16, U - 16 * tlist[i]);
+            if (na) atomicAdd(st.counters + CNT_PAIRS, (uint32_t)(units_on * nwv_ok));
+        }
+        __syncthreads();
+        const int ntact = ntact_s;
+        if (ntact == 0) continue;  // uniform over the block
         // stage: one thread per (window, sample): the sample's channels are
         // contiguous in V ([N][C]), 8 channels per 16-byte LDS store; 16
         // consecutive lanes hold one 16-sample block, so the per-channel block
@@ -177,6 +211,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
         // that 16-position tile p reads (blocks p .. p + (15 + L - 1) / 16)
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
             const int k = e / U, t = e - k * U;
+            if (!tact[mt.tileof[t]]) continue;
             const int r0 = ts.off[t], nr = ts.nch[t];
             float q[NP];
 #pragma unroll
@@ -202,7 +237,8 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
             for (int p = 0; p < NP; p++) Qs[e * NP + p] = q[p];
         }
         __syncthreads();
-        for (int tile = wv; tile < mt.ntiles; tile += nwv) {
+        for (int ti = wv; ti < ntact; ti += nwv) {
+            const int tile = tlist[ti];
             const int nkb = mt.nkb[tile], nbt = mt.nblk[tile];
             const half8* Af = mt.frag + (size_t)mt.aoff[tile] * 64 + lane;
             const _Float16* vbase = Vh + col * RS + mt.chb[tile] * 8;
@@ -357,7 +393,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
             const int k = e / U, t = e - k * U;
             const int64_t wd = wid_s[k];
-            if (wd < 0) continue;
+            if (wd < 0 || !tact[mt.tileof[t]]) continue;
             float mv = -INFINITY;
             uint32_t mp = 0xffffu;
             if (flag[e] != 0u && keys[e] != 0ull) key_max(keys[e], mv, mp);
@@ -524,12 +560,11 @@ size_t likmfma_lds(const TemplateSet& ts, int ntiles)
     const int NP = (ts.L + 15) / 16;
     const int NS = NP * 16 + ts.L - 1;
     const int NSP = (NS + 15) & ~15, NB = NSP / 16;
-    (void)ntiles;
     // the staged windows' region later holds the exact pass's item list and keys
     const size_t NG = (NP * 16 + kPos - 1) / kPos;
     const size_t region = std::max((size_t)kMfWin * NSP * RS * 2, (size_t)kMfWin * ts.U * (NG * 4 + 8));
     return region + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
-           (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8;
+           (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8 + (size_t)ntiles * 8;
 }
 
 void launch_absmax(const float* V, int64_t n, uint32_t* mbits, float* vs, hipStream_t s)
@@ -551,29 +586,31 @@ static void allow_lds_mf(K kernel, size_t bytes)
 
 template <int NP, int LT>
 static void launch_mf(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
-                      const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, size_t lds, dim3 g, hipStream_t s)
+                      const float* vs, const uint32_t* mbits, const uint64_t* dirty, uint32_t w0, uint32_t nb,
+                      size_t lds, dim3 g, hipStream_t s)
 {
     allow_lds_mf(k_likmfma<NP, LT>, lds);
-    hipLaunchKernelGGL((k_likmfma<NP, LT>), g, dim3(kMfThreads), lds, s, ts, st, mt, rowL1, vs, mbits, w0, nb);
+    hipLaunchKernelGGL((k_likmfma<NP, LT>), g, dim3(kMfThreads), lds, s, ts, st, mt, rowL1, vs, mbits, dirty, w0, nb);
 }
 
 // MFMA likelihood of the listed windows W[w0 .. w0+nb), with the exact FP32
 // chain for the 16-position tiles the bound flags.
 void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const float* rowL1,
-                    const float* vs, const uint32_t* mbits, uint32_t w0, uint32_t nb, hipStream_t s)
+                    const float* vs, const uint32_t* mbits, const uint64_t* dirty, uint32_t w0, uint32_t nb,
+                    hipStream_t s)
 {
     if (nb == 0) return;
     const size_t lds = likmfma_lds(ts, mt.ntiles);
     const uint32_t ngroups = (nb + kMfWin - 1) / kMfWin;
     const dim3 g(std::min<uint32_t>(ngroups, 4096u));
     const int NP = (ts.L + 15) / 16;
-    if (ts.L == 48) launch_mf<3, 48>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else if (ts.L == 64) launch_mf<4, 64>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else if (ts.L == 32) launch_mf<2, 32>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else if (NP == 4) launch_mf<4, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else if (NP == 3) launch_mf<3, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else if (NP == 2) launch_mf<2, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
-    else launch_mf<1, 0>(ts, st, mt, rowL1, vs, mbits, w0, nb, lds, g, s);
+    if (ts.L == 48) launch_mf<3, 48>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else if (ts.L == 64) launch_mf<4, 64>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else if (ts.L == 32) launch_mf<2, 32>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else if (NP == 4) launch_mf<4, 0>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else if (NP == 3) launch_mf<3, 0>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else if (NP == 2) launch_mf<2, 0>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
+    else launch_mf<1, 0>(ts, st, mt, rowL1, vs, mbits, dirty, w0, nb, lds, g, s);
 }
 
 size_t screen_mfma_lds(const TemplateSet& ts)

@@ -144,6 +144,21 @@ int fbp_refine_backward(const float *R, int64_t ldr, int64_t N, int32_t C, const
 
 const char *fbp_refine_last_error(void);
 
+/* Multi-channel clips of get_clips (SF/processing/clip_utils.py:501-632, run by
+ * full binary pursuit over every found spike and channel,
+ * SF/overlap/full_binary_pursuit.py:495-502):
+ *   clips[k][q*(w1-w0) + j] = V[chans[q]][event_indices[k] + w0 + j]
+ * with [w0, w1) the sample window of time_window_to_samples (w0 <= 0).  V and
+ * clips hold elements of elem_bytes (2, 4 or 8) bytes, moved bit for bit (any
+ * 16-, 32- or 64-bit sample type).  Device pointers, queued on `stream`, no
+ * allocation.  The caller passes the events get_clips keeps (clips inside
+ * [0, N)); samples outside read as 0.  Handle-free: the message of a failure
+ * is fbp_clips_last_error(). */
+int fbp_get_clips(const void *V, int32_t elem_bytes, int64_t ldv, int64_t N, int32_t C, const int32_t *chans,
+                  int32_t nchan, const int64_t *event_indices, int64_t n_events, int32_t w0, int32_t w1, void *clips,
+                  void *stream);
+const char *fbp_clips_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -159,6 +159,34 @@ int fbp_get_clips(const void *V, int32_t elem_bytes, int64_t ldv, int64_t N, int
                   void *stream);
 const char *fbp_clips_last_error(void);
 
+/*
+ * Inputs of the pursuit (device pointers, queued on `stream`, no allocation,
+ * fixed summation orders).  Handle-free: fbp_inputs_last_error() on failure.
+ */
+
+/* Noise covariance of clip residuals per channel, as
+ * get_channel_covariance_matrix (SF/overlap/full_binary_pursuit.py:863-923)
+ * computes it with np.cov(rowvar=False, ddof=0):
+ *   X_c = R[rows[i]][c*L .. c*L+L)   (i < n_rows; rows NULL: rows 0 .. n_rows-1)
+ *   cov[c][i][j] = sum (X_c - mean)_i (X_c - mean)_j / n_rows        (FP64, [C][L][L])
+ * R is [n_total][>= C*L] float32 with row stride ld; L <= 128. */
+int fbp_covariance_workspace_bytes(int64_t n_rows, int32_t C, int32_t L, int64_t *bytes);
+int fbp_channel_covariance(const float *R, int64_t ld, int64_t n_total, int32_t C, int32_t L, const int64_t *rows,
+                           int64_t n_rows, double *cov, void *work, int64_t work_bytes, void *stream);
+
+/* Robust template of one unit's clips [n][W] (row stride ld), as
+ * calculate_robust_template (SF/processing/clip_utils.py:202-247) with numpy's
+ * float32 semantics, per column: the median of the values within one MAD of
+ * the median (the median when none is).  Exact order statistics. */
+int fbp_robust_template(const float *clips, int64_t n, int64_t ld, int64_t W, float *out, void *stream);
+
+/* var[u] = sum_c t_uc^T cov[c] t_uc for templates T [U][C][L] and cov
+ * [C][L][L], FP64 (compute_separability_metrics,
+ * SF/overlap/neuron_separability.py:158-183). */
+int fbp_template_variances(const double *T, int32_t U, int32_t C, int32_t L, const double *cov, double *var,
+                           void *stream);
+const char *fbp_inputs_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,8 @@ EXPORTS = (
     "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
     "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
     "fbp_get_stats", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_backward",
-    "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error",
+    "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
+    "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
 )
 
 
@@ -101,8 +102,14 @@ def load():
     L.fbp_get_clips.argtypes = [vp, i32, i64, i64, i32, vp, i32, vp, i64, i32, i32, vp, vp]
     L.fbp_clips_last_error.argtypes = []
     L.fbp_clips_last_error.restype = ctypes.c_char_p
+    L.fbp_covariance_workspace_bytes.argtypes = [i64, i32, i32, ctypes.POINTER(i64)]
+    L.fbp_channel_covariance.argtypes = [vp, i64, i64, i32, i32, vp, i64, vp, vp, i64, vp]
+    L.fbp_robust_template.argtypes = [vp, i64, i64, i64, vp, vp]
+    L.fbp_template_variances.argtypes = [vp, i32, i32, i32, vp, vp, vp]
+    L.fbp_inputs_last_error.argtypes = []
+    L.fbp_inputs_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
-        if name not in ("fbp_last_error", "fbp_refine_last_error", "fbp_clips_last_error"):
+        if not name.endswith("last_error"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -112,6 +119,13 @@ def refine_check(rc):
     """Raise for a failed handle-free refinement call."""
     if rc != FBP_OK:
         msg = load().fbp_refine_last_error()
+        _raise(rc, msg.decode() if msg else f"error {rc}")
+
+
+def inputs_check(rc):
+    """Raise for a failed covariance / robust template / variance call."""
+    if rc != FBP_OK:
+        msg = load().fbp_inputs_last_error()
         _raise(rc, msg.decode() if msg else f"error {rc}")
 
 

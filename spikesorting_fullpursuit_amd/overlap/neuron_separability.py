@@ -2,10 +2,14 @@
 
 Restates the parts of SF/overlap/neuron_separability.py (SF =
 spikesorting_fullpursuit/) that the pursuit path reads or writes:
-  * thresholds from templates and per-channel noise covariance (:113-195),
+  * compute_separability_metrics (:113-195), with the quadratic forms on the
+    GPU (fbp_template_variances), and the same formula on the host
+    (template_thresholds) for the synthetic workloads,
   * set_bp_threshold (:319-331),
   * add_n_spikes, called at the end of binary_pursuit (:334-345).
 """
+import ctypes
+
 import numpy as np
 
 
@@ -53,3 +57,54 @@ def add_n_spikes(separability_metrics, neuron_labels):
         counts += np.bincount(valid, minlength=n_templates)[:n_templates]
     separability_metrics["n_spikes"] = counts
     return None
+
+
+def compute_separability_metrics(templates, channel_covariance_mats, sort_info, *, device=0):
+    """The separability_metrics dict of neuron_separability.py:113-195.
+
+    Keys and values as the reference: bp_n_samples_per_chan, templates (stacked),
+    template_SS, template_SS_by_chan (zeros, as the reference leaves it),
+    channel_covariance_mats, contamination, peak_channel, channel_p_noise,
+    neuron_variances (sum_c t_c^T Sigma_c t_c, FP64 on the GPU),
+    neuron_lower_thresholds (-SS/2 + sigma_bp_noise * sqrt(var)) and
+    neuron_lower_CI (SS/2 - sigma_bp_CI * sqrt(var)).
+    """
+    import torch
+    from scipy.stats import norm
+
+    from .. import _native
+
+    n_chans = sort_info["n_channels"]
+    L = sort_info["n_samples_per_chan"]
+    T = np.vstack(templates)
+    U = T.shape[0]
+    m = {"bp_n_samples_per_chan": L, "templates": T}
+    m["template_SS"] = np.sum(T ** 2, axis=1)
+    m["template_SS_by_chan"] = np.zeros((U, n_chans))
+    m["channel_covariance_mats"] = channel_covariance_mats
+    m["contamination"] = np.zeros(U)
+    m["peak_channel"] = np.zeros(U, dtype=np.int64)
+    m["channel_p_noise"] = np.zeros(n_chans)
+    for chan in range(n_chans):
+        m["channel_p_noise"][chan] = (sort_info["n_samples"] - sort_info["n_threshold_crossings"][chan]) / \
+            sort_info["n_samples"]
+    if not torch.cuda.is_available():
+        raise RuntimeError("No GPU available")
+    dev = torch.device("cuda", device)
+    Td = torch.from_numpy(np.ascontiguousarray(T, dtype=np.float64)).to(dev)
+    covd = torch.from_numpy(np.ascontiguousarray(np.stack([np.asarray(c, np.float64) for c in channel_covariance_mats]))
+                            ).to(dev)
+    vard = torch.empty(U, dtype=torch.float64, device=dev)
+    vp = ctypes.c_void_p
+    _native.inputs_check(_native.load().fbp_template_variances(
+        vp(Td.data_ptr()), U, n_chans, L, vp(covd.data_ptr()), vp(vard.data_ptr()),
+        vp(torch.cuda.current_stream(dev).cuda_stream)))
+    var = vard.cpu().numpy()
+    expectation = 0.5 * m["template_SS"]
+    sd = np.sqrt(var)
+    m["neuron_variances"] = var
+    m["neuron_lower_CI"] = expectation - sort_info["sigma_bp_CI"] * sd
+    m["neuron_lower_thresholds"] = -1 * expectation + sort_info["sigma_bp_noise"] * sd
+    m["peak_channel"] = np.argmax(np.abs(T), axis=1) // L
+    m["contamination"] = norm.sf(m["neuron_lower_thresholds"], -expectation, sd)
+    return m

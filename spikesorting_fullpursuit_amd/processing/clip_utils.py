@@ -129,3 +129,41 @@ def get_clips(probe_dict, voltage, neighbors, event_indices, clip_width_s, use_m
         del mm
         spike_clips = np.memmap(clip_fname, dtype=v_dtype, mode="r", shape=(len(ev), width))
     return spike_clips, valid_event_indices
+
+
+def calculate_robust_template(all_channel_clips, *, device=0):
+    """Robust template of one unit's clips [n][W] (clip_utils.py:202-247).
+
+    Per column: the median of the values within one median absolute deviation
+    (scaled by 1 / 0.6745) of the column median, or the median when none is;
+    with numpy's float32 semantics (the mean of the two middle values for an
+    even count, rounded to float32).  One clip (or a 1-D input) is returned
+    squeezed, as the reference does.  float32 clips, a host array or a device
+    tensor; returns a host float32 array.
+    """
+    import torch
+
+    if all_channel_clips.shape[0] == 1 or all_channel_clips.ndim == 1:
+        out = all_channel_clips.squeeze()
+        return out.cpu().numpy() if torch.is_tensor(out) else np.squeeze(out)
+    dev = torch.device("cuda", device) if isinstance(device, int) else torch.device(device)
+    if torch.is_tensor(all_channel_clips):
+        X = all_channel_clips if all_channel_clips.is_cuda else all_channel_clips.to(dev)
+        if X.dtype != torch.float32:
+            raise TypeError("the GPU robust template takes float32 clips")
+        dev = X.device
+    else:
+        Xh = np.asarray(all_channel_clips)
+        if Xh.dtype != np.float32:
+            raise TypeError("the GPU robust template takes float32 clips")
+        if not torch.cuda.is_available():
+            raise RuntimeError("No GPU available")
+        X = torch.from_numpy(np.ascontiguousarray(Xh)).to(dev)
+    if X.stride(1) != 1:
+        X = X.contiguous()
+    n, W = X.shape
+    out = torch.empty(W, dtype=torch.float32, device=dev)
+    _native.inputs_check(_native.load().fbp_robust_template(
+        ctypes.c_void_p(X.data_ptr()), n, X.stride(0), W, ctypes.c_void_p(out.data_ptr()),
+        ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)))
+    return out.cpu().numpy()

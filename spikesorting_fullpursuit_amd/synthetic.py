@@ -347,16 +347,23 @@ def generate_workload_on_device(w, device="cuda", pow2_fft=False):
     lab_t = torch.from_numpy(labels).to(device)
     amp_t = torch.from_numpy(amp.astype(np.float32)).to(device)
     nz = torch.from_numpy(np.any(T != 0, axis=2)).to(device)  # [U][C] channels each template touches
-    # every (spike, touched channel) row at once, in slices of spikes to bound the index tensors
+    # every (spike, touched channel) row at once, in slices of spikes to bound the
+    # index tensors; the deterministic accumulation makes the recording the same
+    # bits on every run although spikes overlap
     step = max(1, (1 << 24) // max(1, C * L))
-    for k0 in range(0, len(onsets), step):
-        lab_k = lab_t[k0:k0 + step]
-        ks, cs = nz[lab_k].nonzero(as_tuple=True)
-        if ks.numel() == 0:
-            continue
-        idx = ((cs * N + on_t[k0:k0 + step][ks])[:, None] + ar[None, :]).reshape(-1)
-        val = (amp_t[k0:k0 + step][ks][:, None] * Tf[lab_k[ks], cs]).reshape(-1)
-        flat.index_add_(0, idx, val)
+    was_det = torch.are_deterministic_algorithms_enabled()
+    torch.use_deterministic_algorithms(True)
+    try:
+        for k0 in range(0, len(onsets), step):
+            lab_k = lab_t[k0:k0 + step]
+            ks, cs = nz[lab_k].nonzero(as_tuple=True)
+            if ks.numel() == 0:
+                continue
+            idx = ((cs * N + on_t[k0:k0 + step][ks])[:, None] + ar[None, :]).reshape(-1)
+            val = (amp_t[k0:k0 + step][ks][:, None] * Tf[lab_k[ks], cs]).reshape(-1)
+            flat.index_put_((idx,), val, accumulate=True)
+    finally:
+        torch.use_deterministic_algorithms(was_det)
     if V.is_cuda:
         torch.cuda.synchronize()
     return package(w, V, T, onsets, labels)

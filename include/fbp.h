@@ -187,6 +187,32 @@ int fbp_template_variances(const double *T, int32_t U, int32_t C, int32_t L, con
                            void *stream);
 const char *fbp_inputs_last_error(void);
 
+/*
+ * Voltage preprocessing (device pointers, queued on `stream`, no allocation).
+ * Handle-free: fbp_preproc_last_error() on failure.  V is [C][N] float32 with
+ * row stride ld.
+ */
+int fbp_preproc_workspace_bytes(int64_t N, int32_t C, int64_t *bytes);
+/* medians[c] = numpy float32 nanmedian(|V[c]|) (SF/threshold/threshold.py:6-70):
+ * the middle value, or fl32(fl32(lo + hi) / 2) for an even count; NaN when a
+ * row is all NaN.  Exact order statistics. */
+int fbp_abs_medians(const float *V, int64_t ld, int64_t N, int32_t C, float *medians, void *work, int64_t work_bytes,
+                    void *stream);
+/* counts[c] = number of x with |V[c][x]| > thresholds[c] (compared in FP64;
+ * threshold.py:51-70). */
+int fbp_count_above(const float *V, int64_t ld, int64_t N, int32_t C, const double *thresholds, uint64_t *counts,
+                    void *stream);
+/* cov = np.cov(V, ddof=ddof) (rows are channels), FP64 [C][C], C <= 64
+ * (get_full_zca_matrix, SF/processing/zca.py:5-31). */
+int fbp_channel_cov_rows(const float *V, int64_t ld, int64_t N, int32_t C, int32_t ddof, double *cov, void *work,
+                         int64_t work_bytes, void *stream);
+/* out[c][x] = float32(sum_k Z[c][k] V[k][x]) with FP64 products and sums, k
+ * ascending: the caller's (zca_matrix @ voltage).astype(float32)
+ * (SF/parallel/spikesorting_parallel.py:295), C <= 64. */
+int fbp_zca_apply(const double *Z, const float *V, int64_t ldv, int64_t N, int32_t C, float *out, int64_t ldo,
+                  void *stream);
+const char *fbp_preproc_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -33,6 +33,8 @@ EXPORTS = (
     "fbp_get_stats", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_backward",
     "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
     "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
+    "fbp_preproc_workspace_bytes", "fbp_abs_medians", "fbp_count_above", "fbp_channel_cov_rows", "fbp_zca_apply",
+    "fbp_preproc_last_error",
 )
 
 
@@ -108,6 +110,13 @@ def load():
     L.fbp_template_variances.argtypes = [vp, i32, i32, i32, vp, vp, vp]
     L.fbp_inputs_last_error.argtypes = []
     L.fbp_inputs_last_error.restype = ctypes.c_char_p
+    L.fbp_preproc_workspace_bytes.argtypes = [i64, i32, ctypes.POINTER(i64)]
+    L.fbp_abs_medians.argtypes = [vp, i64, i64, i32, vp, vp, i64, vp]
+    L.fbp_count_above.argtypes = [vp, i64, i64, i32, vp, vp, vp]
+    L.fbp_channel_cov_rows.argtypes = [vp, i64, i64, i32, i32, vp, vp, i64, vp]
+    L.fbp_zca_apply.argtypes = [vp, vp, i64, i64, i32, vp, i64, vp]
+    L.fbp_preproc_last_error.argtypes = []
+    L.fbp_preproc_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if not name.endswith("last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -126,6 +135,13 @@ def inputs_check(rc):
     """Raise for a failed covariance / robust template / variance call."""
     if rc != FBP_OK:
         msg = load().fbp_inputs_last_error()
+        _raise(rc, msg.decode() if msg else f"error {rc}")
+
+
+def preproc_check(rc):
+    """Raise for a failed preprocessing call."""
+    if rc != FBP_OK:
+        msg = load().fbp_preproc_last_error()
         _raise(rc, msg.decode() if msg else f"error {rc}")
 
 

@@ -213,6 +213,19 @@ int fbp_zca_apply(const double *Z, const float *V, int64_t ldv, int64_t N, int32
                   void *stream);
 const char *fbp_preproc_last_error(void);
 
+/* Shift search of find_overlap_templates (SF/c_cython/sort_cython.pyx:333-462):
+ * for each (test, n1, n2) of triples [n][3] and every shift pair (s1, s2) in
+ * [-n_pre, n_post]^2, R = sum_e (test_e - (a_e + b_e))^2 with a, b the shifted
+ * templates of n1, n2 (T [U][C*L] float32), in numpy's FP64 operation order
+ * (pairwise np.sum), bit-identical.  best_value[k] is the first minimum in
+ * s1-major order and best_shift_pair[k] = (s1 + n_pre) * S + (s2 + n_pre),
+ * S = n_pre + n_post + 1.  C * L <= 16384.  Device pointers, `stream`;
+ * handle-free: fbp_qa_last_error() on failure. */
+int fbp_overlap_residuals(const float *T, int32_t U, int32_t C, int32_t L, int32_t n_pre, int32_t n_post,
+                          const int32_t *triples, int64_t n_triples, double *best_value, int32_t *best_shift_pair,
+                          void *stream);
+const char *fbp_qa_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

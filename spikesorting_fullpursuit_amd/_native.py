@@ -34,7 +34,7 @@ EXPORTS = (
     "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
     "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
     "fbp_preproc_workspace_bytes", "fbp_abs_medians", "fbp_count_above", "fbp_channel_cov_rows", "fbp_zca_apply",
-    "fbp_preproc_last_error",
+    "fbp_preproc_last_error", "fbp_overlap_residuals", "fbp_qa_last_error",
 )
 
 
@@ -117,6 +117,9 @@ def load():
     L.fbp_zca_apply.argtypes = [vp, vp, i64, i64, i32, vp, i64, vp]
     L.fbp_preproc_last_error.argtypes = []
     L.fbp_preproc_last_error.restype = ctypes.c_char_p
+    L.fbp_overlap_residuals.argtypes = [vp, i32, i32, i32, i32, i32, vp, i64, vp, vp, vp]
+    L.fbp_qa_last_error.argtypes = []
+    L.fbp_qa_last_error.restype = ctypes.c_char_p
     for name in EXPORTS:
         if not name.endswith("last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -142,6 +145,13 @@ def preproc_check(rc):
     """Raise for a failed preprocessing call."""
     if rc != FBP_OK:
         msg = load().fbp_preproc_last_error()
+        _raise(rc, msg.decode() if msg else f"error {rc}")
+
+
+def qa_check(rc):
+    """Raise for a failed template-QA call."""
+    if rc != FBP_OK:
+        msg = load().fbp_qa_last_error()
         _raise(rc, msg.decode() if msg else f"error {rc}")
 
 

@@ -108,3 +108,35 @@ def compute_separability_metrics(templates, channel_covariance_mats, sort_info, 
     m["peak_channel"] = np.argmax(np.abs(T), axis=1) // L
     m["contamination"] = norm.sf(m["neuron_lower_thresholds"], -expectation, sd)
     return m
+
+
+def check_template_pair(template_1, template_2, chan_covariance_mats, n_chans, template_samples_per_chan, *,
+                        device=0):
+    """Probability of confusing two aligned templates (neuron_separability.py:59-110):
+    norm.cdf(0, E_L(t1) - E_L(t2), sqrt(var)) with E_L(t1) = t1.t1 / 2,
+    E_L(t2) = t1.t2 - t2.t2 / 2 and var = sum_c d_c^T Sigma_c d_c for d = t1 - t2
+    (the quadratic form in FP64 on the GPU); 1.0 when var <= 0."""
+    import torch
+    from scipy.stats import norm
+
+    from .. import _native
+
+    t1 = np.asarray(template_1, dtype=np.float64)
+    t2 = np.asarray(template_2, dtype=np.float64)
+    E_L_t1 = 0.5 * np.dot(t1, t1)
+    E_L_t2 = np.dot(t1, t2) - 0.5 * np.dot(t2, t2)
+    if not torch.cuda.is_available():
+        raise RuntimeError("No GPU available")
+    dev = torch.device("cuda", device)
+    d = torch.from_numpy(np.ascontiguousarray(t1 - t2)).to(dev)
+    covd = torch.from_numpy(np.ascontiguousarray(np.stack([np.asarray(c, np.float64) for c in chan_covariance_mats]))
+                            ).to(dev)
+    var = torch.empty(1, dtype=torch.float64, device=dev)
+    vp = ctypes.c_void_p
+    _native.inputs_check(_native.load().fbp_template_variances(
+        vp(d.data_ptr()), 1, int(n_chans), int(template_samples_per_chan), vp(covd.data_ptr()), vp(var.data_ptr()),
+        vp(torch.cuda.current_stream(dev).cuda_stream)))
+    var_diff = float(var.cpu().numpy()[0])
+    if var_diff > 0:
+        return norm.cdf(0, E_L_t1 - E_L_t2, np.sqrt(var_diff))
+    return 1.0

@@ -727,7 +727,9 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     const int srows = RG / LP;
     int* stg_ch = reinterpret_cast<int*>(reg + RG);  // [RG / LP] channel of each staged row
     int* slot_of = stg_ch + RG / LP;      // [U] order of unit u among the staged units (-1: none)
-    uint8_t* fin = reinterpret_cast<uint8_t*>(slot_of + U);  // [S] b+f-n in [0, N-L)
+    int* slot_nr = slot_of + U;           // [U] non-zero channels of the unit in each staging slot
+    int* slot_off = slot_nr + U;          // [U] its first compact row
+    uint8_t* fin = reinterpret_cast<uint8_t*>(slot_off + U);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const int wbase = threadIdx.x & ~63;
@@ -1055,10 +1057,34 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             __syncthreads();
             const uint32_t nN = min(nsurv_s, capN), nP = min(nsurvp_s, capP);
             for (uint32_t j = threadIdx.x; j < nN; j += bd) eval_pl((int)surv_e[j], surv_b[j]);
-            for (uint32_t j = threadIdx.x; j < nP * (uint32_t)S; j += bd) {
-                int pi, f;
-                divmod_small((int)j, S, invS, pi, f);
-                eval_all((int)surv_e[capN + pi], surv_b[capN + pi], f);
+            // (pair, f) items, four per thread per round with their G loads issued together
+            // (the order of candidates and of the running maximum does not matter)
+            const uint32_t nPS = nP * (uint32_t)S;
+            for (uint32_t j0 = threadIdx.x; j0 < nPS; j0 += 4u * (uint32_t)bd) {
+                int uq[4], sq[4], fq[4];
+                float bq[4], gq[4];
+                bool okq[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t j = j0 + (uint32_t)q * (uint32_t)bd;
+                    okq[q] = j < nPS;
+                    int pi, f;
+                    divmod_small(okq[q] ? (int)j : 0, S, invS, pi, f);
+                    const int e = (int)surv_e[capN + pi];
+                    bq[q] = surv_b[capN + pi];
+                    divmod_small(e, S, invS, uq[q], sq[q]);
+                    fq[q] = f;
+                    gq[q] = okq[q] ? gterm(uq[q], sq[q], f) : 0.0f;
+                }
+                __builtin_amdgcn_sched_barrier(0);  // the four G loads are in flight together
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int f = fq[q], u = uq[q], s = sq[q];
+                    if (!okq[q] || !fin[f] || abs(f - s) > L) continue;
+                    const float eh = shared ? (A[f] + bq[q]) - gq[q] : A[f] + bq[q];
+                    if (shared) msh = fmaxf(msh, eh - D[u]);
+                    if (!(eh < thr_u(u))) push(eh, u, f, s);
+                }
             }
             for (uint32_t m = spill; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
@@ -1113,23 +1139,47 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             for (uint32_t i = threadIdx.x; i < nc; i += bd) {
                 const int u = (int)(cand_k[i] % (uint32_t)U);
                 if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
-                if (atomicCAS(&slot_of[u], -1, -2) == -1) slot_of[u] = (int)atomicAdd(&nrow_s, 1u);
+                if (atomicCAS(&slot_of[u], -1, -2) == -1) {
+                    const int o = (int)atomicAdd(&nrow_s, 1u);
+                    slot_nr[o] = ts.nch[u];
+                    slot_off[o] = ts.off[u];
+                    slot_of[u] = o;
+                }
             }
             __syncthreads();
             const int nun = (int)nrow_s;
             const int upr = max(1, srows / CM);
+            const int per = CM * LP;  // floats per staged unit
             for (int r0u = 0; r0u < nun; r0u += upr) {
                 if (r0u > 0) __syncthreads();  // the previous round's chains are done
-                for (int u = wv; u < U; u += nw) {
-                    const int o = slot_of[u];
-                    if (o < r0u || o >= r0u + upr) continue;
-                    const int rb = (o - r0u) * CM, r0 = ts.off[u], nr = ts.nch[u];
-                    for (int e = lane; e < nr * LP; e += 64) {
-                        const int k = e / LP, i = e - k * LP;
-                        stgP[(rb + k) * LP + i] =
-                            (i >= n && i < n + L) ? ts.rows[(size_t)(r0 + k) * ts.Lp + (i - n)] : 0.0f;
+                // the round's units staged by the whole block, eight loads in flight per thread
+                const int tot = min(upr, nun - r0u) * per;
+                for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * bd) {
+                    float v[8];
+                    int dst[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int e = e0 + q * bd;
+                        v[q] = 0.0f;
+                        dst[q] = -1;
+                        if (e < tot) {
+                            const int o = e / per, rem = e - o * per;
+                            const int k = rem / LP, i = rem - k * LP;
+                            if (k < slot_nr[r0u + o]) {
+                                dst[q] = e;
+                                if (i >= n && i < n + L)
+                                    v[q] = ts.rows[(size_t)(slot_off[r0u + o] + k) * ts.Lp + (i - n)];
+                            }
+                        }
                     }
-                    for (int k = lane; k < nr; k += 64) stg_ch[rb + k] = ts.chan[r0 + k];
+                    __builtin_amdgcn_sched_barrier(0);  // the eight loads are in flight together
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        if (dst[q] >= 0) stgP[dst[q]] = v[q];
+                }
+                for (int e = threadIdx.x; e < min(upr, nun - r0u) * CM; e += bd) {
+                    const int o = e / CM, k = e - o * CM;
+                    if (k < slot_nr[r0u + o]) stg_ch[e] = ts.chan[slot_off[r0u + o] + k];
                 }
                 __syncthreads();
                 for (uint32_t i = threadIdx.x; i < nc; i += bd) {
@@ -1209,7 +1259,7 @@ size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max
 {
     const int LP = ts.L + S - 1;
     const int rg = screen_select_region(ts, S, cmax, npl_max);
-    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 4 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
+    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 6 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
            2 * (size_t)S + 16;
 }
 

@@ -747,6 +747,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         const uint32_t lab = st.best_lab[id];
         const int64_t b = st.best_idx[id];
         const float bl = st.best_lik[id];
+        if (threadIdx.x == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
         __syncthreads();
         if (window_skipped(ts, st, id)) {
             if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
@@ -764,14 +765,14 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         }
         for (int i = threadIdx.x; i < LP; i += bd) zrow[i] = 0.0f;
         for (int k = threadIdx.x; k < nl; k += bd) cl[k] = ts.chan[ts.off[lab] + k];
-        if (threadIdx.x == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
-        __syncthreads();
+        // the staging above, the pair ids, the label's likelihood row and the
+        // per-pair loads below are independent: no barrier between them, so
+        // their global loads are in flight together
         for (int u = threadIdx.x; u < U; u += bd) {
             const int p = tb.pid[(size_t)lab * U + u];
             pidw[u] = p;
             if (p >= 0) { atomicMin(&pfirst_sh, p); atomicAdd(&npl_sh, 1); }
         }
-        __syncthreads();
         for (int f = threadIdx.x; f < S; f += bd) {
             const int64_t af = b + f - n;
             const bool in = af >= 0 && af < N - L;
@@ -785,7 +786,6 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         // are issued 8 pairs at a time from clamped (always valid) addresses.
         float bv[kSelR];
         uint32_t pum = 0, shm = 0;
-        __syncthreads();
         {
             int u = u_t, s = s_t, t0 = (int)threadIdx.x;
             asm volatile("" : "+v"(u), "+v"(s), "+v"(t0));  // recompute the pair walk, do not keep it live
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     const int uc = ok[j] ? u : 0, ec = ok[j] ? e : 0;
                     lv[j] = st.LIK[(size_t)uc * N + (ok[j] ? as : 0)];
                     bi[j] = ec;
-                    pw[j] = pidw[uc];
+                    pw[j] = tb.pid[(size_t)lab * U + uc];  // global: pidw may not be written yet
                     pair_step(u, s, du, ds, S);
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -817,6 +817,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 }
             }
         }
+        __syncthreads();  // pidw, plo / phi, pfirst / npl are complete
         // least G of every pair of the label over the blocks of f that hold
         // the f with LIK[l][b+f-n] > 0 (the whole row when they span > 3 blocks)
         {

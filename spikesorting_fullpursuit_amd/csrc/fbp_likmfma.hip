@@ -435,7 +435,8 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
     const int US = U * S;
     _Float16* Wh = reinterpret_cast<_Float16*>(smem);  // [kSwWin][LR][RS]
     _Float16* Wl = Wh + kSwWin * LR * RS;              // [kSwWin][LR][RS]
-    const int lane = lane_id(), wv = threadIdx.x >> 6, nwv = blockDim.x >> 6;
+    // the wave index is uniform: tile, nkb, nbt and the k-block skips stay scalar
+    const int lane = lane_id(), wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nwv = blockDim.x >> 6;
     const int q4 = lane >> 4, col = lane & 15;
     const uint32_t nF = st.counters[CNT_F];
     const uint32_t ngroups = (nbatch + kSwWin - 1) / kSwWin;
@@ -498,29 +499,29 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
                     acc[k][p] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
                     acs[k][p] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
                 }
-            half8 ahn = Ah[0], aln = Al[0];
-            for (int kb = 0; kb < nkb; kb++) {
-                const half8 ah = ahn, al = aln;
-                if (kb + 1 < nkb) {
-                    ahn = Ah[(size_t)(kb + 1) * 64];
-                    aln = Al[(size_t)(kb + 1) * 64];
-                }
-                const int r = 4 * kb + q4;
-                int i = r / nbt, bk = r - i * nbt;
+            // (i, bk) = (template sample, channel block) of the lane's k-group r = 4 kb + q4,
+            // advanced by 4 groups per k-block (nbt is uniform: one division per tile)
+            int i = q4 / nbt, bk = q4 - (q4 / nbt) * nbt;
+            auto kblock = [&](const half8 ah, const half8 al) __attribute__((always_inline)) {
                 const bool pad = i >= L;  // padding run: its A values are zero
-                if (pad) { i = 0; bk = 0; }
-                // template samples of the k-block (uniform): runs 4 kb .. 4 kb + 3
-                const int ilo = (4 * kb) / nbt, ihi = min(L - 1, (4 * kb + 3) / nbt);
+                // template samples of the k-block (uniform): lanes 0 (q4 = 0) and 48 (q4 = 3)
+                const int ilo = __builtin_amdgcn_readlane(i, 0);
+                const int ihi = min(L - 1, __builtin_amdgcn_readlane(i, 48));
+                // row offset of window sample j = s + i - n at the tile's first shift s = col;
+                // a padding run reads the zero row at every shift (offset far below zero)
+                const int jo = pad ? INT_MIN / 2 : (col + i - n) * RS;
+                const int ob = cb + (pad ? 0 : bk) * 8;
 #pragma unroll
                 for (int p = 0; p < NPS; p++) {
                     // window samples j = s + i - n over the tile's shifts s
                     const int jlo = 16 * p + ilo - n, jhi = 16 * p + 15 + ihi - n;
                     if (jhi < 0 || jlo >= L) continue;  // every row absent: all-zero B
-                    const int j = 16 * p + col + i - n;
-                    const int row = (j >= 0 && j < L && !pad) ? j : L;
+                    // 0 <= j < L  <=>  0 <= j * RS < L * RS: one unsigned compare
+                    const int jr = jo + 16 * p * RS;
+                    const int ro = (unsigned)jr < (unsigned)(L * RS) ? jr : L * RS;
 #pragma unroll
                     for (int k = 0; k < kSwWin; k++) {
-                        const int o = (k * LR + row) * RS + cb + bk * 8;
+                        const int o = ro + k * LR * RS + ob;
                         const half8 bh = *reinterpret_cast<const half8*>(Wh + o);
                         const half8 bl = *reinterpret_cast<const half8*>(Wl + o);
                         acc[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[k][p], 0, 0, 0);
@@ -528,6 +529,24 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
                         acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acs[k][p], 0, 0, 0);
                     }
                 }
+                bk += 4;
+                while (bk >= nbt) {
+                    bk -= nbt;
+                    ++i;
+                }
+            };
+            // A fragments double-buffered two k-blocks ahead; loads past the end repeat the
+            // last k-block (in bounds, unused)
+            const int last = max(nkb - 1, 0);
+            half8 a0h = Ah[0], a0l = Al[0];
+            half8 a1h = Ah[(size_t)min(1, last) * 64], a1l = Al[(size_t)min(1, last) * 64];
+            for (int kb = 0; kb < nkb; kb += 2) {
+                kblock(a0h, a0l);
+                a0h = Ah[(size_t)min(kb + 2, last) * 64];
+                a0l = Al[(size_t)min(kb + 2, last) * 64];
+                if (kb + 1 < nkb) kblock(a1h, a1l);
+                a1h = Ah[(size_t)min(kb + 3, last) * 64];
+                a1l = Al[(size_t)min(kb + 3, last) * 64];
             }
             // lane holds rows 4*q4 + i (units), column col (shift within the tile)
 #pragma unroll

@@ -421,7 +421,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
 constexpr int kSwWin = 2;  // flagged windows per block iteration
 
 template <int NPS>
-__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_screen_mfma(
+__global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(NPS <= 6 ? 3 : 2, NPS <= 6 ? 3 : 2))) void k_screen_mfma(
     TemplateSet ts, PassState st, MfTables mt, const half8* __restrict__ fragl, const float* __restrict__ Ntr, int S,
     int n, uint32_t f0, uint32_t nbatch, float* __restrict__ Bbuf)
 {

@@ -232,10 +232,11 @@ __device__ __forceinline__ float exact_item_rows(const float* vW, int Lrt, const
 
 // exact_item_rows over zero-padded rows (row length LP = L + 2n, P[i] = T[i - n]):
 // the label's term at sample j is TlP[row][2n - f + j], the unit's
-// Tu[row][2n - s + j], an absent channel reads the zero row.  Same values and
-// operation order as exact_item_rows, without per-sample range tests.
+// Tu[row][2n - s + j], an absent channel reads the zero row; window channel c
+// starts at vW[c * vs].  Same values and operation order as exact_item_rows,
+// without per-sample range tests.
 template <int LT>
-__device__ __forceinline__ float exact_item_padded(const float* vW, int Lrt, const float* TlP, const int* cl, int nl,
+__device__ __forceinline__ float exact_item_padded(const float* vW, int Lrt, int vs, const float* TlP, const int* cl, int nl,
                                                    const float* Tu, const int* cu, int nu, const float* zrow, int LP,
                                                    int f, int s, int n)
 {
@@ -252,7 +253,7 @@ __device__ __forceinline__ float exact_item_padded(const float* vW, int Lrt, con
         const float* tsr = (hs ? Tu + iu * LP : zrow) + os;
         il += hf;
         iu += hs;
-        const float* vr = vW + c * L;
+        const float* vr = vW + c * vs;
         float ss = 0.0f;
         for (int j0 = 0; j0 < L; j0 += 8) {
             float a[8], b[8], vv[8];
@@ -701,8 +702,11 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     const int US = U * S;
     const int64_t N = st.N;
     const int bd = (int)blockDim.x, nw = bd >> 6;
-    float* vW = sm;                       // [C][L]
-    float* vmax = vW + C * L;             // [C]
+    // window rows at an odd stride: lanes rescoring different units read
+    // different channels at one sample without sharing a bank
+    const int VS = L | 1;
+    float* vW = sm;                       // [C][VS]
+    float* vmax = vW + C * VS;            // [C]
     float* A = vmax + C;                  // [S]
     float* RAall = A + S;                 // [S]
     float* RApos = RAall + S;             // [S]
@@ -756,7 +760,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         const float* Bw = Bbuf + (size_t)wi * US;
         for (int e = threadIdx.x; e < C * L; e += bd) {
             const int j = e / C, c = e - j * C;
-            vW[c * L + j] = st.V[(size_t)(b + j) * C + c];
+            vW[c * VS + j] = st.V[(size_t)(b + j) * C + c];
         }
         const int nl = ts.nch[lab];
         for (int e = threadIdx.x; e < nl * LP; e += bd) {
@@ -840,7 +844,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         __syncthreads();
         for (int c = threadIdx.x; c < C; c += bd) {
             float m = 0.0f;
-            for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * L + j]));
+            for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * VS + j]));
             vmax[c] = m;
         }
         // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
@@ -1126,7 +1130,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
                     if (eh < theta) continue;
-                    const float v = recheck_item_value(ts, vW, L, 1, lab, (uint32_t)u, f, s, n);
+                    const float v = recheck_item_value(ts, vW, VS, 1, lab, (uint32_t)u, f, s, n);
                     argmax_merge(bv_best, bk, v, (uint32_t)u + (uint32_t)U * ((uint32_t)f + (uint32_t)S * (uint32_t)s));
                 }
             }
@@ -1191,7 +1195,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     const int o = slot_of[u];
                     if (o < r0u || o >= r0u + upr) continue;
                     const int rb = (o - r0u) * CM;
-                    const float v = exact_item_padded<LT>(vW, L, TlP, cl, nl, stgP + rb * LP, stg_ch + rb, ts.nch[u],
+                    const float v = exact_item_padded<LT>(vW, L, VS, TlP, cl, nl, stgP + rb * LP, stg_ch + rb, ts.nch[u],
                                                           zrow, LP, f, s, n);
                     argmax_merge(bv_best, bk, v, k);
                 }
@@ -1260,7 +1264,7 @@ size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max
 {
     const int LP = ts.L + S - 1;
     const int rg = screen_select_region(ts, S, cmax, npl_max);
-    return (size_t)(ts.C * ts.L + ts.C + 7 * S + 6 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
+    return (size_t)(ts.C * (ts.L | 1) + ts.C + 7 * S + 6 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
            2 * (size_t)S + 16;
 }
 

@@ -1061,36 +1061,45 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             sv = 0;
             __syncthreads();
             const uint32_t nN = min(nsurv_s, capN), nP = min(nsurvp_s, capP);
-            for (uint32_t j = threadIdx.x; j < nN; j += bd) eval_pl((int)surv_e[j], surv_b[j]);
-            // (pair, f) items, four per thread per round with their G loads issued together
-            // (the order of candidates and of the running maximum does not matter)
-            const uint32_t nPS = nP * (uint32_t)S;
-            for (uint32_t j0 = threadIdx.x; j0 < nPS; j0 += 4u * (uint32_t)bd) {
-                int uq[4], sq[4], fq[4];
-                float bq[4], gq[4];
-                bool okq[4];
+            // (pair, f) items of a survivor list, f = fo + t for t < wdt, spread over the
+            // block four per thread per round with their G loads issued together (the
+            // order of candidates and of the running maximum does not matter); an item
+            // counts when f is allowed (posl for the first list, fin for the second)
+            // and within L of s
+            auto run_items = [&](uint32_t base, uint32_t np, int wdt, int fo, bool plm) {
+                const uint32_t nI = np * (uint32_t)wdt;
+                const float invw = 1.0f / (float)wdt;
+                for (uint32_t j0 = threadIdx.x; j0 < nI; j0 += 4u * (uint32_t)bd) {
+                    int uq[4], sq[4], fq[4];
+                    float bq[4], gq[4];
+                    bool okq[4];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t j = j0 + (uint32_t)q * (uint32_t)bd;
-                    okq[q] = j < nPS;
-                    int pi, f;
-                    divmod_small(okq[q] ? (int)j : 0, S, invS, pi, f);
-                    const int e = (int)surv_e[capN + pi];
-                    bq[q] = surv_b[capN + pi];
-                    divmod_small(e, S, invS, uq[q], sq[q]);
-                    fq[q] = f;
-                    gq[q] = okq[q] ? gterm(uq[q], sq[q], f) : 0.0f;
-                }
-                __builtin_amdgcn_sched_barrier(0);  // the four G loads are in flight together
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t j = j0 + (uint32_t)q * (uint32_t)bd;
+                        okq[q] = j < nI;
+                        int pi, t;
+                        divmod_small(okq[q] ? (int)j : 0, wdt, invw, pi, t);
+                        const int e = (int)surv_e[base + pi];
+                        bq[q] = surv_b[base + pi];
+                        divmod_small(e, S, invS, uq[q], sq[q]);
+                        fq[q] = fo + t;
+                        gq[q] = okq[q] ? gterm(uq[q], sq[q], fq[q]) : 0.0f;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);  // the four G loads are in flight together
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int f = fq[q], u = uq[q], s = sq[q];
-                    if (!okq[q] || !fin[f] || abs(f - s) > L) continue;
-                    const float eh = shared ? (A[f] + bq[q]) - gq[q] : A[f] + bq[q];
-                    if (shared) msh = fmaxf(msh, eh - D[u]);
-                    if (!(eh < thr_u(u))) push(eh, u, f, s);
+                    for (int q = 0; q < 4; q++) {
+                        const int f = fq[q], u = uq[q], s = sq[q];
+                        if (!okq[q] || !(plm ? posl[f] : fin[f]) || abs(f - s) > L) continue;
+                        const float eh = shared ? (A[f] + bq[q]) - gq[q] : A[f] + bq[q];
+                        if (shared) msh = fmaxf(msh, eh - D[u]);
+                        if (!(eh < thr_u(u))) push(eh, u, f, s);
+                    }
                 }
-            }
+            };
+            // pairs with LIK[u][b+s-n] <= 0: f in [plo, phi] (none when phi < plo)
+            if (phi >= plo) run_items(0, nN, phi - plo + 1, plo, true);
+            // pairs with LIK[u][b+s-n] > 0: every f
+            run_items(capN, nP, S, 0, false);
             for (uint32_t m = spill; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
                 const int e = (int)threadIdx.x + bd * i;

@@ -511,9 +511,11 @@ __global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __
 
 // ---------------------------------------------------------------------------
 // Residual: compute_residual, binary_pursuit.cl:57-102, without atomics.
-// Every sample covered by several spikes of the pass is written once, by the
-// first covering spike in list order, which subtracts all covering templates
-// in list order.  Covering spikes lie within `reach` list entries.
+// A spike changes only the channels its template holds (a channel that is
+// zero throughout subtracts zeros).  Every (sample, channel) covered by
+// several spikes of the pass whose templates hold that channel is written
+// once, by the first of them in list order, which subtracts all covering
+// templates in list order.  Covering spikes lie within `reach` list entries.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __restrict__ V, int64_t N,
                                                      const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ slab,
@@ -522,18 +524,26 @@ __global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __re
     const int C = ts.C, L = ts.L;
     for (uint32_t k = blockIdx.x; k < nsp; k += gridDim.x) {
         const int64_t pk = sidx[k];
-        if (slab[k] >= (uint32_t)ts.U) continue;
+        const uint32_t lk = slab[k];
+        if (lk >= (uint32_t)ts.U) continue;
         const uint32_t jlo = (k > (uint32_t)reach) ? k - reach : 0u;
         const uint32_t jhi = min(nsp - 1, k + (uint32_t)reach);
-        // one thread per (sample, channel), channels fastest: V is sample-major
-        for (int e = threadIdx.x; e < L * C; e += blockDim.x) {
-            const int i = e / C, c = e - i * C;
+        const int nr = ts.nch[lk], r0 = ts.off[lk];
+        // one thread per (sample, channel of the template), channels fastest:
+        // V is sample-major
+        for (int e = threadIdx.x; e < L * nr; e += blockDim.x) {
+            const int i = e / nr, r = e - i * nr;
+            const int c = ts.chan[r0 + r];
             const int64_t x = pk + i;
             if (x >= N) continue;
             bool owner = true;
             for (uint32_t j = jlo; j < k; j++) {
                 const int64_t pj = sidx[j];
-                if (slab[j] < (uint32_t)ts.U && x >= pj && x < pj + L) { owner = false; break; }
+                const uint32_t lj = slab[j];
+                if (lj < (uint32_t)ts.U && x >= pj && x < pj + L && ts.nz[(size_t)lj * C + c]) {
+                    owner = false;
+                    break;
+                }
             }
             if (!owner) continue;
             float v = V[(size_t)x * C + c];

@@ -83,6 +83,27 @@ def test_correlated_pairs_match_oracle(handle, seed, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True))
 
 
+@pytest.mark.parametrize("seed", [2, 3])
+def test_overlapping_pairs_on_channel_subsets_match_oracle(handle, seed):
+    """Overlapping spikes of templates that hold different channel subsets: the
+    residual subtracts each template on its own channels only, and a sample
+    covered by two spikes on a shared channel is written once."""
+    U, C, L = 5, 6, 16
+    rng = np.random.default_rng(seed)
+    T = cases.dog_templates(U, C, L, seed)
+    for u in range(U):
+        keep = np.zeros(C, bool)
+        keep[rng.choice(C, int(rng.integers(2, 5)), replace=False)] = True
+        T[u, ~keep] = 0.0
+    T = T.astype(np.float32)
+    V, truth = cases.planted_pairs(T, 2400, 300 + seed, min_d=2)
+    V = V + rng.normal(size=V.shape).astype(np.float32) * 0.2
+    bias, thr = cases.bias_thr(T)
+    ora = fbp_oracle.pursuit(V, T, bias, thr, L - 1, return_residual=True)
+    for mode, mixed in (("split", False), ("valu", False), ("split", True)):
+        assert_same(gpu_run(handle, V, T, bias, thr, L - 1, literal=mode, mixed=mixed), ora)
+
+
 @pytest.mark.parametrize("recheck,literal", [(True, "literal"), (True, "split"), (True, "valu"), (False, "split")])
 def test_small_workload_3s_matches_oracle(handle, recheck, literal):
     d = synthetic.generate("small", duration_s=3.0)

@@ -164,18 +164,21 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
 }
 
 // Marks, for every spike subtracted this pass, the windows whose likelihood
-// positions read any of its samples ([p, p + L) meets [wL, wL + 2L - 1) for
-// w in [p/L - 2, p/L + 1], a superset) with the spike's channels.  Integer OR
-// is order independent.
+// positions read any of its samples with the spike's channels: window w
+// computes positions [wL, wL + L), which read samples [wL, wL + 2L - 1), and
+// that range meets the spike's [p, p + L) exactly for
+// w in [ceil((p - 2L + 2) / L), floor((p + L - 1) / L)].  Integer OR is order
+// independent.
 __global__ __launch_bounds__(kBlock) void k_mark_dirty(TemplateSet ts, PassState st, const uint32_t* __restrict__ sidx,
                                                        const uint32_t* __restrict__ slab, uint32_t nsp)
 {
+    const int64_t L = ts.L;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nsp; k += gridDim.x * blockDim.x) {
         const uint32_t u = slab[k];
         if (u >= (uint32_t)ts.U) continue;
-        const int64_t w = (int64_t)sidx[k] / ts.L;
+        const int64_t p = sidx[k];
         const uint64_t m = ts.cmask[u];
-        const int64_t lo = w >= 2 ? w - 2 : 0, hi = min(st.nwin - 1, w + 1);
+        const int64_t lo = p + 1 >= L ? (p + 1 - L) / L : 0, hi = min(st.nwin - 1, (p + L - 1) / L);
         for (int64_t v = lo; v <= hi; v++) atomicOr((unsigned long long*)&st.dirty[v], (unsigned long long)m);
     }
 }

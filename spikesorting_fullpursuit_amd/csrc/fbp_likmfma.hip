@@ -519,14 +519,20 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(NPS 
                     // 0 <= j < L  <=>  0 <= j * RS < L * RS: one unsigned compare
                     const int jr = jo + 16 * p * RS;
                     const int ro = (unsigned)jr < (unsigned)(L * RS) ? jr : L * RS;
+                    // both windows' fragments in flight before the first products
+                    half8 bh[kSwWin], bl[kSwWin];
 #pragma unroll
                     for (int k = 0; k < kSwWin; k++) {
                         const int o = ro + k * LR * RS + ob;
-                        const half8 bh = *reinterpret_cast<const half8*>(Wh + o);
-                        const half8 bl = *reinterpret_cast<const half8*>(Wl + o);
-                        acc[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc[k][p], 0, 0, 0);
-                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acs[k][p], 0, 0, 0);
-                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acs[k][p], 0, 0, 0);
+                        bh[k] = *reinterpret_cast<const half8*>(Wh + o);
+                        bl[k] = *reinterpret_cast<const half8*>(Wl + o);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k = 0; k < kSwWin; k++) {
+                        acc[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[k], acc[k][p], 0, 0, 0);
+                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[k], acs[k][p], 0, 0, 0);
+                        acs[k][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[k], acs[k][p], 0, 0, 0);
                     }
                 }
                 bk += 4;

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
                                                         const uint64_t* __restrict__ dirty, uint32_t w0, uint32_t nb)
 {
     extern __shared__ __align__(16) char smem[];
-    __shared__ uint32_t nitem_s;
+    __shared__ uint32_t nit_s[kMfWin];
     __shared__ int ntact_s;
     __shared__ int64_t wid_s[kMfWin];
     constexpr int KW = kMfWin * NP;  // accumulator tiles per wave
@@ -106,7 +106,8 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
     const int64_t N = st.N;
     constexpr int NGX = (NP * 16 + kPos - 1) / kPos;
     _Float16* Vh = reinterpret_cast<_Float16*>(smem);                        // [kMfWin][NSP][RS]
-    const size_t region = max((size_t)kMfWin * NSP * RS * 2, (size_t)kMfWin * U * (NGX * 4 + 8));
+    const size_t region = max((size_t)kMfWin * NSP * RS * 2,
+                              (size_t)kMfWin * U * (NGX * 4 + 8) + (size_t)C * (2 * L + kPos + 1) * 4);
     float* bmx = reinterpret_cast<float*>(smem + region);                   // [kMfWin][NB][Cp] block max |v|
     float* Qs = bmx + kMfWin * NB * Cp;                                      // [kMfWin][U][NP] sum rowL1 * vmax
     uint32_t* flag = reinterpret_cast<uint32_t*>(Qs + kMfWin * U * NP);     // [kMfWin][U] bit p: tile p rescored
@@ -335,15 +336,18 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
         __syncthreads();
         // exact FP32 pass over the flagged tiles of this group's windows, with
         // the operation order of k_likelihood (bias, the template's rows in
-        // channel order, taps ascending, 8 positions per item); the voltage is
-        // read from V, which the staging above just brought into the caches.
+        // channel order, taps ascending, 8 positions per item), one window at a
+        // time over its FP32 voltage rows staged in LDS as k_likelihood stages
+        // them (the fp16 windows are dead by now).
         // Window maxima of flagged pairs cover their flagged groups; the other
         // pairs' maxima are negative, which the window maximum treats as none.
         constexpr int NG = (NP * 16 + kPos - 1) / kPos;  // 8-position groups per window (upper bound)
         const int ngr = (L + kPos - 1) / kPos;
-        uint32_t* ilist = reinterpret_cast<uint32_t*>(Vh);  // [kMfWin * U * NG] items (the fp16 windows are dead)
+        const int sv = 2 * L + kPos + 1;
+        uint32_t* ilist = reinterpret_cast<uint32_t*>(Vh);  // [kMfWin][U * NG] items (template, group) per window
         unsigned long long* keys = reinterpret_cast<unsigned long long*>(ilist + kMfWin * U * NG);  // [kMfWin * U]
-        if (threadIdx.x == 0) nitem_s = 0;
+        float* sVf = reinterpret_cast<float*>(keys + kMfWin * U);  // [C][sv] one window's rows
+        if (threadIdx.x < kMfWin) nit_s[threadIdx.x] = 0u;
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) keys[e] = 0ull;
         __syncthreads();
         for (int e = threadIdx.x; e < kMfWin * U * ngr; e += blockDim.x) {
@@ -351,44 +355,56 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
             const int rem = e - k * U * ngr;
             const int t = rem / ngr, g = rem - t * ngr;
             if (wid_s[k] < 0) continue;
-            if ((flag[k * U + t] >> ((g * kPos) / 16)) & 1u) ilist[atomicAdd(&nitem_s, 1u)] = (uint32_t)e;
+            if ((flag[k * U + t] >> ((g * kPos) / 16)) & 1u)
+                ilist[k * U * NG + atomicAdd(&nit_s[k], 1u)] = (uint32_t)rem;
         }
         __syncthreads();
-        const int nitems = (int)nitem_s;
-        for (int ii = threadIdx.x; ii < nitems; ii += blockDim.x) {
-            const int e = (int)ilist[ii];
-            const int k = e / (U * ngr);
-            const int rem = e - k * U * ngr;
-            const int t = rem / ngr, g = rem - t * ngr;
-            const int p0 = g * kPos;
-            const int64_t x0 = wid_s[k] * L + p0;
-            float acc[kPos];
-            const float b0 = ts.bias[t];
-#pragma unroll
-            for (int q = 0; q < kPos; q++) acc[q] = b0;
-            const int r0 = ts.off[t], nr = ts.nch[t];
-            const int64_t lim = N - x0;
-            for (int r = 0; r < nr; r++) {
-                const float* trow = ts.rows + (size_t)(r0 + r) * ts.Lp;
-                const float* vcol = st.V + (size_t)x0 * C + rch[r0 + r];
-                if (LT > 0) lik_row_strided<(LT > 0 ? LT : 4)>(trow, vcol, C, lim, acc);
-                else lik_row_generic_strided(trow, vcol, C, L, lim, acc);
+        uint32_t nall = 0;
+#pragma unroll 1
+        for (int k = 0; k < kMfWin; k++) {
+            const int nit = (int)nit_s[k];
+            const int64_t wd = wid_s[k];
+            nall += (uint32_t)nit;
+            if (wd < 0 || nit == 0) continue;  // uniform over the block
+            for (int e = threadIdx.x; e < C * sv; e += blockDim.x) {
+                const int j = e / C, c = e - j * C;
+                const int64_t x = wd * L + j;
+                sVf[c * sv + j] = x < N ? st.V[(size_t)x * C + c] : 0.0f;
             }
-            float* out = st.LIK + (size_t)t * N;
-            float mv = -INFINITY;
-            uint32_t mp = 0xffffu;
+            __syncthreads();
+            for (int ii = threadIdx.x; ii < nit; ii += blockDim.x) {
+                const int rem = (int)ilist[k * U * NG + ii];
+                const int t = rem / ngr, g = rem - t * ngr;
+                const int p0 = g * kPos;
+                const int64_t x0 = wd * L + p0;
+                float acc[kPos];
+                const float b0 = ts.bias[t];
 #pragma unroll
-            for (int q = 0; q < kPos; q++) {
-                const int64_t x = x0 + q;
-                if (p0 + q < L && x < N) {
-                    const float v = (x + L >= N) ? 0.0f : acc[q];
-                    out[x] = v;
-                    if (v > mv) { mv = v; mp = (uint32_t)(p0 + q); }
+                for (int q = 0; q < kPos; q++) acc[q] = b0;
+                const int r0 = ts.off[t], nr = ts.nch[t];
+                for (int r = 0; r < nr; r++) {
+                    const float* trow = ts.rows + (size_t)(r0 + r) * ts.Lp;
+                    const float* vrow = sVf + rch[r0 + r] * sv + p0;
+                    if (LT > 0) lik_row_fixed<(LT > 0 ? LT : 4)>(trow, vrow, acc);
+                    else lik_row_generic(trow, vrow, L, acc);
                 }
+                float* out = st.LIK + (size_t)t * N;
+                float mv = -INFINITY;
+                uint32_t mp = 0xffffu;
+#pragma unroll
+                for (int q = 0; q < kPos; q++) {
+                    const int64_t x = x0 + q;
+                    if (p0 + q < L && x < N) {
+                        const float v = (x + L >= N) ? 0.0f : acc[q];
+                        out[x] = v;
+                        if (v > mv) { mv = v; mp = (uint32_t)(p0 + q); }
+                    }
+                }
+                if (mv != -INFINITY) atomicMax(&keys[k * U + t], max_key(mv, mp));
             }
-            if (mv != -INFINITY) atomicMax(&keys[k * U + t], max_key(mv, mp));
+            __syncthreads();  // the rows are staged again for the next window
         }
-        if (threadIdx.x == 0 && nitems) atomicAdd(st.counters + CNT_LIKX, (uint32_t)nitems * kPos);
+        if (threadIdx.x == 0 && nall) atomicAdd(st.counters + CNT_LIKX, nall * kPos);
         __syncthreads();
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) {
             const int k = e / U, t = e - k * U;
@@ -585,9 +601,11 @@ size_t likmfma_lds(const TemplateSet& ts, int ntiles)
     const int NP = (ts.L + 15) / 16;
     const int NS = NP * 16 + ts.L - 1;
     const int NSP = (NS + 15) & ~15, NB = NSP / 16;
-    // the staged windows' region later holds the exact pass's item list and keys
+    // the staged windows' region later holds the exact pass's item lists, keys
+    // and one window's FP32 voltage rows
     const size_t NG = (NP * 16 + kPos - 1) / kPos;
-    const size_t region = std::max((size_t)kMfWin * NSP * RS * 2, (size_t)kMfWin * ts.U * (NG * 4 + 8));
+    const size_t region = std::max((size_t)kMfWin * NSP * RS * 2,
+                                   (size_t)kMfWin * ts.U * (NG * 4 + 8) + (size_t)ts.C * (2 * ts.L + kPos + 1) * 4);
     return region + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
            (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8 + (size_t)ntiles * 8;
 }

@@ -174,17 +174,21 @@ def main():
         "bytes_per_launch": round(sel_bytes_per_launch),
         "ms_by_class_mixed": {k: round(v, 3) for k, v in classes.items()},
         "ms_select_mixed": round(stm["ms_select"], 3),
-        "likelihood_mfma": {"kernel": "k_likmfma (fp16 MFMA + exact FP32 pass of the flagged tiles)",
+        "likelihood_mfma": {"kernel": "likelihood of the mixed run: k_likmfma (fp16 MFMA + exact FP32 pass of the "
+                                      "flagged tiles) on pass 1, k_likelihood on the changed pairs after",
                             "achieved_tflops_equiv": round(mf_tflops, 2), "peak": 2500.0, "ms": round(mf_ms, 3),
-                            "note": "algorithmic FP32 flops of the likelihood / kernel time"},
+                            "note": "algorithmic FP32 flops of every listed (window, template) pair / class time"},
         "likelihood_fp32": {"kernel": "k_likelihood (FP32 path)", "achieved_tflops": round(lik_done_tflops, 3),
                             "peak": FP32_PEAK_TFLOPS, "frac": round(lik_done_tflops / FP32_PEAK_TFLOPS, 4),
+                            "peak_unpacked": FP32_PEAK_TFLOPS / 2,
+                            "frac_unpacked": round(lik_done_tflops / (FP32_PEAK_TFLOPS / 2), 4),
                             "avg_launch_ms": round(lik_avg_ms, 4),
                             "pairs_computed": st["likelihood_pairs_computed"],
                             "listed_tflops_equiv": round(lik_tflops, 3),
                             "note": "achieved: recomputed (window, template) pairs x 2 L^2 x mean non-zero channels "
                                     "/ kernel time; listed_tflops_equiv counts every template of every listed "
-                                    "window, as if none were skipped"},
+                                    "window, as if none were skipped; the build issues no packed FP32 "
+                                    "instructions, so one FMA per lane per cycle (peak_unpacked) bounds it"},
         "passes": stm["passes"],
     }
 

@@ -149,6 +149,23 @@ __device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, co
     for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
     lik_row_core<LT>(trow, r, acc);
 }
+// lik_row_fixed over a 16-byte aligned row with at least (LT + kPos + 2) & ~3
+// readable floats: the row is read four floats at a time; same chain.
+template <int LT>
+__device__ __forceinline__ void lik_row_fixed_a4(const float* __restrict__ trow, const float* vrow, float* acc)
+{
+    constexpr int NR = (LT + kPos - 1 + 3) & ~3;
+    float r[NR];
+#pragma unroll
+    for (int j = 0; j < NR / 4; j++) {
+        const float4 q = reinterpret_cast<const float4*>(vrow)[j];
+        r[4 * j] = q.x;
+        r[4 * j + 1] = q.y;
+        r[4 * j + 2] = q.z;
+        r[4 * j + 3] = q.w;
+    }
+    lik_row_core<LT>(trow, r, acc);
+}
 
 // Same, reading only vrow[0 .. lim) (zeros past it).
 template <int LT>

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     // [wb][C][sv] voltage rows, then per-item maxima [wb][U][ngr] (value, position),
     // then the block's (window, template) pairs to compute [wb][U], then the
     // channels each window stages [wb][C]
-    extern __shared__ float sV[];
+    extern __shared__ __align__(16) float sV[];
     __shared__ uint32_t npair_s;
     __shared__ unsigned long long cun_s[kLikMaxWb];
     __shared__ int ncl_s[kLikMaxWb];
@@ -126,8 +126,8 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
             for (int k = 0; k < nr; k++) {
                 const int c = ts.chan[r0 + k];
                 const float* trow = ts.rows + (size_t)(r0 + k) * ts.Lp;
-                const float* vrow = sV + (b * C + c) * sv + p0;
-                if (LT > 0) lik_row_fixed<(LT > 0 ? LT : 4)>(trow, vrow, acc);
+                const float* vrow = sV + (b * C + c) * sv + p0;  // 16-byte aligned: sv, p0 multiples of 4
+                if (LT > 0) lik_row_fixed_a4<(LT > 0 ? LT : 4)>(trow, vrow, acc);
                 else lik_row_generic(trow, vrow, L, acc);
             }
             float* out = LIK + (size_t)t * N;
@@ -687,7 +687,7 @@ static int64_t likelihood_lds_per_window(const TemplateSet& ts, int sv)
 void likelihood_plan(const TemplateSet& ts, int* wb, int* sv)
 {
     const int L = ts.L;
-    *sv = 2 * L + kPos + 1;
+    *sv = (2 * L + kPos + 1 + 3) & ~3;  // a multiple of 4: rows are read as float4
     const int ngr = (L + kPos - 1) / kPos;
     const int per_win = ts.U * ngr;
     int w = (2 * kBlock + per_win - 1) / per_win;

@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
     constexpr int NGX = (NP * 16 + kPos - 1) / kPos;
     _Float16* Vh = reinterpret_cast<_Float16*>(smem);                        // [kMfWin][NSP][RS]
     const size_t region = max((size_t)kMfWin * NSP * RS * 2,
-                              (size_t)kMfWin * U * (NGX * 4 + 8) + (size_t)C * (2 * L + kPos + 1) * 4);
+                              (size_t)kMfWin * U * (NGX * 4 + 8) + (size_t)C * ((2 * L + kPos + 4) & ~3) * 4);
     float* bmx = reinterpret_cast<float*>(smem + region);                   // [kMfWin][NB][Cp] block max |v|
     float* Qs = bmx + kMfWin * NB * Cp;                                      // [kMfWin][U][NP] sum rowL1 * vmax
     uint32_t* flag = reinterpret_cast<uint32_t*>(Qs + kMfWin * U * NP);     // [kMfWin][U] bit p: tile p rescored
@@ -343,10 +343,10 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
         // pairs' maxima are negative, which the window maximum treats as none.
         constexpr int NG = (NP * 16 + kPos - 1) / kPos;  // 8-position groups per window (upper bound)
         const int ngr = (L + kPos - 1) / kPos;
-        const int sv = 2 * L + kPos + 1;
+        const int sv = (2 * L + kPos + 4) & ~3;  // a multiple of 4: rows are read as float4
         uint32_t* ilist = reinterpret_cast<uint32_t*>(Vh);  // [kMfWin][U * NG] items (template, group) per window
         unsigned long long* keys = reinterpret_cast<unsigned long long*>(ilist + kMfWin * U * NG);  // [kMfWin * U]
-        float* sVf = reinterpret_cast<float*>(keys + kMfWin * U);  // [C][sv] one window's rows
+        float* sVf = reinterpret_cast<float*>(keys + kMfWin * U);  // [C][sv] one window's rows, 16-byte aligned
         if (threadIdx.x < kMfWin) nit_s[threadIdx.x] = 0u;
         for (int e = threadIdx.x; e < kMfWin * U; e += blockDim.x) keys[e] = 0ull;
         __syncthreads();
@@ -385,7 +385,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
                 for (int r = 0; r < nr; r++) {
                     const float* trow = ts.rows + (size_t)(r0 + r) * ts.Lp;
                     const float* vrow = sVf + rch[r0 + r] * sv + p0;
-                    if (LT > 0) lik_row_fixed<(LT > 0 ? LT : 4)>(trow, vrow, acc);
+                    if (LT > 0) lik_row_fixed_a4<(LT > 0 ? LT : 4)>(trow, vrow, acc);
                     else lik_row_generic(trow, vrow, L, acc);
                 }
                 float* out = st.LIK + (size_t)t * N;
@@ -605,7 +605,7 @@ size_t likmfma_lds(const TemplateSet& ts, int ntiles)
     // and one window's FP32 voltage rows
     const size_t NG = (NP * 16 + kPos - 1) / kPos;
     const size_t region = std::max((size_t)kMfWin * NSP * RS * 2,
-                                   (size_t)kMfWin * ts.U * (NG * 4 + 8) + (size_t)ts.C * (2 * ts.L + kPos + 1) * 4);
+                                   (size_t)kMfWin * ts.U * (NG * 4 + 8) + (size_t)ts.C * ((2 * ts.L + kPos + 4) & ~3) * 4);
     return region + (size_t)kMfWin * Cp * NB * 4 + (size_t)kMfWin * ts.U * NP * 4 +
            (size_t)kMfWin * ts.U * 4 + (size_t)ts.R * 8 + (size_t)ntiles * 8;
 }

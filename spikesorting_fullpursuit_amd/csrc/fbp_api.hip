@@ -296,6 +296,7 @@ static ScreenTables tables_view(const fbp_handle* h)
     tb.SS = h->SS.as<float>();
     tb.n = (int)h->tab_n;
     tb.S = (int)(2 * h->tab_n + 1);
+    tb.sg = (tb.S + 3) & ~3;
     tb.cmax = std::max(h->Cmax, 1);
     // delta = kappa * (Q_l + Q_u + 2 (SS_l + SS_u)) bounds |E-hat - E_fp32|:
     // both are sums of at most K rounded terms (gamma_K each), doubled for safety.
@@ -503,7 +504,7 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     HIP_TRY(h->pid.ensure(pid.size() * 4), "alloc pid");
     HIP_TRY(h->pair_l.ensure(pl.size() * 4), "alloc pairs");
     HIP_TRY(h->pair_u.ensure(pu.size() * 4), "alloc pairs");
-    HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * S * 4), "alloc gram table");
+    HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * ((S + 3) & ~3) * 4), "alloc gram table");
     HIP_TRY(h->minG.ensure((size_t)std::max<int64_t>(np, 1) * S * 4), "alloc gram minima");
     HIP_TRY(h->minGb.ensure((size_t)std::max<int64_t>(np, 1) * S * ((S + kGB - 1) / kGB) * 4), "alloc gram minima");
     HIP_TRY(h->Ntr.ensure((size_t)U * S * 4), "alloc energies");

@@ -62,7 +62,7 @@ struct ScreenTables {
     const int* pid;       // [U*U] pair index of (l, u) sharing a channel, or -1
     const int* pair_l;    // [npairs]
     const int* pair_u;    // [npairs]
-    float* G;             // [npairs][S][S]  G[p][s][f]: window-truncated cross product
+    float* G;             // [npairs][S][sg] G[p][s][f]: window-truncated cross product (f >= S: 0)
     float* minG;          // [npairs][S]     min over f
     float* minGb;         // [npairs][S][nbg] min over each block of kGB consecutive f
     int nbg;              // blocks of f: ceil(S / kGB)
@@ -72,6 +72,7 @@ struct ScreenTables {
     const float* rowL1;   // [R] L1 norm of each compact template row (rounded up)
     const float* SS;      // [U] sum of squares of each template (rounded up)
     int S;
+    int sg;               // floats per row of G: S rounded up to a multiple of 4 (rows read as float4)
     int n;
     int cmax;             // most non-zero channels of any template
     // delta(l, u) = P_l + P_u bounds |E-hat - E_fp32| of a pair, with

@@ -31,12 +31,16 @@ __device__ __forceinline__ float shifted(const TemplateSet& ts, int u, int c, in
 
 __global__ __launch_bounds__(kBlock) void k_tab_gram(TemplateSet ts, ScreenTables tb, int64_t npairs)
 {
-    const int S = tb.S, n = tb.n, L = ts.L, C = ts.C;
-    const int64_t total = npairs * S * S;
+    const int S = tb.S, SG = tb.sg, n = tb.n, L = ts.L, C = ts.C;
+    const int64_t total = npairs * S * SG;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = e / ((int64_t)S * S);
-        const int rem = (int)(e - p * S * S);
-        const int s = rem / S, f = rem - s * S;
+        const int64_t p = e / ((int64_t)S * SG);
+        const int rem = (int)(e - p * S * SG);
+        const int s = rem / SG, f = rem - s * SG;
+        if (f >= S) {  // row padding
+            tb.G[e] = 0.0f;
+            continue;
+        }
         const int l = tb.pair_l[p], u = tb.pair_u[p];
         const int fsh = f - n, ssh = s - n;
         const int jlo = max(0, max(fsh, ssh)), jhi = min(L, min(L + fsh, L + ssh));
@@ -56,7 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_tab_min_ntr(TemplateSet ts, ScreenTa
     const int64_t total = total_min + (int64_t)ts.U * S;
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         if (e < total_min) {
-            const float* row = tb.G + e * S;
+            const float* row = tb.G + e * tb.sg;
             float m = INFINITY;
             for (int g = 0; g < tb.nbg; g++) {
                 float mb = INFINITY;
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const float Du = D[u];
                 const float tp = fmaxf(Mns - Du, bl - Du);
                 const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
-                const float* Gr = tb.G + ((size_t)pidl[u] * S + s) * S;
+                const float* Gr = tb.G + ((size_t)pidl[u] * S + s) * tb.sg;
                 for (int f = lane; f < S; f += 64) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = (A[f] + Bu) - Gr[f];
@@ -602,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
                 const int p = pidl[u];
                 const bool pu = (posu[e >> 5] >> (e & 31)) & 1u;
                 const float theta = fmaxf(Mp - D[u], bl - D[u]);
-                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * tb.sg : nullptr;
                 for (int f = 0; f < S; f++) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;
@@ -649,6 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 // surviving pairs one pair at a time, so no phase needs a block-wide list.
 // ---------------------------------------------------------------------------
 constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
+constexpr int kQR = 2;              // (pair, f-quad) items per thread per round of G loads
 static_assert(kSelR <= 32, "pair masks are 32-bit");
 constexpr int kSelMaxBlock = 1024;  // threads per block at most (U * S <= kSelR * kSelMaxBlock)
 constexpr size_t kSelMaxLds = 96 * 1024;  // dynamic LDS of the register-resident selection at most
@@ -985,7 +990,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             const float Mref = shared ? Mns : fmaxf(Mns, Mp_sh);
             auto thr_u = [&](int u) { return fmaxf(Mref - D[u], bl - D[u]); };
             auto gterm = [&](int u, int s, int f) -> float {
-                return shared ? tb.G[((size_t)pidw[u] * S + s) * S + f] : 0.0f;
+                return shared ? tb.G[((size_t)pidw[u] * S + s) * tb.sg + f] : 0.0f;
             };
             auto eval_pl = [&](int e, float Bu) {
                 int u, s;
@@ -1061,45 +1066,59 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             sv = 0;
             __syncthreads();
             const uint32_t nN = min(nsurv_s, capN), nP = min(nsurvp_s, capP);
-            // (pair, f) items of a survivor list, f = fo + t for t < wdt, spread over the
-            // block four per thread per round with their G loads issued together (the
-            // order of candidates and of the running maximum does not matter); an item
-            // counts when f is allowed (posl for the first list, fin for the second)
-            // and within L of s
-            auto run_items = [&](uint32_t base, uint32_t np, int wdt, int fo, bool plm) {
-                const uint32_t nI = np * (uint32_t)wdt;
-                const float invw = 1.0f / (float)wdt;
-                for (uint32_t j0 = threadIdx.x; j0 < nI; j0 += 4u * (uint32_t)bd) {
-                    int uq[4], sq[4], fq[4];
-                    float bq[4], gq[4];
-                    bool okq[4];
+            // (pair, f-quad) items of a survivor list: the f in [flo, fhi] taken four at a
+            // time from fo = flo rounded down to a multiple of 4, so that a quad's G terms
+            // are one aligned float4 of the pair's G row; spread over the block kQR
+            // quads per thread per round, with their loads issued together (the order of
+            // candidates and of the running maximum does not matter).  An f counts when
+            // it lies in [flo, fhi], is allowed (posl for the first list, fin for the
+            // second) and lies within L of s.
+            auto run_items = [&](uint32_t base, uint32_t np, int flo, int fhi, bool plm) {
+                const int fo = flo & ~3;
+                const int nq = (fhi - fo) / 4 + 1;
+                const uint32_t nI = np * (uint32_t)nq;
+                const float invq = 1.0f / (float)nq;
+                for (uint32_t j0 = threadIdx.x; j0 < nI; j0 += (uint32_t)kQR * (uint32_t)bd) {
+                    int uq[kQR], sq[kQR], fq[kQR];
+                    float bq[kQR];
+                    float4 gq[kQR];
+                    bool okq[kQR];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
+                    for (int q = 0; q < kQR; q++) {
                         const uint32_t j = j0 + (uint32_t)q * (uint32_t)bd;
                         okq[q] = j < nI;
                         int pi, t;
-                        divmod_small(okq[q] ? (int)j : 0, wdt, invw, pi, t);
+                        divmod_small(okq[q] ? (int)j : 0, nq, invq, pi, t);
                         const int e = (int)surv_e[base + pi];
                         bq[q] = surv_b[base + pi];
                         divmod_small(e, S, invS, uq[q], sq[q]);
-                        fq[q] = fo + t;
-                        gq[q] = okq[q] ? gterm(uq[q], sq[q], fq[q]) : 0.0f;
+                        fq[q] = fo + 4 * t;
+                        gq[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if (okq[q] && shared)
+                            gq[q] = *reinterpret_cast<const float4*>(
+                                tb.G + ((size_t)pidw[uq[q]] * S + sq[q]) * tb.sg + fq[q]);
                     }
-                    __builtin_amdgcn_sched_barrier(0);  // the four G loads are in flight together
+                    __builtin_amdgcn_sched_barrier(0);  // the quads' G loads are in flight together
 #pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        const int f = fq[q], u = uq[q], s = sq[q];
-                        if (!okq[q] || !(plm ? posl[f] : fin[f]) || abs(f - s) > L) continue;
-                        const float eh = shared ? (A[f] + bq[q]) - gq[q] : A[f] + bq[q];
-                        if (shared) msh = fmaxf(msh, eh - D[u]);
-                        if (!(eh < thr_u(u))) push(eh, u, f, s);
+                    for (int q = 0; q < kQR; q++) {
+                        if (!okq[q]) continue;
+                        const int u = uq[q], s = sq[q];
+                        const float gv[4] = {gq[q].x, gq[q].y, gq[q].z, gq[q].w};
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int f = fq[q] + r;
+                            if (f < flo || f > fhi || !(plm ? posl[f] : fin[f]) || abs(f - s) > L) continue;
+                            const float eh = shared ? (A[f] + bq[q]) - gv[r] : A[f] + bq[q];
+                            if (shared) msh = fmaxf(msh, eh - D[u]);
+                            if (!(eh < thr_u(u))) push(eh, u, f, s);
+                        }
                     }
                 }
             };
             // pairs with LIK[u][b+s-n] <= 0: f in [plo, phi] (none when phi < plo)
-            if (phi >= plo) run_items(0, nN, phi - plo + 1, plo, true);
+            if (phi >= plo) run_items(0, nN, plo, phi, true);
             // pairs with LIK[u][b+s-n] > 0: every f
-            run_items(capN, nP, S, 0, false);
+            run_items(capN, nP, 0, S - 1, false);
             for (uint32_t m = spill; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
                 const int e = (int)threadIdx.x + bd * i;
@@ -1134,7 +1153,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 const int p = pidw[u];
                 const bool pu = st.LIK[(size_t)u * N + as] > 0.0f;
                 const float theta = fmaxf(Mp - D[u], bl - D[u]);
-                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * S : nullptr;
+                const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * tb.sg : nullptr;
                 for (int f = 0; f < S; f++) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
                     const float eh = p >= 0 ? (A[f] + Bu) - Gr[f] : A[f] + Bu;

@@ -141,16 +141,8 @@ __device__ __forceinline__ void lik_row_core(const float* __restrict__ trow, con
     }
 }
 
-template <int LT>
-__device__ __forceinline__ void lik_row_fixed(const float* __restrict__ trow, const float* vrow, float* acc)
-{
-    float r[LT + kPos - 1];
-#pragma unroll
-    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
-    lik_row_core<LT>(trow, r, acc);
-}
-// lik_row_fixed over a 16-byte aligned row with at least (LT + kPos + 2) & ~3
-// readable floats: the row is read four floats at a time; same chain.
+// A 16-byte aligned voltage row with at least (LT + kPos + 2) & ~3 readable
+// floats, read four floats at a time into registers, then the chain above.
 template <int LT>
 __device__ __forceinline__ void lik_row_fixed_a4(const float* __restrict__ trow, const float* vrow, float* acc)
 {
@@ -167,45 +159,7 @@ __device__ __forceinline__ void lik_row_fixed_a4(const float* __restrict__ trow,
     lik_row_core<LT>(trow, r, acc);
 }
 
-// Same, reading only vrow[0 .. lim) (zeros past it).
-template <int LT>
-__device__ __forceinline__ void lik_row_fixed_bounded(const float* __restrict__ trow, const float* __restrict__ vrow,
-                                                      int64_t lim, float* acc)
-{
-    float r[LT + kPos - 1];
-#pragma unroll
-    for (int j = 0; j < LT + kPos - 1; j++) r[j] = (j < lim) ? vrow[j] : 0.0f;
-    lik_row_core<LT>(trow, r, acc);
-}
-
-// Same, the voltage row read from sample-major memory: sample j at vcol[j * stride],
-// zeros from sample lim on.
-template <int LT>
-__device__ __forceinline__ void lik_row_strided(const float* __restrict__ trow, const float* __restrict__ vcol,
-                                                int stride, int64_t lim, float* acc)
-{
-    float r[LT + kPos - 1];
-#pragma unroll
-    for (int j = 0; j < LT + kPos - 1; j++) r[j] = (j < lim) ? vcol[(size_t)j * stride] : 0.0f;
-    lik_row_core<LT>(trow, r, acc);
-}
-
-__device__ __forceinline__ void lik_row_generic_strided(const float* __restrict__ trow, const float* __restrict__ vcol,
-                                                        int stride, int L, int64_t lim, float* acc)
-{
-    float r[kPos];
-#pragma unroll
-    for (int q = 0; q < kPos; q++) r[q] = (q < lim) ? vcol[(size_t)q * stride] : 0.0f;
-    for (int i = 0; i < L; i++) {
-        const float tv = trow[i];
-#pragma unroll
-        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
-#pragma unroll
-        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
-        r[kPos - 1] = (i + kPos < lim) ? vcol[(size_t)(i + kPos) * stride] : 0.0f;
-    }
-}
-
+// Runtime L: the same chain with a sliding window of kPos registers.
 __device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, const float* vrow, int L, float* acc)
 {
     float r[kPos];
@@ -218,22 +172,6 @@ __device__ __forceinline__ void lik_row_generic(const float* __restrict__ trow, 
 #pragma unroll
         for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
         r[kPos - 1] = vrow[i + kPos];
-    }
-}
-
-__device__ __forceinline__ void lik_row_generic_bounded(const float* __restrict__ trow, const float* __restrict__ vrow,
-                                                        int L, int64_t lim, float* acc)
-{
-    float r[kPos];
-#pragma unroll
-    for (int q = 0; q < kPos; q++) r[q] = (q < lim) ? vrow[q] : 0.0f;
-    for (int i = 0; i < L; i++) {
-        const float tv = trow[i];
-#pragma unroll
-        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
-#pragma unroll
-        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
-        r[kPos - 1] = (i + kPos < lim) ? vrow[i + kPos] : 0.0f;
     }
 }
 

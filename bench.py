@@ -174,8 +174,9 @@ def main():
         "bytes_per_launch": round(sel_bytes_per_launch),
         "ms_by_class_mixed": {k: round(v, 3) for k, v in classes.items()},
         "ms_select_mixed": round(stm["ms_select"], 3),
-        "likelihood_mfma": {"kernel": "likelihood of the mixed run: k_likmfma (fp16 MFMA + exact FP32 pass of the "
-                                      "flagged tiles) on pass 1, k_likelihood on the changed pairs after",
+        "likelihood_mfma": {"kernel": "likelihood of the mixed run: pass 1 by k_likmfma (fp16 MFMA + exact FP32 pass "
+                                      "of the flagged tiles) where its tiles' channel padding pays, else by "
+                                      "k_likelihood; k_likelihood on the changed pairs after",
                             "achieved_tflops_equiv": round(mf_tflops, 2), "peak": 2500.0, "ms": round(mf_ms, 3),
                             "note": "algorithmic FP32 flops of every listed (window, template) pair / class time"},
         "likelihood_fp32": {"kernel": "k_likelihood (FP32 path)", "achieved_tflops": round(lik_done_tflops, 3),

@@ -41,6 +41,7 @@ extern "C" {
 #define FBP_FLAG_SCREEN_FP32 16u     /* overlap screen correlations on FP32 vector units instead of split-fp16 GEMMs */
 #define FBP_FLAG_PRECISION_MIXED 32u /* likelihood by fp16 GEMM + exact FP32 rescoring near and above zero (same spikes) */
 #define FBP_FLAG_SELECT_STREAMING 64u /* overlap screen selection without register-resident pair state (taken anyway when U*(2n+1) > 24576) */
+#define FBP_FLAG_LIKELIHOOD_MFMA 128u /* with FBP_FLAG_PRECISION_MIXED: the matrix-core first pass even where the tiles' channel padding predicts the FP32 kernel faster (same spikes either way) */
 
 typedef struct fbp_handle fbp_handle;
 

@@ -25,6 +25,7 @@ FLAG_TIMING = 8
 FLAG_SCREEN_FP32 = 16
 FLAG_PRECISION_MIXED = 32
 FLAG_SELECT_STREAMING = 64
+FLAG_LIKELIHOOD_MFMA = 128
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (

@@ -51,6 +51,7 @@ void launch_screen_select(const TemplateSet& ts, const PassState& st, const Scre
                           uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s);
 // MFMA likelihood tables (fbp_likmfma.hip).
 constexpr int kMfMaxBlk = 32;
+constexpr double kMfPadMax = 2.5;  // tile channel rows per template channel row up to which the matrix cores pay
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 struct MfTables {
     const half8* frag;
@@ -160,6 +161,7 @@ struct fbp_handle {
     float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
     float scr_k2 = 0.0f, scr_k3 = 0.0f, scr_k4 = 0.0f;  // screen bound terms of the split-fp16 B
     bool mf_ok = false;
+    bool mf_pays = false;  // the matrix-core first pass is expected to beat the FP32 kernel
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, Vtmp, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
@@ -443,6 +445,18 @@ static int build_mf_tables(fbp_handle* h)
         h->scr_k3 = (float)(2.0 * 2.0 * std::ldexp(1.0, -27) * (1.0 + std::ldexp(1.0, -10)));
         h->scr_k4 = (float)(2.0 * std::ldexp(1.0, -27) * (1.0 + std::ldexp(1.0, -10)) * (double)tmax * (double)L);
     }
+    // First-pass choice: the matrix-core kernel multiplies every tile over the union of
+    // its templates' 8-channel blocks, the FP32 kernel only over each template's own
+    // channels.  Measured per row, the matrix cores win while the tiles carry at most
+    // about 2.5 times the templates' own channel rows: dense templates (ratio 1.0) take
+    // 0.4x the FP32 time, 7-of-64-channel templates (3.5) 1.15x, 5 units on 4 channels
+    // (6.4) 3.4x.
+    {
+        double rows_mf = 0.0, rows_fp = 0.0;
+        for (int t = 0; t < ntiles; t++) rows_mf += 16.0 * 8.0 * nblk[t];
+        for (size_t i = 0; i < h->h_nz.size(); i++) rows_fp += h->h_nz[i] ? 1.0 : 0.0;
+        h->mf_pays = rows_mf <= kMfPadMax * std::max(rows_fp, 1.0);
+    }
     h->mf_ok = true;
     return FBP_OK;
 }
@@ -725,8 +739,9 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             int rc = build_mf_tables(h);
             if (rc != FBP_OK) return rc;
         }
-        // outside the matrix-core tables' limits the FP32 kernels run (same results)
-        mixed = mixed && h->mf_ok;
+        // outside the matrix-core tables' limits the FP32 kernels run (same results),
+        // and the first pass takes the FP32 kernel where the tiles' padding makes it faster
+        mixed = mixed && h->mf_ok && (h->mf_pays || (flags & FBP_FLAG_LIKELIHOOD_MFMA) != 0);
         screen_mf = screen_mf && h->mf_ok;
     }
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);

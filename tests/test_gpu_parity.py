@@ -30,7 +30,8 @@ def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="liter
     matrix-core correlations, the default path), "valu" (screen with FP32
     vector correlations) or "stream" (split screen with the streaming
     selection kernel, the path for U*(2n+1) > 24576); booleans map to
-    literal / split."""
+    literal / split.  mixed: True takes the matrix-core first likelihood pass
+    whatever the tiles' padding, "auto" lets the library choose it."""
     h.set_templates(T, bias, thr)
     flags = (_native.FLAG_OVERLAP_RECHECK if recheck else 0) | (_native.FLAG_ADJUSTED_CLIPS if clips else 0)
     mode = {True: "literal", False: "split"}.get(literal, literal)
@@ -42,6 +43,8 @@ def gpu_run(h, V, T, bias, thr, n_max, recheck=True, clips=False, literal="liter
         flags |= _native.FLAG_SELECT_STREAMING
     if mixed:
         flags |= _native.FLAG_PRECISION_MIXED
+        if mixed != "auto":
+            flags |= _native.FLAG_LIKELIHOOD_MFMA
     print(f"[gpu_run] C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n_max={n_max} "
           f"recheck={recheck}", flush=True)
     h.run(V, n_max, flags)
@@ -122,13 +125,14 @@ def test_mixed_precision_noisy_cases_match_oracle(handle, seed):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
+@pytest.mark.parametrize("mixed", [True, "auto"])
 @pytest.mark.parametrize("name,dur", [("small", 3.0), ("probe16", 0.5), ("probe64", 0.1)])
-def test_mixed_precision_workloads_match_oracle(handle, name, dur):
+def test_mixed_precision_workloads_match_oracle(handle, name, dur, mixed):
     d = synthetic.generate(name, duration_s=dur)
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=True),
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=mixed),
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 

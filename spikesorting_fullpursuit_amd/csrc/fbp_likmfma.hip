@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(NPS 
     extern __shared__ __align__(16) char smem[];
     __shared__ int64_t b_s[kSwWin];
     __shared__ float vs_s[kSwWin];
-    __shared__ float red[kMfThreads / 64];
+    __shared__ float red[kSwWin * (kMfThreads / 64)];
     const int C = ts.C, U = ts.U, L = ts.L;
     const int Cp = (C + 7) & ~7, RS = Cp + 8;
     const int LR = L + 1;  // window rows + one zero row
@@ -468,19 +468,37 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(NPS 
             b_s[threadIdx.x] = b;
         }
         __syncthreads();
-        // window maxima -> per-window scale
-        for (int k = 0; k < kSwWin; k++) {
-            const int64_t b = b_s[k];
-            float m = 0.0f;
-            if (b >= 0)
-                for (int e = threadIdx.x; e < L * C; e += blockDim.x) m = fmaxf(m, fabsf(st.V[(size_t)b * C + e]));
+        // window maxima -> per-window scale, both windows in one pass (float4 loads when
+        // the channels come in fours)
+        {
+            float m[kSwWin];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-            if (lane == 0) red[wv] = m;
+            for (int k = 0; k < kSwWin; k++) {
+                const int64_t b = b_s[k];
+                m[k] = 0.0f;
+                if (b < 0) continue;
+                if (C % 4 == 0) {
+                    const float4* src = reinterpret_cast<const float4*>(st.V + (size_t)b * C);
+                    for (int e = threadIdx.x; e < L * C / 4; e += blockDim.x) {
+                        const float4 q = src[e];
+                        m[k] = fmaxf(fmaxf(m[k], fmaxf(fabsf(q.x), fabsf(q.y))), fmaxf(fabsf(q.z), fabsf(q.w)));
+                    }
+                } else {
+                    for (int e = threadIdx.x; e < L * C; e += blockDim.x)
+                        m[k] = fmaxf(m[k], fabsf(st.V[(size_t)b * C + e]));
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kSwWin; k++) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) m[k] = fmaxf(m[k], __shfl_xor(m[k], off, 64));
+                if (lane == 0) red[k * nwv + wv] = m[k];
+            }
             __syncthreads();
-            if (threadIdx.x == 0) {
-                float mm = red[0];
-                for (int i = 1; i < nwv; i++) mm = fmaxf(mm, red[i]);
+            if (threadIdx.x < kSwWin) {
+                const int k = threadIdx.x;
+                float mm = red[k * nwv];
+                for (int i = 1; i < nwv; i++) mm = fmaxf(mm, red[k * nwv + i]);
                 int ex = 0;
                 if (mm > 0.0f) frexpf(mm, &ex);
                 vs_s[k] = mm > 0.0f ? ldexpf(1.0f, 14 - ex) : 1.0f;
@@ -488,16 +506,46 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(NPS 
             __syncthreads();
         }
         // stage the windows as split fp16 rows [sample][channel]; row L is zero
-        for (int e = threadIdx.x; e < kSwWin * LR * Cp; e += blockDim.x) {
-            const int k = e / (LR * Cp);
-            const int rem = e - k * LR * Cp;
-            const int j = rem / Cp, c = rem - j * Cp;
-            const int64_t b = b_s[k];
-            float v = 0.0f;
-            if (b >= 0 && j < L && c < C) v = st.V[(size_t)(b + j) * C + c] * vs_s[k];
-            const _Float16 hi = (_Float16)v;
-            Wh[(k * LR + j) * RS + c] = hi;
-            Wl[(k * LR + j) * RS + c] = (_Float16)(v - (float)hi);
+        if (C % 8 == 0) {
+            // eight channels per item: two float4 loads, two 16-byte LDS stores (Cp == C)
+            const int C8 = C / 8;
+            for (int e = threadIdx.x; e < kSwWin * LR * C8; e += blockDim.x) {
+                const int k = e / (LR * C8);
+                const int rem = e - k * LR * C8;
+                const int j = rem / C8, g = rem - j * C8;
+                const int64_t b = b_s[k];
+                float v[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) v[i] = 0.0f;
+                if (b >= 0 && j < L) {
+                    const float4* src = reinterpret_cast<const float4*>(st.V + (size_t)(b + j) * C + 8 * g);
+                    const float4 q0 = src[0], q1 = src[1];
+                    const float sc = vs_s[k];
+                    v[0] = q0.x * sc; v[1] = q0.y * sc; v[2] = q0.z * sc; v[3] = q0.w * sc;
+                    v[4] = q1.x * sc; v[5] = q1.y * sc; v[6] = q1.z * sc; v[7] = q1.w * sc;
+                }
+                half8 hh, hl;
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    const _Float16 hi = (_Float16)v[i];
+                    hh[i] = hi;
+                    hl[i] = (_Float16)(v[i] - (float)hi);
+                }
+                *reinterpret_cast<half8*>(Wh + (k * LR + j) * RS + 8 * g) = hh;
+                *reinterpret_cast<half8*>(Wl + (k * LR + j) * RS + 8 * g) = hl;
+            }
+        } else {
+            for (int e = threadIdx.x; e < kSwWin * LR * Cp; e += blockDim.x) {
+                const int k = e / (LR * Cp);
+                const int rem = e - k * LR * Cp;
+                const int j = rem / Cp, c = rem - j * Cp;
+                const int64_t b = b_s[k];
+                float v = 0.0f;
+                if (b >= 0 && j < L && c < C) v = st.V[(size_t)(b + j) * C + c] * vs_s[k];
+                const _Float16 hi = (_Float16)v;
+                Wh[(k * LR + j) * RS + c] = hi;
+                Wl[(k * LR + j) * RS + c] = (_Float16)(v - (float)hi);
+            }
         }
         __syncthreads();
         for (int tile = wv; tile < mt.ntiles; tile += nwv) {

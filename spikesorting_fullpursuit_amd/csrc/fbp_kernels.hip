@@ -69,26 +69,59 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
         const int npair = (int)npair_s;
         if (npair == 0) continue;  // uniform over the block
         if (threadIdx.x == 0) atomicAdd(cnt_pairs, (uint32_t)npair);
-        // channels to stage per window: those of its computed templates
+        // channels to stage per window: those of its computed templates; with C a
+        // multiple of 4, the channel quads that hold any of them (clist then lists quads)
+        const bool quads = C % 4 == 0;
         if ((int)threadIdx.x < nb) {
             const int b = threadIdx.x;
             int n = 0;
-            if (dirty == nullptr) {
-                for (int c = 0; c < C; c++) clist[b * C + n++] = c;
+            const uint64_t m = dirty == nullptr ? ~0ull : cun_s[b];  // dirty == nullptr whenever C > 64
+            if (quads) {
+                for (int q = 0; q < C / 4; q++)
+                    if (dirty == nullptr || ((m >> (4 * q)) & 0xFull) != 0ull) clist[b * C + n++] = q;
             } else {
-                const uint64_t m = cun_s[b];
                 for (int c = 0; c < C; c++)
-                    if ((m >> c) & 1ull) clist[b * C + n++] = c;
+                    if (dirty == nullptr || ((m >> c) & 1ull) != 0ull) clist[b * C + n++] = c;
             }
             ncl_s[b] = n;
         }
         __syncthreads();
-        // V is sample-major ([N][C]): read along the staged channels, store
-        // channel rows; eight loads in flight per thread
+        // V is sample-major ([N][C]): read along the staged channels (float4 per quad),
+        // store channel rows; several loads in flight per thread
         for (int b = 0; b < nb; b++) {
             const int ncl = ncl_s[b];
             const int tot = ncl * sv;
             const int64_t xb = (int64_t)W[w0 + b] * L;
+            if (quads) {
+                for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * blockDim.x) {
+                    float4 v[4];
+                    int c0[4], jj[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int e = e0 + q * blockDim.x;
+                        v[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        c0[q] = -1;
+                        jj[q] = 0;
+                        if (e < tot) {
+                            const int j = e / ncl, qi = e - j * ncl;
+                            const int64_t x = xb + j;
+                            c0[q] = 4 * clist[b * C + qi];
+                            jj[q] = j;
+                            if (x < N) v[q] = *reinterpret_cast<const float4*>(V + (size_t)x * C + c0[q]);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        if (c0[q] < 0) continue;
+                        float* dst = sV + (size_t)(b * C + c0[q]) * sv + jj[q];
+                        dst[0] = v[q].x;
+                        dst[sv] = v[q].y;
+                        dst[2 * sv] = v[q].z;
+                        dst[3 * sv] = v[q].w;
+                    }
+                }
+                continue;
+            }
             for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * blockDim.x) {
                 float v[8];
 #pragma unroll

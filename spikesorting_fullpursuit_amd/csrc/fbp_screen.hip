@@ -654,6 +654,8 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 // ---------------------------------------------------------------------------
 constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
 constexpr int kQR = 2;              // (pair, f-quad) items per thread per round of G loads
+constexpr int kSelLR = 12;          // pairs per round of the per-pair loads (divides kSelR)
+static_assert(kSelR % kSelLR == 0, "whole rounds");
 static_assert(kSelR <= 32, "pair masks are 32-bit");
 constexpr int kSelMaxBlock = 1024;  // threads per block at most (U * S <= kSelR * kSelMaxBlock)
 constexpr size_t kSelMaxLds = 96 * 1024;  // dynamic LDS of the register-resident selection at most
@@ -792,35 +794,33 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             A[f] = Bw[(int)lab * S + f];
         }
         // per-pair state: B, sign of LIK[u][b+s-n], shared-channel bit.  Loads
-        // are issued 8 pairs at a time from clamped (always valid) addresses.
+        // are issued kSelLR pairs at a time from clamped (always valid) addresses,
+        // B straight into the pair's register.
         float bv[kSelR];
         uint32_t pum = 0, shm = 0;
         {
             int u = u_t, s = s_t, t0 = (int)threadIdx.x;
             asm volatile("" : "+v"(u), "+v"(s), "+v"(t0));  // recompute the pair walk, do not keep it live
 #pragma unroll
-            for (int i0 = 0; i0 < kSelR; i0 += 8) {
-                float gb[8], lv[8];
-                int pw[8], bi[8];
-                bool ok[8];
+            for (int i0 = 0; i0 < kSelR; i0 += kSelLR) {
+                float lv[kSelLR];
+                int pw[kSelLR];
+                bool ok[kSelLR];
 #pragma unroll
-                for (int j = 0; j < 8; j++) {
+                for (int j = 0; j < kSelLR; j++) {
                     const int e = t0 + bd * (i0 + j);
                     const int64_t as = b + s - n;
                     ok[j] = e < US && as >= 0 && as < N - L;
                     const int uc = ok[j] ? u : 0, ec = ok[j] ? e : 0;
                     lv[j] = st.LIK[(size_t)uc * N + (ok[j] ? as : 0)];
-                    bi[j] = ec;
                     pw[j] = tb.pid[(size_t)lab * U + uc];  // global: pidw may not be written yet
+                    bv[i0 + j] = Bw[ec];
                     pair_step(u, s, du, ds, S);
                 }
-                __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(0);  // the round's loads go out together
 #pragma unroll
-                for (int j = 0; j < 8; j++) gb[j] = Bw[bi[j]];
-                __builtin_amdgcn_sched_barrier(0);  // issue the 8 pairs' loads together
-#pragma unroll
-                for (int j = 0; j < 8; j++) {
-                    bv[i0 + j] = ok[j] ? gb[j] : -INFINITY;
+                for (int j = 0; j < kSelLR; j++) {
+                    if (!ok[j]) bv[i0 + j] = -INFINITY;
                     if (ok[j] && lv[j] > 0.0f) pum |= 1u << (i0 + j);
                     if (ok[j] && pw[j] >= 0) shm |= 1u << (i0 + j);
                 }

@@ -105,6 +105,12 @@ int fbp_copy_clips(fbp_handle *h, float *clips);
 /* Residual voltage [C][N] of the last run into a host buffer. */
 int fbp_copy_residual(fbp_handle *h, float *out);
 int fbp_get_stats(fbp_handle *h, fbp_stats *out);
+/* Per-pass trace of the last fbp_run, one row of three int64 per pass:
+ * spikes added (num_additional_spikes, binary_pursuit_parallel.py:1210-1217),
+ * windows listed (new_window_indices, :1242) and windows flagged for the
+ * overlap recheck (overlap_window_indices, :977-979; 0 when the recheck is
+ * off).  Copies min(n_rows, max_rows) rows; *n_rows = number of passes. */
+int fbp_get_pass_log(fbp_handle *h, int64_t *rows, int32_t max_rows, int32_t *n_rows);
 
 /*
  * Template refinement at fixed spike times.  No counterpart in the reference:

@@ -446,6 +446,73 @@ done:
     return rc;
 }
 
+/*
+ * Single steps of the pass loop on state the caller holds (arrays of nwin =
+ * ceil(N / L) windows), so that known-answer tests can pin the window
+ * bookkeeping rule by rule:
+ *   fbpo_window_max_step  compute_template_maximum_likelihood for t = 0..U-1
+ *                         over the listed windows (binary_pursuit.cl:350-441)
+ *   fbpo_reassign_step    check_overlap_reassignments over the flagged
+ *                         windows (binary_pursuit.cl:822-875)
+ *   fbpo_pursuit_step     binary_pursuit over the listed windows
+ *                         (binary_pursuit.cl:918-1063); returns the number of
+ *                         spikes written to out_idx / out_lab
+ */
+static tset step_tset(int64_t N, int32_t U, int32_t L)
+{
+    tset ts;
+    ts.T = NULL; ts.C = 0; ts.N = N; ts.U = U; ts.L = L; ts.chan_nz = NULL;
+    return ts;
+}
+
+int fbpo_window_max_step(const float *LIK, int64_t N, int32_t U, int32_t L, const uint32_t *win, int64_t nw,
+                         uint32_t *best_idx, uint32_t *best_lab, float *best_lik, uint32_t *obest_idx,
+                         uint32_t *obest_lab, uint8_t *check, uint8_t *orc)
+{
+    if (N <= 0 || U <= 0 || L <= 0 || nw < 0) return FBPO_EINVAL;
+    tset ts = step_tset(N, U, L);
+    pstate st;
+    st.nwin = (N + L - 1) / L;
+    st.best_idx = best_idx; st.best_lab = best_lab; st.best_lik = best_lik;
+    st.obest_idx = obest_idx; st.obest_lab = obest_lab; st.check = check; st.orc = orc;
+    window_max(&ts, LIK, win, nw, &st);
+    return FBPO_OK;
+}
+
+int fbpo_reassign_step(int64_t N, int32_t L, const uint32_t *flist, int64_t nf, uint32_t *best_idx,
+                       uint32_t *best_lab, uint32_t *obest_idx, uint32_t *obest_lab, uint8_t *check, uint8_t *orc)
+{
+    if (N <= 0 || L <= 0 || nf < 0) return FBPO_EINVAL;
+    tset ts = step_tset(N, 1, L);
+    pstate st;
+    st.nwin = (N + L - 1) / L;
+    st.best_idx = best_idx; st.best_lab = best_lab; st.best_lik = NULL;
+    st.obest_idx = obest_idx; st.obest_lab = obest_lab; st.check = check; st.orc = orc;
+    uint8_t *snap = (uint8_t *)malloc((size_t)st.nwin);
+    if (!snap) return FBPO_ENOMEM;
+    reassign(&ts, flist, nf, &st, snap);
+    free(snap);
+    return FBPO_OK;
+}
+
+int64_t fbpo_pursuit_step(int64_t N, int32_t L, const float *thr, const uint32_t *win, int64_t nw,
+                          uint32_t *best_idx, uint32_t *best_lab, float *best_lik, uint8_t *check, uint8_t *orc,
+                          uint32_t *out_idx, uint32_t *out_lab)
+{
+    if (N <= 0 || L <= 0 || nw < 0) return FBPO_EINVAL;
+    tset ts = step_tset(N, 1, L);
+    pstate st;
+    st.nwin = (N + L - 1) / L;
+    st.best_idx = best_idx; st.best_lab = best_lab; st.best_lik = best_lik;
+    st.obest_idx = NULL; st.obest_lab = NULL; st.check = check; st.orc = orc;
+    uint8_t *has = (uint8_t *)malloc((size_t)(nw > 0 ? nw : 1));
+    uint8_t *snap = (uint8_t *)malloc((size_t)st.nwin);
+    if (!has || !snap) { free(has); free(snap); return FBPO_ENOMEM; }
+    int64_t n = pursuit(&ts, thr, win, nw, &st, has, snap, out_idx, out_lab);
+    free(has); free(snap);
+    return n;
+}
+
 /* get_adjusted_clips, binary_pursuit.cl:1088-1130: clip = residual + template. */
 int fbpo_adjusted_clips(const float *R, int64_t C, int64_t N, const float *T, int32_t U, int32_t L,
                         const uint32_t *idx, const uint32_t *lab, int64_t n, float *clips)

@@ -172,6 +172,7 @@ struct fbp_handle {
     int64_t n_spikes = 0;
     bool have_result = false, have_clips = false;
     fbp_stats stats{};
+    std::vector<int64_t> pass_log;  // per pass: spikes added, windows listed, windows flagged
 };
 
 static int set_err(fbp_handle* h, int code, const std::string& msg)
@@ -796,6 +797,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     int64_t total = 0;
     int passes = 0;
     int64_t pairs_total = 0;
+    h->pass_log.clear();
     for (;;) {
         launch_fill_u32(st.counters + CNT_PAIRS, 0u, 1, s);
         stats.windows_listed += nW;
@@ -883,6 +885,9 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         passes++;
         pairs_total += h->h_counters[CNT_PAIRS];
         const uint32_t nadd = h->h_counters[CNT_SPIKES];
+        h->pass_log.push_back(nadd);
+        h->pass_log.push_back(nW);
+        h->pass_log.push_back(nF);
         if (nadd == 0) break;
         tm.begin(TM_RESID);
         launch_residual(ts, st.V, N, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, reach, s);
@@ -980,6 +985,17 @@ int fbp_get_stats(fbp_handle* h, fbp_stats* out)
 {
     if (!h || !out) return FBP_E_INVALID;
     *out = h->stats;
+    return FBP_OK;
+}
+
+int fbp_get_pass_log(fbp_handle* h, int64_t* rows, int32_t max_rows, int32_t* n_rows)
+{
+    if (!h || !n_rows || max_rows < 0 || (max_rows > 0 && !rows)) return FBP_E_INVALID;
+    if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
+    const int32_t n = (int32_t)(h->pass_log.size() / 3);
+    *n_rows = n;
+    const size_t k = (size_t)std::min(n, max_rows) * 3;
+    if (k) std::memcpy(rows, h->pass_log.data(), k * sizeof(int64_t));
     return FBP_OK;
 }
 

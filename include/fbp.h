@@ -138,6 +138,19 @@ int fbp_refine_forward(const float *V, int64_t ldv, int64_t N, int32_t C, const 
                        const uint32_t *idx, const uint32_t *lab, const float *amp, int64_t n_spikes,
                        float *R, int64_t ldr, double *energy, void *work, int64_t work_bytes, void *stream);
 
+/* fbp_refine_forward without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * per-block partials live in a static device array of 3 MiB per GPU (held
+ * until the process ends), so N is limited to 67108864 samples (larger N:
+ * FBP_E_UNSUPPORTED).  Allocates nothing; the stream must belong to the
+ * current device.  Every call on a GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other threads) race.  Results equal
+ * fbp_refine_forward's bit for bit. */
+int fbp_refine_forward_noalloc(const float *V, int64_t ldv, int64_t N, int32_t C, const float *T, int32_t U,
+                               int32_t L, const uint32_t *idx, const uint32_t *lab, const float *amp,
+                               int64_t n_spikes, float *R, int64_t ldr, double *energy, void *stream);
+
 /* Gradients of E from the residual R of fbp_refine_forward:
  *   grad_T[u][c][i] = -sum_{k of unit u, p_k+i < N} a_k R[c][p_k+i]     ([U][C][L], may be NULL)
  *   grad_amp[k]     = -sum_{c, i: p_k+i < N} T[lab_k][c][i] R[c][p_k+i] ([n_spikes], may be NULL)

@@ -31,7 +31,7 @@ FLAG_LIKELIHOOD_MFMA = 128
 EXPORTS = (
     "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
     "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
-    "fbp_get_stats", "fbp_get_pass_log", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_backward",
+    "fbp_get_stats", "fbp_get_pass_log", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_forward_noalloc", "fbp_refine_backward",
     "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
     "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
     "fbp_preproc_workspace_bytes", "fbp_abs_medians", "fbp_count_above", "fbp_channel_cov_rows", "fbp_zca_apply",
@@ -99,6 +99,7 @@ def load():
     L.fbp_get_pass_log.argtypes = [vp, vp, i32, ctypes.POINTER(i32)]
     L.fbp_refine_workspace_bytes.argtypes = [i64, i64, i32, i32, i32, ctypes.POINTER(i64)]
     L.fbp_refine_forward.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, i64, vp, vp, i64, vp]
+    L.fbp_refine_forward_noalloc.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, i64, vp, vp]
     L.fbp_refine_backward.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, vp, vp, vp, vp, i64,
                                       vp]
     L.fbp_refine_last_error.argtypes = []

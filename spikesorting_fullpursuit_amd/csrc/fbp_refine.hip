@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/fbp.h"
@@ -327,6 +328,21 @@ __global__ __launch_bounds__(256) void k_grad_tmpl_final(const float* __restrict
 
 thread_local std::string t_err;
 
+// Scratch of fbp_refine_forward_noalloc: the residual kernel's per-block FP64 energy
+// partials (kNoAllocMaxBlocks doubles) followed by its per-block input-check flags
+// (kNoAllocMaxBlocks u32), i.e. recordings of up to 2^26 samples.  Every block writes both
+// entries before k_energy_final reads them, so nothing needs clearing between calls.  Each
+// device holds its own copy until the process ends; host code uses the per-device address
+// from hipGetSymbolAddress, never the array's name.
+constexpr int64_t kNoAllocMaxBlocks = (int64_t)1 << 18;
+constexpr int64_t kNoAllocFlagOff = kNoAllocMaxBlocks * 8;
+constexpr int64_t kNoAllocBytes = kNoAllocFlagOff + kNoAllocMaxBlocks * 4;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
 int fail(int code, const char* msg)
 {
     t_err = msg;
@@ -387,6 +403,57 @@ extern "C" int fbp_refine_forward(const float* V, int64_t ldv, int64_t N, int32_
                        (int)L, idx, lab, amp, n_spikes, R, ldr, epart, flag);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kRed), 0, s, epart, flag, lay.nblk, energy);
     return launch_status("fbp_refine_forward");
+}
+
+// Device address of `scratch` on the current device, looked up once per device.
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        t_err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return FBP_E_HIP;
+    }
+    if (dev < 0 || dev >= kMaxDevices) return fail(FBP_E_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            t_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return FBP_E_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FBP_OK;
+}
+
+extern "C" int fbp_refine_forward_noalloc(const float* V, int64_t ldv, int64_t N, int32_t C, const float* T,
+                                          int32_t U, int32_t L, const uint32_t* idx, const uint32_t* lab,
+                                          const float* amp, int64_t n_spikes, float* R, int64_t ldr, double* energy,
+                                          void* stream)
+{
+    int rc = check_shape(N, C, U, L, n_spikes);
+    if (rc != FBP_OK) return rc;
+    if (ceil_div(N, kTile) > kNoAllocMaxBlocks)
+        return fail(FBP_E_UNSUPPORTED, "fbp_refine_forward_noalloc: N above 67108864 samples (its static scratch "
+                                       "holds 262144 blocks of 256 samples); use fbp_refine_forward");
+    if (!V || !T || !R || !energy) return fail(FBP_E_INVALID, "null device pointer");
+    if (n_spikes > 0 && (!idx || !lab || !amp)) return fail(FBP_E_INVALID, "null spike arrays");
+    if (ldv < N || ldr < N) return fail(FBP_E_INVALID, "row strides must be >= N");
+    void* base = nullptr;
+    rc = scratch_address(&base);
+    if (rc != FBP_OK) return rc;
+    const int64_t nblk = ceil_div(N, kTile);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char* w = static_cast<char*>(base);
+    double* epart = reinterpret_cast<double*>(w);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(w + kNoAllocFlagOff);
+    hipLaunchKernelGGL(k_residual_energy, dim3((unsigned)nblk), dim3(kTile), 0, s, V, ldv, N, (int)C, T, (int)U,
+                       (int)L, idx, lab, amp, n_spikes, R, ldr, epart, flag);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(kRed), 0, s, epart, flag, nblk, energy);
+    return launch_status("fbp_refine_forward_noalloc");
 }
 
 extern "C" int fbp_refine_backward(const float* R, int64_t ldr, int64_t N, int32_t C, const float* T, int32_t U,

@@ -117,6 +117,44 @@ def test_small_workload_3s_matches_oracle(handle, recheck, literal):
                 fbp_oracle.pursuit(V, T, bias, thr, n, do_recheck=recheck, return_residual=True))
 
 
+_FULL_SMALL = {}
+
+
+def _small_full():
+    """The 4-channel configuration at its full length (40 kHz, 30 s, 5 units) and its oracle run."""
+    if not _FULL_SMALL:
+        d = synthetic.generate("small")
+        T, bias, thr = synthetic.kernel_inputs(d)
+        n = d["n_max_shift_inds"]
+        _FULL_SMALL["args"] = (d["voltage"], T, bias, thr, n)
+        _FULL_SMALL["oracle"] = fbp_oracle.pursuit(d["voltage"], T, bias, thr, n, return_residual=True)
+    return _FULL_SMALL["args"], _FULL_SMALL["oracle"]
+
+
+@pytest.mark.parametrize("literal,mixed", [("literal", False), ("split", False), ("valu", False), ("split", "auto"),
+                                           ("split", True)])
+def test_small_workload_full_30s_matches_oracle(handle, literal, mixed):
+    args, ora = _small_full()
+    assert_same(gpu_run(handle, *args, literal=literal, mixed=mixed), ora)
+
+
+@pytest.mark.parametrize("name,dur", [("probe64", 10.0), ("probe16", 30.0)])
+def test_screened_recheck_matches_literal_gpu(handle, name, dur):
+    """The 64-channel configuration on 10 s and the 16-channel one on 30 s, where the CPU
+    oracle's literal recheck is too slow: every screen path against the literal GPU recheck
+    (every overlap item evaluated in the reference's order)."""
+    d = synthetic.generate(name, duration_s=dur)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = d["voltage"]
+    ref = gpu_run(handle, V, T, bias, thr, n, literal="literal")
+    ref_log = handle.pass_log()
+    assert len(ref[0]) > 1000
+    for mode, mixed in (("split", "auto"), ("valu", False), ("stream", "auto")):
+        assert_same(gpu_run(handle, V, T, bias, thr, n, literal=mode, mixed=mixed), ref)
+        np.testing.assert_array_equal(handle.pass_log(), ref_log)
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_mixed_precision_noisy_cases_match_oracle(handle, seed):
     V, T, bias, thr, on, lab = cases.noisy_case(seed)

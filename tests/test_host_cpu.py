@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from spikesorting_fullpursuit_amd import _native
+from spikesorting_fullpursuit_amd import _native, synthetic
 from spikesorting_fullpursuit_amd.overlap import binary_pursuit_parallel as bpp
 from spikesorting_fullpursuit_amd.overlap import neuron_separability
 from spikesorting_fullpursuit_amd.processing.conversions import time_window_to_samples
@@ -165,3 +165,23 @@ def test_refine_abi_checks_arguments_without_a_gpu():
     rc = lib.fbp_refine_backward(vp(8), 1000, 1000, 4, vp(8), 3, 32, vp(8), vp(8), vp(8), 10, vp(8), vp(8), None,
                                  None, vp(8), 1 << 20, None)
     assert rc == _native.FBP_E_INVALID and b"no gradient" in lib.fbp_refine_last_error()
+
+
+def test_probe16_overlap_fraction():
+    """The 16-channel configuration asks that >= 15 % of spikes lie within 1 ms of a
+    spike of another unit (40 units, 2-80 Hz, 300 s)."""
+    w = synthetic.workload("probe16")
+    rng = np.random.default_rng(w.seed)
+    synthetic.make_templates(w, rng)
+    on, lab, _ = synthetic.spike_trains(w, rng, w.n_samples)
+    o = np.argsort(on, kind="stable")
+    on, lab = on[o], lab[o]
+    win = int(round(0.001 * w.fs))
+    close = np.zeros(len(on), bool)
+    d = 1
+    while d < len(on) and np.any(on[d:] - on[:-d] <= win):
+        hit = (on[d:] - on[:-d] <= win) & (lab[d:] != lab[:-d])
+        close[d:] |= hit
+        close[:-d] |= hit
+        d += 1
+    assert close.mean() >= 0.15

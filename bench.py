@@ -2,8 +2,10 @@
 
 One step = one drop-in binary_pursuit() call over the whole recording
 (64 ch, 30 kHz, 600 s, 120 units, 48-sample templates on 7-channel
-neighbourhoods, n_max_shift_inds = L - 1, overlap recheck on), with the voltage
-already resident in HBM.  Prints one JSON line.
+neighbourhoods, n_max_shift_inds = L - 1, overlap recheck on) with the voltage
+a host numpy array, so the step includes its host-to-device copy; the rate
+with the voltage already resident in HBM is reported beside it.  Prints one
+JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--duration S]
 """
@@ -51,8 +53,9 @@ def main():
     ap.add_argument("--workload", default="probe64")
     ap.add_argument("--duration", type=float, default=None, help="recording length in s (default: full)")
     ap.add_argument("--literal", action="store_true", help="evaluate every overlap item literally")
-    ap.add_argument("--cpu-sample-s", type=float, default=0.1)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-sample-s", type=float, default=1.0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: os.cpu_count(), capped by OMP_NUM_THREADS when set)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--variant", default="pursuit", choices=["pursuit", "refine", "batch"],
                     help="refine: template refinement on the 32-channel 120 s recording; batch: 256 independent "
@@ -60,7 +63,9 @@ def main():
     ap.add_argument("--segments", type=int, default=256, help="segments of the batch variant")
     ap.add_argument("--profile-run", action="store_true",
                     help="for hardware counters: three 2 GiB device-to-device copies (the copy kernel of known "
-                         "size that scales FETCH_SIZE / WRITE_SIZE), then one mixed-precision drop-in call; no JSON")
+                         "size that scales FETCH_SIZE / WRITE_SIZE), then one mixed-precision drop-in call "
+                         "(FP32 with --profile-fp32) on the device-resident voltage; no JSON")
+    ap.add_argument("--profile-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -82,12 +87,15 @@ def main():
     C, N = V.shape
     w = d["workload"]
     U, _, L = d["templates"].shape
+    # The drop-in's usual input is a host array (the reference reads the voltage from host memory and
+    # copies it to the device per chunk, binary_pursuit_parallel.py:524-545): the headline times that.
+    Vh = V.cpu().numpy()
     log(f"generated {args.workload}: C={C} N={N} U={U} L={L} spikes={len(d['truth_onsets'])} "
-        f"in {time.time() - t0:.1f}s")
+        f"in {time.time() - t0:.1f}s (host copy {Vh.nbytes / 1e9:.2f} GB)")
 
-    def step(precision, literal=args.literal):
+    def step(precision, voltage, literal=args.literal):
         m = dict(d["separability_metrics"])
-        return bpp.binary_pursuit(V, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"],
+        return bpp.binary_pursuit(voltage, np.float32, d["sort_info"], m, n_max_shift_inds=d["n_max_shift_inds"],
                                   recheck_literal=literal, precision=precision)
 
     if args.profile_run:
@@ -98,33 +106,47 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-        out = step("mixed")
+        out = step("fp32" if args.profile_fp32 else "mixed", V)
         torch.cuda.synchronize()
-        log(f"profile run: {nbytes} B copied 3 times; one mixed call, spikes={len(out[0])}")
+        log(f"profile run: {nbytes} B copied 3 times; one {'FP32' if args.profile_fp32 else 'mixed'} call, "
+            f"spikes={len(out[0])}")
         return
 
-    def timed(precision, k):
+    def timed(precision, voltage, k):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(k):
-            o = step(precision)
+            o = step(precision, voltage)
         torch.cuda.synchronize()
         return (time.perf_counter() - t1) / k, o
 
     for i in range(args.warmup):
         t1 = time.time()
-        out = step("mixed")
-        out32 = step("fp32")
+        out = step("mixed", Vh)
+        out32 = step("fp32", Vh)
+        step("mixed", V)
         torch.cuda.synchronize()
         log(f"warmup {i}: {time.time() - t1:.3f}s spikes={len(out[0])}")
-    sec_mixed, out = timed("mixed", args.steps)
-    sec_fp32, out32 = timed("fp32", max(1, args.steps // 2))
-    identical = bool(np.array_equal(out[0], out32[0]) and np.array_equal(out[1], out32[1]))
+    k32 = max(1, args.steps // 2)
+    sec_mixed, out = timed("mixed", Vh, args.steps)
+    sec_fp32, out32 = timed("fp32", Vh, k32)
+    sec_mixed_dev, out_dev = timed("mixed", V, args.steps)
+    sec_fp32_dev, out32_dev = timed("fp32", V, k32)
+
+    def same(a, b):
+        return bool(np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]))
+
+    identical = same(out, out32)
+    identical_all = identical and same(out, out_dev) and same(out, out32_dev)
     ms_per_step = 1000.0 * sec_mixed
     value = N * C / sec_mixed / 1e6
     value32 = N * C / sec_fp32 / 1e6
-    log(f"mixed: {ms_per_step:.1f} ms/step, {value:.1f} Msamples*ch/s; fp32: {1000 * sec_fp32:.1f} ms/step, "
-        f"{value32:.1f}; identical spike set: {identical}; spikes={len(out[0])}")
+    value_dev = N * C / sec_mixed_dev / 1e6
+    value32_dev = N * C / sec_fp32_dev / 1e6
+    log(f"host input: mixed {ms_per_step:.1f} ms/step = {value:.1f} Msamples*ch/s, fp32 {1000 * sec_fp32:.1f} ms = "
+        f"{value32:.1f}; device-resident: mixed {1000 * sec_mixed_dev:.1f} ms = {value_dev:.1f}, fp32 "
+        f"{1000 * sec_fp32_dev:.1f} ms = {value32_dev:.1f}; identical spike sets: {identical_all}; "
+        f"spikes={len(out[0])}")
 
     # Untimed runs with per-kernel HIP-event timing for the roofline.
     from spikesorting_fullpursuit_amd import _native
@@ -142,7 +164,10 @@ def main():
     log("kernel stats mixed:", json.dumps(stm))
     classes = {"likelihood": stm["ms_likelihood"], "recheck": stm["ms_recheck"], "window_max": stm["ms_window_max"],
                "pursuit": stm["ms_pursuit"], "residual": stm["ms_residual"], "other": stm["ms_other"]}
+    classes32 = {"likelihood": st["ms_likelihood"], "recheck": st["ms_recheck"], "window_max": st["ms_window_max"],
+                 "pursuit": st["ms_pursuit"], "residual": st["ms_residual"], "other": st["ms_other"]}
     S = 2 * int(d["n_max_shift_inds"]) + 1
+    nz = np.count_nonzero(np.any(np.asarray(T3) != 0, axis=2), axis=1)
     # dominant kernel of the mixed path: the overlap screen selection.  Its
     # algorithmic traffic per flagged window is B(u, s) and the sign of
     # LIK[u][b+s-n] for every (unit, shift) pair plus the window's voltage.
@@ -151,61 +176,81 @@ def main():
     sel_bytes_per_launch = stm["windows_flagged"] * sel_bytes_per_window / max(1, stm["n_select_launches"])
     sel_gbs = sel_bytes_per_launch / (sel_launch_ms * 1e-3) / 1e9 if sel_launch_ms > 0 else 0.0
     traffic = profiled_traffic("k_screen_select_reg", args.workload) if args.duration is None else None
-    mf_ms = stm["ms_likelihood"]
-    mf_tflops = stm["likelihood_flops"] / (mf_ms * 1e-3) / 1e12 if mf_ms > 0 else 0.0
+    # FP32 path: its dominant kernel is k_screen_corr, the window-truncated correlations B(u, s) of every
+    # unit and shift of each flagged window, 2 L nz(u) flops per (unit, shift) on the vector units.
+    corr_flops_per_window = 2.0 * L * S * float(np.sum(nz))
+    corr_ms = st["ms_recheck"] - st["ms_select"]
+    corr_launch_ms = corr_ms / max(1, st["n_select_launches"])
+    corr_flops_per_launch = st["windows_flagged"] * corr_flops_per_window / max(1, st["n_select_launches"])
+    corr_tflops = corr_flops_per_launch / (corr_launch_ms * 1e-3) / 1e12 if corr_launch_ms > 0 else 0.0
     lik_avg_ms = st["ms_likelihood"] / max(1, st["n_likelihood_launches"])
-    lik_tflops = st["likelihood_flops"] / (st["ms_likelihood"] * 1e-3) / 1e12 if st["ms_likelihood"] > 0 else 0.0
-    # flops the FP32 kernel actually performs: after the first pass it recomputes
+    # flops the FP32 likelihood kernel performs: after the first pass it recomputes
     # only the (window, template) pairs whose channels changed, each 2 L^2 nz flops
-    nz_mean = float(np.mean(np.count_nonzero(np.any(np.asarray(T3) != 0, axis=2), axis=1)))
-    lik_done = st["likelihood_pairs_computed"] * 2.0 * L * L * nz_mean
+    lik_done = st["likelihood_pairs_computed"] * 2.0 * L * L * float(np.mean(nz))
     lik_done_tflops = lik_done / (st["ms_likelihood"] * 1e-3) / 1e12 if st["ms_likelihood"] > 0 else 0.0
     roofline = {
-        "kernel": "k_screen_select_reg (overlap screen selection)",
+        "kernel": "k_screen_select_reg (overlap screen selection, dominant kernel of the mixed path)",
         "bound": "hbm",
         "achieved": round(sel_gbs, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(sel_gbs / HBM_PEAK_GBS, 5),
         "traffic": traffic,
-        "traffic_source": TRAFFIC_PROFILE if traffic is not None else None,
+        "traffic_source": (f"{TRAFFIC_PROFILE}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of bench.py --profile-run, "
+                           "copy-kernel scaled (not measured in this run)") if traffic is not None else None,
         "avg_launch_ms": round(sel_launch_ms, 4),
         "algorithmic_bytes_per_window": sel_bytes_per_window,
         "bytes_per_launch": round(sel_bytes_per_launch),
         "ms_by_class_mixed": {k: round(v, 3) for k, v in classes.items()},
+        "ms_by_class_fp32": {k: round(v, 3) for k, v in classes32.items()},
         "ms_select_mixed": round(stm["ms_select"], 3),
-        "likelihood_mfma": {"kernel": "likelihood of the mixed run: pass 1 by k_likmfma (fp16 MFMA + exact FP32 pass "
-                                      "of the flagged tiles) where its tiles' channel padding pays, else by "
-                                      "k_likelihood; k_likelihood on the changed pairs after",
-                            "achieved_tflops_equiv": round(mf_tflops, 2), "peak": 2500.0, "ms": round(mf_ms, 3),
-                            "note": "algorithmic FP32 flops of every listed (window, template) pair / class time"},
-        "likelihood_fp32": {"kernel": "k_likelihood (FP32 path)", "achieved_tflops": round(lik_done_tflops, 3),
+        "fp32_path": {
+            "kernel": "k_screen_corr (FP32 overlap-screen correlations, dominant kernel of the FP32 path)",
+            "bound": "fp32 vector FMA", "achieved": round(corr_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(corr_tflops / FP32_PEAK_TFLOPS, 4),
+            "peak_unpacked": FP32_PEAK_TFLOPS / 2, "frac_unpacked": round(corr_tflops / (FP32_PEAK_TFLOPS / 2), 4),
+            "avg_launch_ms": round(corr_launch_ms, 4), "flops_per_window": corr_flops_per_window,
+            "note": "algorithmic flops 2 L S sum_u nz(u) per flagged window / HIP-event time of the recheck class "
+                    "minus the selection; the build issues no packed FP32 instructions, so one FMA per lane per "
+                    "cycle (peak_unpacked) bounds it"},
+        "likelihood_fp32": {"kernel": "k_likelihood (FP32 likelihood)", "achieved_tflops": round(lik_done_tflops, 3),
                             "peak": FP32_PEAK_TFLOPS, "frac": round(lik_done_tflops / FP32_PEAK_TFLOPS, 4),
                             "peak_unpacked": FP32_PEAK_TFLOPS / 2,
                             "frac_unpacked": round(lik_done_tflops / (FP32_PEAK_TFLOPS / 2), 4),
                             "avg_launch_ms": round(lik_avg_ms, 4),
                             "pairs_computed": st["likelihood_pairs_computed"],
-                            "listed_tflops_equiv": round(lik_tflops, 3),
-                            "note": "achieved: recomputed (window, template) pairs x 2 L^2 x mean non-zero channels "
-                                    "/ kernel time; listed_tflops_equiv counts every template of every listed "
-                                    "window, as if none were skipped; the build issues no packed FP32 "
-                                    "instructions, so one FMA per lane per cycle (peak_unpacked) bounds it"},
+                            "note": "recomputed (window, template) pairs x 2 L^2 x mean non-zero channels / kernel "
+                                    "time"},
         "passes": stm["passes"],
     }
+    if stm["n_likelihood_mfma_launches"] > 0:
+        mf_ms = stm["ms_likelihood"]
+        roofline["likelihood_mixed"] = {
+            "kernel": "likelihood class of the mixed run: k_likmfma (matrix-core pass 1) + k_likelihood after",
+            "launches_matrix_core": stm["n_likelihood_mfma_launches"], "ms": round(mf_ms, 3)}
 
     cpu = None
     parity = None
     if not args.no_cpu:
         from oracle import fbp_oracle
         ns = int(round(args.cpu_sample_s * w.fs))
-        Vs = V[:, :ns].cpu().numpy()
-        nthr = min(args.cpu_threads, os.cpu_count() or 1)
+        Vs = np.ascontiguousarray(Vh[:, :ns])
+        omp = os.environ.get("OMP_NUM_THREADS")
+        nthr = args.cpu_threads or min(os.cpu_count() or 1, int(omp) if omp and omp.isdigit() else 1 << 30)
         t2 = time.perf_counter()
         oi, ol = fbp_oracle.pursuit(Vs, T3, bias, thr, d["n_max_shift_inds"], nthreads=nthr)
         tc = time.perf_counter() - t2
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": round(ns * C / tc / 1e6, 6), "unit": "Msamples·ch/s", "cores": nthr, "kind": "port",
-               "sample": f"first {args.cpu_sample_s} s of the same recording ({ns} samples x {C} ch), "
-                         f"C restatement with OpenMP, {tc:.1f} s"}
+               "sample": f"first {args.cpu_sample_s} s of the same recording ({ns} samples x {C} ch, "
+                         f"{len(oi)} spikes), C restatement with OpenMP on {nthr} threads, literal overlap recheck, "
+                         f"{tc:.1f} s",
+               "cpu_model": model, "os_cpu_count": os.cpu_count(), "omp_num_threads": omp}
         h.run(None, d["n_max_shift_inds"], _native.FLAG_OVERLAP_RECHECK | _native.FLAG_PRECISION_MIXED |
               (_native.FLAG_RECHECK_LITERAL if args.literal else 0), ld=V.stride(0), n_samples=ns,
               device_ptr=V.data_ptr())
@@ -228,7 +273,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.workload}: {C} ch, {w.fs / 1000:g} kHz, {N / w.fs:g} s, {U} units, L={L}, "
-                        f"n_max_shift_inds={d['n_max_shift_inds']}, overlap recheck on, voltage resident in HBM",
+                        f"n_max_shift_inds={d['n_max_shift_inds']}, overlap recheck on; voltage a host numpy "
+                        f"float32 array, so each step includes its H2D (pinned staging) and the D2H of the spikes",
             "spikes_found": int(len(out[0])),
             "spikes_planted": int(len(d["truth_onsets"])),
             "recheck": "literal" if args.literal else "screened",
@@ -237,6 +283,12 @@ def main():
             "fp32_ms_per_step": round(1000 * sec_fp32, 3),
             "mixed_over_fp32": round(value / value32, 3),
             "identical_spike_set_mixed_vs_fp32": identical,
+            "identical_spike_set_all_runs": identical_all,
+            "device_resident_value": round(value_dev, 3),
+            "device_resident_ms_per_step": round(1000 * sec_mixed_dev, 3),
+            "device_resident_fp32_value": round(value32_dev, 3),
+            "device_resident_mixed_over_fp32": round(value_dev / value32_dev, 3),
+            "h2d_ms_per_step": round(1000 * (sec_mixed - sec_mixed_dev), 3),
             "likelihood_rescored_frac": round(stm["likelihood_rescored"] / max(1, stm["likelihood_positions"]), 5),
         },
         "roofline": roofline,

@@ -72,6 +72,7 @@ typedef struct fbp_stats {
     int64_t likelihood_pairs_computed; /* (window, template) pairs whose likelihood was computed; the others
                                           had no channel change in the window's samples since their last
                                           computation and kept their (identical) values */
+    int64_t n_likelihood_mfma_launches; /* launches of the matrix-core likelihood pass (mixed precision) */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

@@ -65,6 +65,7 @@ class FbpStats(ctypes.Structure):
         ("ms_select", ctypes.c_double),
         ("n_select_launches", ctypes.c_int64),
         ("likelihood_pairs_computed", ctypes.c_int64),
+        ("n_likelihood_mfma_launches", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -82,6 +83,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is not built; run `python -m spikesorting_fullpursuit_amd.build`")
+    # One HIP runtime per process: torch ships its own libamdhip64 with the same
+    # SONAME (libamdhip64.so.7) as /opt/rocm's, so whichever is loaded first serves
+    # every later user, and torch cannot initialise its devices on a runtime other
+    # than its own.  Load torch's first when torch is installed.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, u32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint32
     L.fbp_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]

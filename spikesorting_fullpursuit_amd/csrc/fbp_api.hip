@@ -7,10 +7,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fbp.h"
@@ -133,6 +138,82 @@ hipError_t grow_keep(DevBuf& b, size_t bytes, size_t used, hipStream_t s)
     return hipSuccess;
 }
 
+// Host threads that copy the rows of one voltage piece between the caller's
+// (pageable) array and a pinned staging buffer.  run() hands out jobs 0..n-1,
+// the calling thread takes part, and it returns when every job is done.
+class CopyPool {
+public:
+    explicit CopyPool(int n)
+    {
+        for (int i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int njobs, const std::function<void(int)>& f)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        job_ = &f;
+        njobs_ = njobs;
+        next_ = 0;
+        done_ = 0;
+        gen_++;
+        cv_.notify_all();
+        while (next_ < njobs_) {
+            const int j = next_++;
+            lk.unlock();
+            f(j);
+            lk.lock();
+            ++done_;
+        }
+        cv_done_.wait(lk, [&] { return done_ == njobs_; });
+        job_ = nullptr;
+        njobs_ = 0;
+    }
+
+private:
+    void loop()
+    {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || (gen_ != seen && next_ < njobs_); });
+            if (stop_) return;
+            seen = gen_;
+            while (next_ < njobs_) {
+                const int j = next_++;
+                const std::function<void(int)>* f = job_;
+                lk.unlock();
+                (*f)(j);
+                lk.lock();
+                if (++done_ == njobs_) cv_done_.notify_all();
+            }
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    const std::function<void(int)>* job_ = nullptr;
+    int njobs_ = 0, next_ = 0, done_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Host voltage moves through kStageBufs pieces of at most kStageBytes: the
+// rows of piece k are copied into pinned buffer k % kStageBufs by the pool
+// while the DMA of piece k-1 runs; each piece is transposed into the
+// sample-major working copy on the device (no full-size staging copy).
+constexpr int kStageBufs = 3;
+constexpr size_t kStageBytes = (size_t)32 << 20;
+constexpr int kCopyThreads = 8;
+constexpr size_t kCopySegBytes = (size_t)1 << 20;  // rows split into jobs of at most 1 MiB
+
 }  // namespace
 
 struct fbp_handle {
@@ -164,7 +245,7 @@ struct fbp_handle {
     bool mf_pays = false;  // the matrix-core first pass is expected to beat the FP32 kernel
     // chunk state
     int64_t N = 0, nwin = 0;
-    DevBuf V, Vtmp, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
+    DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
     DevBuf check, check_snap, orc, giveup, has, W, F, cand_idx, cand_lab, counters, blockbuf, dirty;
     DevBuf part_v, part_k;
     DevBuf sp_idx, sp_lab, clips;
@@ -173,6 +254,12 @@ struct fbp_handle {
     bool have_result = false, have_clips = false;
     fbp_stats stats{};
     std::vector<int64_t> pass_log;  // per pass: spikes added, windows listed, windows flagged
+    // host <-> device staging of the voltage (created on first use)
+    float* pinned[kStageBufs] = {};
+    DevBuf dstage[kStageBufs];
+    hipEvent_t stage_ev[kStageBufs] = {};
+    bool stage_busy[kStageBufs] = {};
+    std::unique_ptr<CopyPool> pool;
 };
 
 static int set_err(fbp_handle* h, int code, const std::string& msg)
@@ -279,6 +366,15 @@ struct Timer {
                 hipEventDestroy(p.first);
                 hipEventDestroy(p.second);
             }
+    }
+};
+// Two events owned for the length of a scope.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
     }
 };
 enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT };
@@ -537,6 +633,92 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     return FBP_OK;
 }
 
+static int ensure_staging(fbp_handle* h)
+{
+    if (!h->pool) h->pool.reset(new CopyPool(kCopyThreads));
+    for (int b = 0; b < kStageBufs; b++) {
+        if (!h->pinned[b])
+            HIP_TRY(hipHostMalloc((void**)&h->pinned[b], kStageBytes, hipHostMallocDefault), "pinned staging");
+        HIP_TRY(h->dstage[b].ensure(kStageBytes), "alloc device staging");
+        if (!h->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&h->stage_ev[b], hipEventDisableTiming), "event");
+    }
+    return FBP_OK;
+}
+
+// Samples per staged piece for C channels.
+static int64_t stage_piece(int64_t C) { return std::max<int64_t>(64, (int64_t)(kStageBytes / 4) / C); }
+
+// Row copies of one piece, split into jobs of at most kCopySegBytes:
+// dst[c * dld + j] = src[c * sld + j] for j < n.
+static void pool_copy_rows(CopyPool& pool, float* dst, int64_t dld, const float* src, int64_t sld, int64_t C,
+                           int64_t n)
+{
+    const int64_t seg = std::max<int64_t>(1, (int64_t)(kCopySegBytes / 4));
+    const int64_t nseg = (n + seg - 1) / seg;
+    pool.run((int)(C * nseg), [&](int j) {
+        const int64_t c = j / nseg, g = j - c * nseg;
+        const int64_t a = g * seg, len = std::min(seg, n - a);
+        std::memcpy(dst + c * dld + a, src + c * sld + a, (size_t)len * 4);
+    });
+}
+
+// Host voltage [C][N] (row stride ld) into the sample-major device copy dst [N][C].
+static int stage_in(fbp_handle* h, const float* V, int64_t ld, int64_t N, float* dst)
+{
+    int rc = ensure_staging(h);
+    if (rc != FBP_OK) return rc;
+    const int64_t C = h->C, P = stage_piece(C);
+    hipStream_t s = h->stream;
+    int64_t k = 0;
+    for (int64_t x0 = 0; x0 < N; x0 += P, k++) {
+        const int64_t np = std::min(P, N - x0);
+        const int b = (int)(k % kStageBufs);
+        if (h->stage_busy[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]), "staging wait");
+        pool_copy_rows(*h->pool, h->pinned[b], np, V + x0, ld, C, np);
+        HIP_TRY(hipMemcpyAsync(h->dstage[b].p, h->pinned[b], (size_t)C * np * 4, hipMemcpyHostToDevice, s),
+                "copy voltage");
+        HIP_TRY(hipEventRecord(h->stage_ev[b], s), "event");
+        h->stage_busy[b] = true;
+        launch_to_sample_major(h->dstage[b].as<float>(), np, dst + x0 * C, C, np, s);
+    }
+    return FBP_OK;
+}
+
+// Sample-major device voltage src [N][C] into the host array out [C][N] (row stride ldo).
+static int stage_out(fbp_handle* h, const float* src, int64_t N, float* out, int64_t ldo)
+{
+    int rc = ensure_staging(h);
+    if (rc != FBP_OK) return rc;
+    const int64_t C = h->C, P = stage_piece(C);
+    const int64_t npieces = (N + P - 1) / P;
+    hipStream_t s = h->stream;
+    auto issue = [&](int64_t k) -> int {
+        const int64_t x0 = k * P, np = std::min(P, N - x0);
+        const int b = (int)(k % kStageBufs);
+        launch_to_channel_major(src + x0 * C, h->dstage[b].as<float>(), C, np, s);
+        HIP_TRY(hipMemcpyAsync(h->pinned[b], h->dstage[b].p, (size_t)C * np * 4, hipMemcpyDeviceToHost, s),
+                "copy residual");
+        HIP_TRY(hipEventRecord(h->stage_ev[b], s), "event");
+        h->stage_busy[b] = true;
+        return FBP_OK;
+    };
+    for (int64_t k = 0; k < std::min<int64_t>(npieces, kStageBufs - 1); k++) {
+        rc = issue(k);
+        if (rc != FBP_OK) return rc;
+    }
+    for (int64_t k = 0; k < npieces; k++) {
+        if (k + kStageBufs - 1 < npieces) {
+            rc = issue(k + kStageBufs - 1);
+            if (rc != FBP_OK) return rc;
+        }
+        const int b = (int)(k % kStageBufs);
+        HIP_TRY(hipEventSynchronize(h->stage_ev[b]), "staging wait");
+        const int64_t x0 = k * P, np = std::min(P, N - x0);
+        pool_copy_rows(*h->pool, out + x0, ldo, h->pinned[b], np, C, np);
+    }
+    return FBP_OK;
+}
+
 extern "C" {
 
 int fbp_create(int device, fbp_handle** out)
@@ -564,7 +746,7 @@ int fbp_destroy(fbp_handle* h)
     if (!h) return FBP_OK;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->Vtmp, &h->LIK, &h->wmax_v, &h->wmax_p,
+    DevBuf* bufs[] = {&h->T, &h->rows, &h->nch, &h->off, &h->chan, &h->nz, &h->bias, &h->thr, &h->V, &h->LIK, &h->wmax_v, &h->wmax_p,
                       &h->best_idx, &h->best_lab, &h->best_lik, &h->obest_idx, &h->obest_lab, &h->check,
                       &h->check_snap, &h->orc, &h->giveup, &h->has, &h->W, &h->F, &h->cand_idx, &h->cand_lab,
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
@@ -572,6 +754,12 @@ int fbp_destroy(fbp_handle* h)
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty};
     for (DevBuf* b : bufs) b->release();
+    for (int b = 0; b < kStageBufs; b++) {
+        h->dstage[b].release();
+        if (h->pinned[b]) (void)hipHostFree(h->pinned[b]);
+        if (h->stage_ev[b]) (void)hipEventDestroy(h->stage_ev[b]);
+    }
+    h->pool.reset();
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -746,9 +934,10 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         screen_mf = screen_mf && h->mf_ok;
     }
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
-    hipEvent_t ev0, ev1;
-    HIP_TRY(hipEventCreate(&ev0), "event");
-    HIP_TRY(hipEventCreate(&ev1), "event");
+    EventPair evp;  // destroyed on every return path
+    HIP_TRY(hipEventCreate(&evp.a), "event");
+    HIP_TRY(hipEventCreate(&evp.b), "event");
+    hipEvent_t ev0 = evp.a, ev1 = evp.b;
     HIP_TRY(hipEventRecord(ev0, s), "event");
 
     PassState st = pass_view(h);
@@ -756,9 +945,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     if (v_on_device) {
         launch_to_sample_major(Vsrc, ld, st.V, C, N, s);
     } else {
-        HIP_TRY(h->Vtmp.ensure((size_t)C * N * 4), "alloc voltage staging");
-        HIP_TRY(hipMemcpy2DAsync(h->Vtmp.p, N * 4, Vsrc, ld * 4, N * 4, C, hipMemcpyHostToDevice, s), "copy voltage");
-        launch_to_sample_major(h->Vtmp.as<float>(), N, st.V, C, N, s);
+        int rc = stage_in(h, Vsrc, ld, N, st.V);
+        if (rc != FBP_OK) return rc;
     }
     if (mixed) launch_absmax(st.V, (int64_t)C * N, h->mf_mbits.as<uint32_t>(), h->mf_vs.as<float>(), s);
     // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
@@ -813,9 +1001,11 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         // Both kernels leave exact values wherever a decision can depend on them.
         if (mixed && dirty == nullptr) {
             const MfTables mt = mf_view(h);
-            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch)
+            for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
                 launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(),
                                dirty, w0, std::min(lik_batch, nW - w0), s);
+                stats.n_likelihood_mfma_launches++;
+            }
         } else {
             launch_likelihood(ts, st, nW, dirty, s);
         }
@@ -907,8 +1097,6 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(hipEventSynchronize(ev1), "sync");
     float ms = 0;
     hipEventElapsedTime(&ms, ev0, ev1);
-    hipEventDestroy(ev0);
-    hipEventDestroy(ev1);
     stats.ms_total = ms;
     stats.ms_likelihood = tm.total(TM_LIK);
     stats.ms_window_max = tm.total(TM_MAX);
@@ -973,10 +1161,9 @@ int fbp_copy_residual(fbp_handle* h, float* out)
     if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
     if (!out) return set_err(h, FBP_E_INVALID, "null output");
     HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
-    HIP_TRY(h->Vtmp.ensure((size_t)h->C * h->N * 4), "alloc voltage staging");
-    launch_to_channel_major(h->V.as<float>(), h->Vtmp.as<float>(), h->C, h->N, h->stream);
+    int rc = stage_out(h, h->V.as<float>(), h->N, out, h->N);
+    if (rc != FBP_OK) return rc;
     HIP_TRY(hipGetLastError(), "residual layout");
-    HIP_TRY(hipMemcpyAsync(out, h->Vtmp.p, (size_t)h->C * h->N * 4, hipMemcpyDeviceToHost, h->stream), "copy");
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     return FBP_OK;
 }

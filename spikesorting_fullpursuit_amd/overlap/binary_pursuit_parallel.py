@@ -36,12 +36,51 @@ def release_device_buffers():
     _handles.clear()
 
 
+# Limits of libfbp_hip.so's buffers (csrc/fbp_api.hip) used by library_device_bytes.
+_MAX_GRAM_BYTES = 8 << 30
+_MAX_SCREEN_BUFFER_BYTES = 2 << 30
+_STAGING_BYTES = 3 * (32 << 20)
+# Bytes per window besides the per-template maxima: best index / label / value and
+# the overlap copies (5 x 4), five flag bytes, four window lists (4 x 4), the
+# dirty-channel mask (8) and the spike list with room to grow (4 x 8).
+_PER_WINDOW_BYTES = 20 + 5 + 16 + 8 + 32
+
+
+def library_device_bytes(n_chans, n_samples, n_templates, template_samples_per_chan, n_max_shift_inds,
+                         shared_pairs=None):
+    """Device memory libfbp_hip.so allocates for one chunk of n_samples:
+    (constant bytes, bytes per sample).  shared_pairs is the number of ordered
+    template pairs that share a channel (U^2 when None: every pair)."""
+    C, U, L = int(n_chans), int(n_templates), int(template_samples_per_chan)
+    S = 2 * int(n_max_shift_inds) + 1
+    pairs = U * U if shared_pairs is None else max(1, int(shared_pairs))
+    gram = pairs * S * (((S + 3) // 4) * 4) * 4 + pairs * S * 4 + pairs * S * (-(-S // 8)) * 4
+    if pairs * S * S * 4 > _MAX_GRAM_BYTES:
+        gram = 0  # the literal recheck runs instead
+    screen = 0
+    if U * S * 4 <= _MAX_SCREEN_BUFFER_BYTES:
+        screen = min(65536, max(1, _MAX_SCREEN_BUFFER_BYTES // (U * S * 4))) * U * S * 4 + U * S * 4
+    templates = 3 * U * C * L * 4
+    constant = gram + screen + templates + _STAGING_BYTES
+    per_sample = 4.0 * C + 4.0 * U + (6.0 * U + _PER_WINDOW_BYTES) / L
+    return constant, per_sample
+
+
 def chunk_plan(n_chans, n_samples, sampling_rate, n_templates, template_samples_per_chan,
-               n_max_shift_inds, gpu_total_memory, max_gpu_memory=None, template_bytes=None):
+               n_max_shift_inds, gpu_total_memory, max_gpu_memory=None, template_bytes=None,
+               resident_bytes=0, shared_pairs=None):
     """Chunk length and onsets, binary_pursuit_parallel.py:387-507.
 
     Returns (num_indices_per_chunk, chunk_onsets).  Raises ValueError when a
     chunk would hold fewer than 4 template widths (:472-475).
+
+    The reference's formula gives the chunk length wherever that chunk fits
+    this library's own buffers (library_device_bytes) beside `resident_bytes`
+    of caller data already on the device (a voltage tensor passed in place)
+    with 1 GiB to spare; otherwise the chunk is shortened to the longest that
+    fits.  Its cost model counts buffers this library does not have (the
+    overlap kernels' per-work-group partials) and misses ones it has (the
+    screen tables and buffer), so on long recordings the two differ.
     """
     gpu_memory_size = float(gpu_total_memory)
     gpu_memory_size = max(gpu_memory_size * 0.5, gpu_memory_size - (1024 * 1024 * 1024))
@@ -62,6 +101,10 @@ def chunk_plan(n_chans, n_samples, sampling_rate, n_templates, template_samples_
     max_addressable_seconds = ((1 << DEVICE_ADDRESS_BITS) - 1) / (4.0 * sampling_rate * n_chans)
     num_seconds_per_chunk = min(num_seconds_per_chunk, np.floor(max_addressable_seconds))
     num_indices_per_chunk = int(np.floor(num_seconds_per_chunk * sampling_rate))
+    constant, per_sample = library_device_bytes(n_chans, n_samples, n_templates, L, n_max_shift_inds,
+                                                shared_pairs)
+    fit = (float(gpu_total_memory) - (1 << 30) - float(resident_bytes) - constant) / per_sample
+    num_indices_per_chunk = min(num_indices_per_chunk, int(np.floor(max(fit, 0.0))))
     if num_indices_per_chunk < 4 * L:
         raise ValueError(
             "Cannot fit enough data on GPU to run binary pursuit. Decrease template samples or get a new GPU.")
@@ -124,9 +167,19 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
         n_max_shift_inds = template_samples_per_chan // 2
     n_max_shift_inds = np.uint32(n_max_shift_inds)
 
+    on_device = _is_device_tensor(voltage)
+    resident = 0
+    if on_device:
+        import torch
+        resident = voltage.numel() * voltage.element_size()
+        if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
+            resident += voltage.numel() * 4  # the float32 copy made below
+    nz = np.any(templates.reshape(n_templates, n_chans, template_samples_per_chan) != 0, axis=2)
+    shared_pairs = int(np.count_nonzero((nz.astype(np.int64) @ nz.T.astype(np.int64)) > 0))
     num_indices_per_chunk, chunk_onsets = chunk_plan(
         n_chans, n_samples, sampling_rate, n_templates, template_samples_per_chan, n_max_shift_inds,
-        h.device_memory(), max_gpu_memory, template_bytes=templates.nbytes)
+        h.device_memory(), max_gpu_memory, template_bytes=templates.nbytes, resident_bytes=resident,
+        shared_pairs=shared_pairs)
 
     T3 = templates.reshape(n_templates, n_chans, template_samples_per_chan)
     h.set_templates(T3, np.atleast_1d(template_sum_squared), np.atleast_1d(neuron_lower_thresholds))
@@ -142,9 +195,7 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     else:
         flags |= _native.FLAG_SCREEN_FP32
 
-    on_device = _is_device_tensor(voltage)
     if on_device:
-        import torch
         if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
             voltage = voltage.to(torch.float32).contiguous()
         # the library works on its own stream: the producer's writes must be done
@@ -158,7 +209,12 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
             h.run(None, n_max_shift_inds, flags, ld=voltage.stride(0), n_samples=stop_index - start_index,
                   device_ptr=voltage.data_ptr() + 4 * start_index)
         else:
-            chunk_voltage = np.float32(voltage[:, start_index:stop_index])
+            # float32 rows are read in place (the library stages them through pinned
+            # pieces); other types are cast per chunk as binary_pursuit_parallel.py:524-527
+            chunk_voltage = voltage[:, start_index:stop_index]
+            if (not isinstance(chunk_voltage, np.ndarray) or chunk_voltage.dtype != np.float32
+                    or chunk_voltage.strides[1] != 4 or chunk_voltage.strides[0] % 4 != 0):
+                chunk_voltage = np.float32(chunk_voltage)
             h.run(chunk_voltage, n_max_shift_inds, flags)
         idx, lab = h.result()
         if stats is not None:

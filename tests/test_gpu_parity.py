@@ -387,3 +387,44 @@ def test_dropin_no_spikes_returns_empty_lists():
                              n_max_shift_inds=d["n_max_shift_inds"])
     assert out == ([], [], [], [])
     assert np.all(m["n_spikes"] == 0)
+
+
+@pytest.mark.parametrize("name,dur,offset", [("probe64", 10.5, 7), ("small", 150.0, 3)])
+def test_host_input_staged_in_pieces_matches_device_input(handle, name, dur, offset):
+    """Host voltage moves to the device in 32 MiB pinned pieces and the residual comes back the
+    same way: a float32 row view with row stride above N (no host copy) spanning several pieces,
+    the last one partial, must give the spikes and residual of the same voltage passed as a
+    device tensor."""
+    import torch
+
+    d = synthetic.generate(name, duration_s=dur)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    C, N = d["voltage"].shape
+    wide = np.zeros((C, N + offset + 5), np.float32)
+    wide[:, offset:offset + N] = d["voltage"]
+    view = wide[:, offset:offset + N]
+    assert view.strides[0] > 4 * N and 4 * C * N > 2 * (32 << 20)
+    flags = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_PRECISION_MIXED
+    handle.set_templates(T, bias, thr)
+    print(f"[staging] {name} C={C} N={N} host view, then device tensor", flush=True)
+    handle.run(view, n, flags)
+    hi, hl = handle.result()
+    hR = handle.residual(N)
+    Vd = torch.tensor(d["voltage"], device="cuda")
+    torch.cuda.synchronize()
+    handle.run(None, n, flags, ld=Vd.stride(0), n_samples=N, device_ptr=Vd.data_ptr())
+    di, dl = handle.result()
+    dR = handle.residual(N)
+    assert len(hi) > 1000
+    np.testing.assert_array_equal(hi, di)
+    np.testing.assert_array_equal(hl, dl)
+    np.testing.assert_array_equal(hR, dR)
+    np.testing.assert_array_equal(wide[:, offset:offset + N], d["voltage"])  # the caller's array is untouched
+    # the residual is the voltage minus every found template
+    ref = d["voltage"].astype(np.float64).copy()
+    L = T.shape[2]
+    for p, u in zip(hi.tolist(), hl.tolist()):
+        m = min(L, N - p)
+        ref[:, p:p + m] -= T[u, :, :m]
+    assert np.max(np.abs(hR - ref)) <= 1e-3 * max(1.0, np.max(np.abs(ref)))

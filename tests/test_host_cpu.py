@@ -77,6 +77,40 @@ def test_chunk_plan_formula_and_overlap():
     assert onsets[-1] < 1_200_000 - L <= onsets[-1] + n - 3 * L
 
 
+def _fbp_run_device_bytes(C, N, U, L, nmax):
+    """Every device buffer fbp_run holds for an N-sample chunk with every template pair
+    sharing a channel, listed from csrc/fbp_api.hip: working voltage, likelihood,
+    per-template window maxima (f32 + u16), 5 per-window u32/f32 arrays, 5 flag
+    arrays, 4 window lists, the dirty masks, spikes with twice the windows'
+    capacity after growth, screen tables and buffer, staging, templates."""
+    S = 2 * nmax + 1
+    nwin = -(-N // L)
+    pairs = U * U
+    b = 4 * C * N + 4 * U * N + 6 * U * nwin + 20 * nwin + 5 * nwin + 16 * nwin + 8 * nwin + 2 * 2 * 4 * nwin
+    b += pairs * S * ((S + 3) // 4 * 4) * 4 + pairs * S * 4 + pairs * S * (-(-S // 8)) * 4 + U * S * 4
+    b += min(65536, (2 << 30) // (U * S * 4)) * U * S * 4
+    b += 3 * (32 << 20) + 3 * U * C * L * 4
+    return b
+
+
+@pytest.mark.parametrize("hours,on_device", [(1 / 6, False), (3, False), (10, False), (40, False), (1 / 6, True),
+                                             (3, True), (6, True)])
+def test_chunk_plan_fits_library_buffers_probe64(hours, on_device):
+    """64 channels, 30 kHz, 120 units, L = 48 on a 288 GB device: every chunk the plan makes
+    fits fbp_run's buffers beside a voltage tensor passed in place; short recordings keep
+    the reference's single chunk."""
+    C, fs, U, L, nmax = 64, 30000.0, 120, 48, 47
+    total = 288 * 10**9
+    N = int(hours * 3600 * fs)
+    resident = 4 * C * N if on_device else 0
+    n, onsets = bpp.chunk_plan(C, N, fs, U, L, nmax, total, resident_bytes=resident)
+    assert _fbp_run_device_bytes(C, n, U, L, nmax) + resident <= total - (1 << 30)
+    assert onsets[0] == 0 and all(b - a == n - 3 * L for a, b in zip(onsets, onsets[1:]))
+    assert onsets[-1] + n >= N - L
+    if hours <= 1:
+        assert onsets == [0]
+
+
 def test_chunk_plan_too_small_raises():
     with pytest.raises(ValueError):
         bpp.chunk_plan(4, 100000, 40000.0, 5, 64, 63, 288 * 2**30, max_gpu_memory=4096)

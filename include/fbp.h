@@ -42,6 +42,8 @@ extern "C" {
 #define FBP_FLAG_PRECISION_MIXED 32u /* likelihood by fp16 GEMM + exact FP32 rescoring near and above zero (same spikes) */
 #define FBP_FLAG_SELECT_STREAMING 64u /* overlap screen selection without register-resident pair state (taken anyway when U*(2n+1) > 24576) */
 #define FBP_FLAG_LIKELIHOOD_MFMA 128u /* with FBP_FLAG_PRECISION_MIXED: the matrix-core first pass even where the tiles' channel padding predicts the FP32 kernel faster (same spikes either way) */
+#define FBP_FLAG_PRECISION_BF16 256u /* likelihood of every pass by BF16 matrix-core products (one template per tile, its own channels) + exact FP32 rescoring near and above zero (same spikes) */
+#define FBP_FLAG_PRECISION_FP8 512u  /* the same with FP8 (e4m3) operands */
 
 typedef struct fbp_handle fbp_handle;
 

@@ -26,6 +26,8 @@ FLAG_SCREEN_FP32 = 16
 FLAG_PRECISION_MIXED = 32
 FLAG_SELECT_STREAMING = 64
 FLAG_LIKELIHOOD_MFMA = 128
+FLAG_PRECISION_BF16 = 256
+FLAG_PRECISION_FP8 = 512
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "libfbp_hip.so")
-SOURCES = ["fbp_kernels.hip", "fbp_screen.hip", "fbp_likmfma.hip", "fbp_api.hip", "fbp_refine.hip", "fbp_clips.hip", "fbp_inputs.hip", "fbp_preproc.hip", "fbp_qa.hip"]
+SOURCES = ["fbp_kernels.hip", "fbp_screen.hip", "fbp_likmfma.hip", "fbp_likmma.hip", "fbp_api.hip", "fbp_refine.hip", "fbp_clips.hip", "fbp_inputs.hip", "fbp_preproc.hip", "fbp_qa.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

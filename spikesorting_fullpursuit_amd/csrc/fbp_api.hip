@@ -28,7 +28,8 @@ void launch_iota(uint32_t* p, int64_t n, hipStream_t s);
 void launch_to_sample_major(const float* src, int64_t ld, float* dst, int64_t C, int64_t N, hipStream_t s);
 void launch_to_channel_major(const float* src, float* dst, int64_t C, int64_t N, hipStream_t s);
 bool likelihood_fits(const TemplateSet& ts);
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s);
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s,
+                       const uint8_t* gmask = nullptr);
 void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_t* sidx, const uint32_t* slab,
                        uint32_t nsp, hipStream_t s);
 void launch_clear_dirty(const PassState& st, uint32_t nW, hipStream_t s);
@@ -77,6 +78,14 @@ void launch_likmfma(const TemplateSet& ts, const PassState& st, const MfTables& 
                     const float* vs, const uint32_t* mbits, const uint64_t* dirty, uint32_t w0, uint32_t nb,
                     hipStream_t s);
 constexpr int kScreenMfmaMaxS = 128;  // shifts of the matrix-core screen (8 tiles of 16)
+// Per-template BF16 / FP8 matrix-core likelihood (fbp_likmma.hip).
+void launch_frag_convert(const float* src, void* dst, int64_t n8, int fmt, hipStream_t s);
+void launch_quantize(const float* V, int64_t N, int C, const MmaVoltage& mv, int fmt, hipStream_t s);
+void launch_requantize(const float* V, int64_t N, int C, int L, const MmaVoltage& mv, int fmt, const uint32_t* sidx,
+                       uint32_t nsp, hipStream_t s);
+void launch_vscale_to(const uint32_t* mbits, float* vs, int e_target, hipStream_t s);
+void launch_likmma(const TemplateSet& ts, const PassState& st, const MmaTables& mt, const MmaVoltage& mv, int fmt,
+                   uint32_t nW, uint8_t* gmask, const uint64_t* dirty, hipStream_t s);
 void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const void* fragl,
                         const float* Ntr, int S, int n, uint32_t f0, uint32_t nbatch, float* Bbuf, hipStream_t s);
 }  // namespace fbp
@@ -243,6 +252,13 @@ struct fbp_handle {
     float scr_k2 = 0.0f, scr_k3 = 0.0f, scr_k4 = 0.0f;  // screen bound terms of the split-fp16 B
     bool mf_ok = false;
     bool mf_pays = false;  // the matrix-core first pass is expected to beat the FP32 kernel
+    // per-template BF16 / FP8 matrix-core likelihood: tables for mma_fmt (-1: none built)
+    int mma_fmt = -1;
+    int mma_KB = 0;
+    float mma_kl = 0.0f, mma_kb = 0.0f, mma_ka = 0.0f, mma_kd = 0.0f;
+    DevBuf mma_frag, mma_tsc, mma_T1, mma_scal, mma_q, mma_bmax, mma_gmask;
+    std::vector<float> h_rows;
+    std::vector<int> h_off;
     // chunk state
     int64_t N = 0, nwin = 0;
     DevBuf V, LIK, wmax_v, wmax_p, best_idx, best_lab, best_lik, obest_idx, obest_lab;
@@ -577,6 +593,97 @@ static MfTables mf_view(const fbp_handle* h)
     return mt;
 }
 
+// Tables of the per-template BF16 (fmt 0) / FP8 e4m3 (fmt 1) likelihood: B
+// fragments of each template row (the row shifted by n = lane % 16 samples,
+// sample j = 8 KB (lane / 16) + 8 kb + e of k-block kb), each template scaled by a power of two
+// (FP8: its largest |value| into [128, 256); BF16: 1), and the bound constants.
+// Operand x (scaled) enters as x' with |x - x'| <= eps |x| + a (BF16: eps =
+// 2^-8, a = 2^-134, half its smallest sub-normal; FP8 e4m3: eps = 2^-4, a =
+// 2^-10; neither format's sub-normals are flushed by the matrix cores), so a
+// product errs by (2 eps + eps^2) |t v| + a (1 + eps) (|t| + |v|) + a^2; the
+// products are exact in FP32 and summed in FP32 (K = Cmax * 32 KB terms, gamma_K
+// doubled for an unknown order); the exact chain errs by gamma_K2 (|bias| + sum
+// |t v|), K2 = Cmax * L + 1; the bias add rounds once.  All doubled.
+static int build_mma_tables(fbp_handle* h, int fmt)
+{
+    const int U = h->U, L = h->L, Lp = h->Lp;
+    const int KB = (L + 15 + 31) / 32;
+    const int Rr = (int)h->h_rowL1.size();
+    std::vector<float> tsc(U, 1.0f), T1(U, 0.0f);
+    for (int u = 0; u < U; u++) {
+        double a = 0.0;
+        float mx = 0.0f;
+        for (int i = 0; i < h->h_nch[u]; i++)
+            for (int j = 0; j < L; j++) {
+                const float v = h->h_rows[(size_t)(h->h_off[u] + i) * Lp + j];
+                a += std::fabs((double)v);
+                mx = std::max(mx, std::fabs(v));
+            }
+        T1[u] = (float)(a * (1.0 + 1e-6));
+        if (fmt == 1 && mx > 0.0f) {
+            int e = 0;
+            std::frexp(mx, &e);
+            tsc[u] = std::ldexp(1.0f, 8 - e);
+        }
+    }
+    const size_t nfr = (size_t)std::max(Rr, 1) * KB;
+    std::vector<float> vals(nfr * 64 * 8, 0.0f);
+    for (int u = 0; u < U; u++)
+        for (int i = 0; i < h->h_nch[u]; i++) {
+            const int r = h->h_off[u] + i;
+            for (int kb = 0; kb < KB; kb++)
+                for (int lane = 0; lane < 64; lane++)
+                    for (int e = 0; e < 8; e++) {
+                        const int j = 8 * KB * (lane >> 4) + 8 * kb + e - (lane & 15);
+                        const float v = (j >= 0 && j < L) ? h->h_rows[(size_t)r * Lp + j] * tsc[u] : 0.0f;
+                        vals[(((size_t)r * KB + kb) * 64 + lane) * 8 + e] = v;
+                    }
+        }
+    DevBuf tmp;
+    HIP_TRY(tmp.ensure(vals.size() * 4), "alloc fragment staging");
+    HIP_TRY(h->mma_frag.ensure(nfr * 64 * (fmt == 0 ? 16 : 8)), "alloc likelihood fragments");
+    HIP_TRY(h->mma_tsc.ensure((size_t)U * 4), "alloc likelihood tables");
+    HIP_TRY(h->mma_T1.ensure((size_t)U * 4), "alloc likelihood tables");
+    HIP_TRY(h->mma_scal.ensure(16), "alloc likelihood tables");
+    HIP_TRY(hipMemcpy(tmp.p, vals.data(), vals.size() * 4, hipMemcpyHostToDevice), "copy fragments");
+    HIP_TRY(hipMemcpy(h->mma_tsc.p, tsc.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mma_T1.p, T1.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
+    launch_frag_convert(tmp.as<float>(), h->mma_frag.p, (int64_t)(nfr * 64), fmt, h->stream);
+    HIP_TRY(hipGetLastError(), "fragment conversion");
+    HIP_TRY(hipStreamSynchronize(h->stream), "fragment conversion");
+    tmp.release();
+    const double u = std::ldexp(1.0, -24);
+    const double eps = fmt == 0 ? std::ldexp(1.0, -8) : std::ldexp(1.0, -4);
+    const double a = fmt == 0 ? std::ldexp(1.0, -134) : std::ldexp(1.0, -10);
+    const double K = (double)std::max(h->Cmax, 1) * 32.0 * KB, K2 = (double)std::max(h->Cmax, 1) * L + 1.0;
+    const double gK = K * u / (1.0 - K * u), gK2 = K2 * u / (1.0 - K2 * u);
+    h->mma_kl = (float)(2.0 * ((2.0 * eps + eps * eps) + 2.0 * gK * (1.0 + eps) * (1.0 + eps) + gK2 +
+                               std::ldexp(1.0, -23)));
+    h->mma_kb = (float)(2.0 * (gK2 + std::ldexp(1.0, -24)));
+    h->mma_ka = (float)(2.0 * a * (1.0 + eps) * (1.0 + 2.0 * gK));
+    h->mma_kd = (float)(2.0 * K * a * a * (1.0 + 2.0 * gK));
+    h->mma_KB = KB;
+    h->mma_fmt = fmt;
+    return FBP_OK;
+}
+
+static MmaTables mma_view(const fbp_handle* h)
+{
+    MmaTables mt;
+    mt.frag = h->mma_frag.p;
+    mt.tsc = h->mma_tsc.as<float>();
+    mt.T1 = h->mma_T1.as<float>();
+    mt.rowL1 = h->rowL1.as<float>();
+    mt.KB = h->mma_KB;
+    mt.R = (h->L + 15) / 16;
+    mt.P = 16 / mt.R;
+    mt.kl = h->mma_kl;
+    mt.kb = h->mma_kb;
+    mt.ka = h->mma_ka;
+    mt.kd = h->mma_kd;
+    return mt;
+}
+
 // Builds the voltage-independent tables of the screened recheck for shift
 // range n.  Returns false (literal recheck) when they do not fit the limits.
 static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
@@ -752,7 +859,8 @@ int fbp_destroy(fbp_handle* h)
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
-                      &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty};
+                      &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
         h->dstage[b].release();
@@ -841,6 +949,9 @@ int fbp_set_templates(fbp_handle* h, const float* templates, int32_t U, int32_t 
     h->Lp = Lp;
     h->h_nch = nch;
     h->h_nz = nz;
+    h->h_rows = rows;
+    h->h_off = off;
+    h->mma_fmt = -1;
     h->Cmax = 0;
     for (int u = 0; u < U; u++) h->Cmax = std::max(h->Cmax, nch[u]);
     h->h_rowL1.assign(chan.size(), 0.0f);
@@ -933,6 +1044,17 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         mixed = mixed && h->mf_ok && (h->mf_pays || (flags & FBP_FLAG_LIKELIHOOD_MFMA) != 0);
         screen_mf = screen_mf && h->mf_ok;
     }
+    // per-template BF16 / FP8 matrix-core likelihood on every pass; a window's
+    // 8-position groups to rescore fit one byte (L <= 64), else the FP32 kernel runs
+    const int mma_req = (flags & FBP_FLAG_PRECISION_FP8) ? 1 : ((flags & FBP_FLAG_PRECISION_BF16) ? 0 : -1);
+    const bool mma = mma_req >= 0 && L <= 64;
+    if (mma) {
+        mixed = false;
+        if (h->mma_fmt != mma_req) {
+            int rc = build_mma_tables(h, mma_req);
+            if (rc != FBP_OK) return rc;
+        }
+    }
     Timer tm(s, (flags & FBP_FLAG_TIMING) != 0);
     EventPair evp;  // destroyed on every return path
     HIP_TRY(hipEventCreate(&evp.a), "event");
@@ -949,6 +1071,29 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         if (rc != FBP_OK) return rc;
     }
     if (mixed) launch_absmax(st.V, (int64_t)C * N, h->mf_mbits.as<uint32_t>(), h->mf_vs.as<float>(), s);
+    MmaVoltage mv{};
+    if (mma) {
+        // channel-major quantized copy of the residual (rows read up to x + 63 + 32 past a window start)
+        const int64_t ldq = ((N + L + 96 + 63) / 64) * 64;
+        HIP_TRY(h->mma_q.ensure((size_t)C * ldq * (mma_req == 0 ? 2 : 1)), "alloc quantized voltage");
+        HIP_TRY(h->mma_bmax.ensure((size_t)C * (ldq / 16) * 4), "alloc block maxima");
+        HIP_TRY(h->mma_gmask.ensure((size_t)nwin * U), "alloc rescore groups");
+        // [0] max |V| bits, [1] unused scale, [2] voltage scale
+        float* scal = h->mma_scal.as<float>();
+        if (mma_req == 1) {
+            // FP8: the largest |V| into [64, 128), room below 448 for what the residual adds
+            launch_absmax(st.V, (int64_t)C * N, reinterpret_cast<uint32_t*>(scal), scal + 1, s);
+            launch_vscale_to(reinterpret_cast<uint32_t*>(scal), scal + 2, 6, s);
+        } else {
+            launch_fill_u32(reinterpret_cast<uint32_t*>(scal + 2), 0x3f800000u, 1, s);  // BF16: 1.0
+        }
+        mv.q = h->mma_q.p;
+        mv.bmax = h->mma_bmax.as<float>();
+        mv.vs = scal + 2;
+        mv.ldq = ldq;
+        mv.nbq = ldq / 16;
+        launch_quantize(st.V, N, (int)C, mv, mma_req, s);
+    }
     // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
     // best_* zeros, flags zeros, window list 0..nwin-1).
     launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
@@ -999,7 +1144,13 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         // few templates per window, which the vector-unit kernel skips one by one
         // (the matrix-core kernel only by 16-template tiles over 4 windows).
         // Both kernels leave exact values wherever a decision can depend on them.
-        if (mixed && dirty == nullptr) {
+        if (mma) {
+            // matrix-core approximation of every computed pair, then the exact FP32
+            // chain on the 8-position groups it could not settle
+            launch_likmma(ts, st, mma_view(h), mv, mma_req, nW, h->mma_gmask.as<uint8_t>(), dirty, s);
+            stats.n_likelihood_mfma_launches++;
+            launch_likelihood(ts, st, nW, dirty, s, h->mma_gmask.as<uint8_t>());
+        } else if (mixed && dirty == nullptr) {
             const MfTables mt = mf_view(h);
             for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
                 launch_likmfma(ts, st, mt, h->rowL1.as<float>(), h->mf_vs.as<float>(), h->mf_mbits.as<uint32_t>(),
@@ -1082,6 +1233,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         tm.begin(TM_RESID);
         launch_residual(ts, st.V, N, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, reach, s);
         launch_mark_dirty(ts, st, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, s);
+        if (mma) launch_requantize(st.V, N, (int)C, (int)L, mv, mma_req, h->sp_idx.as<uint32_t>() + total, nadd, s);
         tm.end(TM_RESID);
         total += nadd;
         nW = h->h_counters[CNT_W];

@@ -84,6 +84,27 @@ struct ScreenTables {
     float kappa3, kappa4; // absolute residue of the fp16 split (0 for the FP32 B)
 };
 
+// Per-template matrix-core likelihood in BF16 or FP8 (fbp_likmma.hip).  delta =
+// kl Q + kb |bias| + ka (T1 / vs + 64 S / ts) + kd / (ts vs) bounds |approximation -
+// exact FP32 chain| on a 16-position row (Q = sum_c rowL1 * max|v|, S = sum_c max|v|).
+struct MmaTables {
+    const void* frag;   // [sum_t nch_t * KB][64 lanes]: 16 bytes (bf16 x 8) or 8 bytes (fp8 x 8) per lane
+    const float* tsc;   // [U] template scale (a power of two)
+    const float* T1;    // [U] sum |T| over the template (rounded up)
+    const float* rowL1; // [R] L1 norm of each compact template row (rounded up)
+    int KB;             // k-blocks of 32 per channel
+    int R;              // rows per window: ceil(L / 16)
+    int P;              // windows per tile: 16 / R
+    float kl, kb, ka, kd;
+};
+
+struct MmaVoltage {
+    void* q;            // [C][ldq] quantized voltage, channel-major (zero beyond N)
+    float* bmax;        // [C][nbq] max |v| over each 16-sample block
+    const float* vs;    // voltage scale (device scalar, a power of two)
+    int64_t ldq, nbq;
+};
+
 enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7,
                CNT_PAIRS = 8, CNT_N = 9 };
 

@@ -26,7 +26,8 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
                                                        float* __restrict__ LIK, float* __restrict__ wmax_v,
                                                        uint16_t* __restrict__ wmax_p,
                                                        const uint64_t* __restrict__ dirty,
-                                                       uint32_t* __restrict__ cnt_pairs)
+                                                       uint32_t* __restrict__ cnt_pairs,
+                                                       const uint8_t* __restrict__ gmask)
 {
     // [wb][C][sv] voltage rows, then per-item maxima [wb][U][ngr] (value, position),
     // then the block's (window, template) pairs to compute [wb][U], then the
@@ -53,10 +54,14 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
         __syncthreads();
         // pairs whose channels changed since the window was last computed
         // (every pair without a dirty map); the others keep LIK and maxima
+        // gmask (exact pass after a matrix-core approximation): only the pairs
+        // with 8-position groups to rescore, and only those groups
         for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
             const int b = pr / U, t = pr - b * U;
+            if (gmask != nullptr && gmask[(size_t)W[w0 + b] * U + t] == 0) continue;
             if (dirty == nullptr) {
                 plist[atomicAdd(&npair_s, 1u)] = (uint32_t)pr;
+                if (gmask != nullptr && C <= 64) atomicOr(&cun_s[b], (unsigned long long)ts.cmask[t]);
             } else {
                 const uint64_t m = ts.cmask[t];
                 if ((m & dirty[W[w0 + b]]) != 0ull) {
@@ -68,20 +73,22 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
         __syncthreads();
         const int npair = (int)npair_s;
         if (npair == 0) continue;  // uniform over the block
-        if (threadIdx.x == 0) atomicAdd(cnt_pairs, (uint32_t)npair);
+        if (threadIdx.x == 0 && cnt_pairs != nullptr) atomicAdd(cnt_pairs, (uint32_t)npair);
         // channels to stage per window: those of its computed templates; with C a
         // multiple of 4, the channel quads that hold any of them (clist then lists quads)
         const bool quads = C % 4 == 0;
         if ((int)threadIdx.x < nb) {
             const int b = threadIdx.x;
             int n = 0;
-            const uint64_t m = dirty == nullptr ? ~0ull : cun_s[b];  // dirty == nullptr whenever C > 64
+            // channels of the block's computed templates (every channel without a mask)
+            const bool all = dirty == nullptr && (gmask == nullptr || C > 64);
+            const uint64_t m = all ? ~0ull : cun_s[b];
             if (quads) {
                 for (int q = 0; q < C / 4; q++)
-                    if (dirty == nullptr || ((m >> (4 * q)) & 0xFull) != 0ull) clist[b * C + n++] = q;
+                    if (all || ((m >> (4 * q)) & 0xFull) != 0ull) clist[b * C + n++] = q;
             } else {
                 for (int c = 0; c < C; c++)
-                    if (dirty == nullptr || ((m >> c) & 1ull) != 0ull) clist[b * C + n++] = c;
+                    if (all || ((m >> c) & 1ull) != 0ull) clist[b * C + n++] = c;
             }
             ncl_s[b] = n;
         }
@@ -149,6 +156,11 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
             const int j = it / ngr, g = it - j * ngr;
             const int pr = (int)plist[j];
             const int b = pr / U, t = pr - b * U;
+            if (gmask != nullptr && ((gmask[(size_t)W[w0 + b] * U + t] >> g) & 1) == 0) {
+                pv[it] = -INFINITY;  // its approximations are negative: no window maximum there
+                pp[it] = 0xffffu;
+                continue;
+            }
             const int p0 = g * kPos;
             const int64_t x0 = (int64_t)W[w0 + b] * L + p0;
             float acc[kPos];
@@ -738,7 +750,8 @@ bool likelihood_fits(const TemplateSet& ts)
     return wb * likelihood_lds_per_window(ts, sv) <= 64 * 1024;
 }
 
-void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s)
+void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, const uint64_t* dirty, hipStream_t s,
+                       const uint8_t* gmask)
 {
     if (nW == 0) return;
     int wb, sv;
@@ -751,16 +764,16 @@ void launch_likelihood(const TemplateSet& ts, const PassState& st, uint32_t nW, 
     const uint32_t* W = st.W;
     if (ts.L == 48)
         hipLaunchKernelGGL(k_likelihood<48>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
-                           dirty, st.counters + CNT_PAIRS);
+                           dirty, gmask ? nullptr : st.counters + CNT_PAIRS, gmask);
     else if (ts.L == 64)
         hipLaunchKernelGGL(k_likelihood<64>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
-                           dirty, st.counters + CNT_PAIRS);
+                           dirty, gmask ? nullptr : st.counters + CNT_PAIRS, gmask);
     else if (ts.L == 32)
         hipLaunchKernelGGL(k_likelihood<32>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
-                           dirty, st.counters + CNT_PAIRS);
+                           dirty, gmask ? nullptr : st.counters + CNT_PAIRS, gmask);
     else
         hipLaunchKernelGGL(k_likelihood<0>, g, b, lds, s, ts, V, N, W, nW, wb, sv, st.LIK, st.wmax_v, st.wmax_p,
-                           dirty, st.counters + CNT_PAIRS);
+                           dirty, gmask ? nullptr : st.counters + CNT_PAIRS, gmask);
 }
 
 void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_t* sidx, const uint32_t* slab,

@@ -62,7 +62,9 @@ def library_device_bytes(n_chans, n_samples, n_templates, template_samples_per_c
         screen = min(65536, max(1, _MAX_SCREEN_BUFFER_BYTES // (U * S * 4))) * U * S * 4 + U * S * 4
     templates = 3 * U * C * L * 4
     constant = gram + screen + templates + _STAGING_BYTES
-    per_sample = 4.0 * C + 4.0 * U + (6.0 * U + _PER_WINDOW_BYTES) / L
+    # BF16 likelihood (the largest of the precisions): the quantized voltage (2 C), its
+    # 16-sample block maxima (C / 4) and one rescore byte per (window, template)
+    per_sample = 4.0 * C + 4.0 * U + (6.0 * U + _PER_WINDOW_BYTES) / L + 2.25 * C + float(U) / L
     return constant, per_sample
 
 
@@ -140,8 +142,8 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     stream and buffers) instead of the cached one of `device`, so that several
     threads can run pursuits concurrently.
     """
-    if precision not in ("mixed", "fp32"):
-        raise ValueError("precision must be 'mixed' or 'fp32'")
+    if precision not in ("mixed", "fp32", "bf16", "fp8"):
+        raise ValueError("precision must be 'mixed', 'fp32', 'bf16' or 'fp8'")
     n_chans = int(voltage.shape[0])
     n_samples = int(voltage.shape[1])
     sampling_rate = sort_info["sampling_rate"]
@@ -192,6 +194,10 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
         flags |= _native.FLAG_RECHECK_LITERAL
     if precision == "mixed":
         flags |= _native.FLAG_PRECISION_MIXED
+    elif precision == "bf16":
+        flags |= _native.FLAG_PRECISION_BF16
+    elif precision == "fp8":
+        flags |= _native.FLAG_PRECISION_FP8
     else:
         flags |= _native.FLAG_SCREEN_FP32
 

@@ -82,11 +82,13 @@ def _fbp_run_device_bytes(C, N, U, L, nmax):
     sharing a channel, listed from csrc/fbp_api.hip: working voltage, likelihood,
     per-template window maxima (f32 + u16), 5 per-window u32/f32 arrays, 5 flag
     arrays, 4 window lists, the dirty masks, spikes with twice the windows'
-    capacity after growth, screen tables and buffer, staging, templates."""
+    capacity after growth, BF16-likelihood buffers, screen tables and buffer, staging, templates."""
     S = 2 * nmax + 1
     nwin = -(-N // L)
     pairs = U * U
     b = 4 * C * N + 4 * U * N + 6 * U * nwin + 20 * nwin + 5 * nwin + 16 * nwin + 8 * nwin + 2 * 2 * 4 * nwin
+    ldq = (N + L + 96 + 63) // 64 * 64
+    b += 2 * C * ldq + 4 * C * (ldq // 16) + U * nwin  # BF16 likelihood: quantized voltage, block maxima, groups
     b += pairs * S * ((S + 3) // 4 * 4) * 4 + pairs * S * 4 + pairs * S * (-(-S // 8)) * 4 + U * S * 4
     b += min(65536, (2 << 30) // (U * S * 4)) * U * S * 4
     b += 3 * (32 << 20) + 3 * U * C * L * 4

@@ -44,6 +44,11 @@ extern "C" {
 #define FBP_FLAG_LIKELIHOOD_MFMA 128u /* with FBP_FLAG_PRECISION_MIXED: the matrix-core first pass even where the tiles' channel padding predicts the FP32 kernel faster (same spikes either way) */
 #define FBP_FLAG_PRECISION_BF16 256u /* likelihood of every pass by BF16 matrix-core products (one template per tile, its own channels) + exact FP32 rescoring near and above zero (same spikes) */
 #define FBP_FLAG_PRECISION_FP8 512u  /* the same with FP8 (e4m3) operands */
+/* Diagnostics (not for sorting): stop after the first likelihood pass (no spikes; the likelihood
+ * of every position stays for fbp_copy_likelihood), and keep every BF16 / FP8 approximation
+ * (no exact rescoring, so decisions are no longer guaranteed to be those of FP32). */
+#define FBP_FLAG_FIRST_PASS_ONLY 1024u
+#define FBP_FLAG_NO_RESCORE 2048u
 
 typedef struct fbp_handle fbp_handle;
 
@@ -108,6 +113,9 @@ int fbp_copy_clips(fbp_handle *h, float *clips);
 /* Residual voltage [C][N] of the last run into a host buffer. */
 int fbp_copy_residual(fbp_handle *h, float *out);
 int fbp_get_stats(fbp_handle *h, fbp_stats *out);
+/* Likelihood [U][N] of the last fbp_run into a host buffer (the values the last pass left:
+ * with FBP_FLAG_FIRST_PASS_ONLY, the first pass over every position). */
+int fbp_copy_likelihood(fbp_handle *h, float *out);
 /* Per-pass trace of the last fbp_run, one row of three int64 per pass:
  * spikes added (num_additional_spikes, binary_pursuit_parallel.py:1210-1217),
  * windows listed (new_window_indices, :1242) and windows flagged for the

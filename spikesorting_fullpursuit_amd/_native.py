@@ -28,12 +28,14 @@ FLAG_SELECT_STREAMING = 64
 FLAG_LIKELIHOOD_MFMA = 128
 FLAG_PRECISION_BF16 = 256
 FLAG_PRECISION_FP8 = 512
+FLAG_FIRST_PASS_ONLY = 1024
+FLAG_NO_RESCORE = 2048
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (
     "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
     "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
-    "fbp_get_stats", "fbp_get_pass_log", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_forward_noalloc", "fbp_refine_backward",
+    "fbp_get_stats", "fbp_copy_likelihood", "fbp_get_pass_log", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_forward_noalloc", "fbp_refine_backward",
     "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
     "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
     "fbp_preproc_workspace_bytes", "fbp_abs_medians", "fbp_count_above", "fbp_channel_cov_rows", "fbp_zca_apply",
@@ -108,6 +110,7 @@ def load():
     L.fbp_copy_residual.argtypes = [vp, vp]
     L.fbp_get_stats.argtypes = [vp, ctypes.POINTER(FbpStats)]
     L.fbp_get_pass_log.argtypes = [vp, vp, i32, ctypes.POINTER(i32)]
+    L.fbp_copy_likelihood.argtypes = [vp, vp]
     L.fbp_refine_workspace_bytes.argtypes = [i64, i64, i32, i32, i32, ctypes.POINTER(i64)]
     L.fbp_refine_forward.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, i64, vp, vp, i64, vp]
     L.fbp_refine_forward_noalloc.argtypes = [vp, i64, i64, i32, vp, i32, i32, vp, vp, vp, i64, vp, i64, vp, vp]
@@ -275,6 +278,12 @@ class Handle:
         s = FbpStats()
         self._check(self._lib.fbp_get_stats(self._h, ctypes.byref(s)))
         return s.as_dict()
+
+    def likelihood(self, n_samples):
+        """[U][N] float32 likelihood the last run left (diagnostics)."""
+        out = np.zeros((self.U, n_samples), np.float32)
+        self._check(self._lib.fbp_copy_likelihood(self._h, out.ctypes.data))
+        return out
 
     def pass_log(self):
         """[passes][3] int64: spikes added, windows listed, windows flagged."""

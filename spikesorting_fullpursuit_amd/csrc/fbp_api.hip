@@ -1149,7 +1149,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             // chain on the 8-position groups it could not settle
             launch_likmma(ts, st, mma_view(h), mv, mma_req, nW, h->mma_gmask.as<uint8_t>(), dirty, s);
             stats.n_likelihood_mfma_launches++;
-            launch_likelihood(ts, st, nW, dirty, s, h->mma_gmask.as<uint8_t>());
+            if (!(flags & FBP_FLAG_NO_RESCORE)) launch_likelihood(ts, st, nW, dirty, s, h->mma_gmask.as<uint8_t>());
         } else if (mixed && dirty == nullptr) {
             const MfTables mt = mf_view(h);
             for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
@@ -1163,6 +1163,10 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         launch_clear_dirty(st, nW, s);
         tm.end(TM_LIK);
         stats.n_likelihood_launches++;
+        if (flags & FBP_FLAG_FIRST_PASS_ONLY) {
+            passes = 1;
+            break;
+        }
         tm.begin(TM_MAX);
         launch_window_max(ts, st, nW, s);
         tm.end(TM_MAX);
@@ -1316,6 +1320,17 @@ int fbp_copy_residual(fbp_handle* h, float* out)
     int rc = stage_out(h, h->V.as<float>(), h->N, out, h->N);
     if (rc != FBP_OK) return rc;
     HIP_TRY(hipGetLastError(), "residual layout");
+    HIP_TRY(hipStreamSynchronize(h->stream), "sync");
+    return FBP_OK;
+}
+
+int fbp_copy_likelihood(fbp_handle* h, float* out)
+{
+    if (!h) return FBP_E_INVALID;
+    if (!h->have_result) return set_err(h, FBP_E_STATE, "no result: call fbp_run first");
+    if (!out) return set_err(h, FBP_E_INVALID, "null output");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    HIP_TRY(hipMemcpyAsync(out, h->LIK.p, (size_t)h->U * h->N * 4, hipMemcpyDeviceToHost, h->stream), "copy");
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     return FBP_OK;
 }

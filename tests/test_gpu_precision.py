@@ -125,3 +125,75 @@ def test_dropin_precision_option(fmt):
                            precision="fp32")
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_array_equal(a[1], b[1])
+
+
+_FP64 = {}
+
+
+def _fp64_case(name, dur, n):
+    """A workload slice and the FP64 NumPy oracle's spikes on it (computed once)."""
+    key = (name, dur, n)
+    if key not in _FP64:
+        from oracle import fbp_fp64
+        d = synthetic.generate(name, duration_s=dur)
+        T, bias, thr = synthetic.kernel_inputs(d)
+        nn = d["n_max_shift_inds"] if n is None else n
+        _FP64[key] = (d["voltage"], T, bias, thr, nn, fbp_fp64.pursuit(d["voltage"], T, bias, thr, nn))
+    return _FP64[key]
+
+
+@pytest.mark.parametrize("fmt", sorted(FMTS))
+@pytest.mark.parametrize("name,dur,n", [("small", 1.0, None), ("probe16", 0.1, 8), ("probe64", 0.05, 3)])
+def test_decisions_agree_with_fp64_oracle(handle, fmt, name, dur, n):
+    """Decision agreement with the independent FP64 restatement (sliding-window products,
+    energy-drop overlap values) on slices it finishes in seconds: the shift range is cut
+    on the 16- and 64-channel slices (its overlap items are materialised)."""
+    V, T, bias, thr, nn, (i64, l64) = _fp64_case(name, dur, n)
+    idx, lab, _ = run_fmt(handle, V, T, bias, thr, nn, fmt)
+    assert len(i64) > 20
+    np.testing.assert_array_equal(idx, i64)
+    np.testing.assert_array_equal(lab, l64)
+
+
+def _first_pass(h, V, T, bias, thr, flags):
+    h.set_templates(T, bias, thr)
+    h.run(V, T.shape[2] - 1, flags | _native.FLAG_FIRST_PASS_ONLY)
+    return h.likelihood(V.shape[1])
+
+
+@pytest.mark.parametrize("name,dur", [("small", 1.0), ("probe64", 0.2)])
+def test_first_pass_likelihood_fp32_equals_oracle(handle, name, dur):
+    d = synthetic.generate(name, duration_s=dur)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    LIK = _first_pass(handle, d["voltage"], T, bias, thr, _native.FLAG_SCREEN_FP32)
+    np.testing.assert_array_equal(LIK, fbp_oracle.full_likelihood(d["voltage"], T, bias))
+
+
+@pytest.mark.parametrize("fmt", sorted(FMTS))
+@pytest.mark.parametrize("name,dur", [("small", 1.0), ("probe16", 0.3), ("probe64", 0.2)])
+def test_first_pass_positive_values_exact(handle, fmt, name, dur):
+    """After the exact rescoring every positive likelihood is the FP32 value bit for bit
+    and every other value is <= 0; without the rescoring the approximations stay
+    within a few units of the format's rounding of sum |t v|."""
+    d = synthetic.generate(name, duration_s=dur)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    V = d["voltage"]
+    exact = fbp_oracle.full_likelihood(V, T, bias)
+    got = _first_pass(handle, V, T, bias, thr, FMTS[fmt])
+    pos = exact > 0
+    np.testing.assert_array_equal(got[pos], exact[pos])
+    assert np.all(got[~pos] <= 0)
+    raw = _first_pass(handle, V, T, bias, thr, FMTS[fmt] | _native.FLAG_NO_RESCORE)
+    eps = {"bf16": 2.0 ** -8, "fp8": 2.0 ** -4}[fmt]
+    U, C, L = T.shape
+    N = V.shape[1]
+    # Q(t, x) = sum_c sum_j |T[t,c,j]| |V[c,x+j]|, the scale of the rounding error
+    Q = np.zeros((U, N))
+    A = np.abs(V.astype(np.float64))
+    for t in range(U):
+        for c in range(C):
+            if np.any(T[t, c]):
+                Q[t, :N - L + 1] += np.correlate(A[c], np.abs(T[t, c].astype(np.float64)), "valid")
+    Q[:, N - L:] = 0.0
+    err = np.abs(raw.astype(np.float64) - exact)
+    assert np.all(err <= 4 * eps * Q + 1e-6 * np.abs(bias.astype(np.float64))[:, None] + 1e-30)

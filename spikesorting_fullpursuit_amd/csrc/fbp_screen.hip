@@ -259,22 +259,41 @@ __device__ __forceinline__ float exact_item_padded(const float* vW, int Lrt, int
         iu += hs;
         const float* vr = vW + c * vs;
         float ss = 0.0f;
-        for (int j0 = 0; j0 < L; j0 += 8) {
-            float a[8], b[8], vv[8];
+        // eight samples at a time, the next eight read while these are consumed
+        float a[8], b[8], vv[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                const int j = (LT > 0 || j0 + q < L) ? j0 + q : 0;
-                a[q] = tf[j];
-                b[q] = tsr[j];
-                vv[q] = vr[j];
+        for (int q = 0; q < 8; q++) {
+            const int j = (LT > 0 || q < L) ? q : 0;
+            a[q] = tf[j];
+            b[q] = tsr[j];
+            vv[q] = vr[j];
+        }
+#pragma unroll
+        for (int j0 = 0; j0 < L; j0 += 8) {
+            float na[8], nb[8], nv[8];
+            if (j0 + 8 < L) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int j = (LT > 0 || j0 + 8 + q < L) ? j0 + 8 + q : 0;
+                    na[q] = tf[j];
+                    nb[q] = tsr[j];
+                    nv[q] = vr[j];
+                }
             }
-            __builtin_amdgcn_sched_barrier(0);  // keep the 24 loads ahead of the chains
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 if (LT == 0 && j0 + q >= L) break;
                 const float sum = a[q] + b[q];
                 ss = fmaf(sum, sum, ss);
                 lik = fmaf(sum, vv[q], lik);
+            }
+            if (j0 + 8 < L) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    a[q] = na[q];
+                    b[q] = nb[q];
+                    vv[q] = nv[q];
+                }
             }
         }
         if (ss > 0.0f) lik = fmaf(-0.5f, ss, lik);
@@ -646,20 +665,26 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 // ---------------------------------------------------------------------------
 // Register-resident selection: the same bounds, candidates and rescoring as
 // k_screen_select, organised so that every (unit, shift) pair of the window is
-// read once.  Thread t of a block of bd threads owns the pairs e = t + bd * i,
-// i < kSelR; it keeps B(u, s) (-inf when b+s-n is out of bounds) and two bit
-// masks (LIK[u][b+s-n] > 0; u shares a channel with the label) in registers
-// through the three phases.  Each wave enumerates the shifts f of its own
-// surviving pairs one pair at a time, so no phase needs a block-wide list.
+// read once.  The block's first KU * S threads own the pairs (KU = ceil(U /
+// kSelR) unit slices): thread t keeps the shift s = t mod S and the units
+// u = t / S + KU * i, i < kSelR, so its shift-indexed bounds are two registers
+// and its pairs' unit-indexed values lie at a fixed LDS stride.  It keeps
+// B(u, s) (-inf when b+s-n is out of bounds or u >= U) and two bit masks
+// (LIK[u][b+s-n] > 0; u shares a channel with the label) in registers through
+// the three phases.  Each wave enumerates the shifts f of its own surviving
+// pairs one pair at a time, so no phase needs a block-wide list.
 // ---------------------------------------------------------------------------
 constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
 constexpr int kQR = 2;              // (pair, f-quad) items per thread per round of G loads
 constexpr int kSelLR = 12;          // pairs per round of the per-pair loads (divides kSelR)
 static_assert(kSelR % kSelLR == 0, "whole rounds");
 static_assert(kSelR <= 32, "pair masks are 32-bit");
-constexpr int kSelMaxBlock = 1024;  // threads per block at most (U * S <= kSelR * kSelMaxBlock)
+constexpr int kSelMaxBlock = 1024;  // threads per block at most (KU * S <= kSelMaxBlock)
 constexpr size_t kSelMaxLds = 96 * 1024;  // dynamic LDS of the register-resident selection at most
 constexpr int kStageRegion = 8192;  // floats of the selection's phase-dependent LDS region at least
+
+// Unit slices of the register-resident selection's pair layout.
+__host__ __device__ inline int sel_slices(int U) { return (U + kSelR - 1) / kSelR; }
 
 __device__ __forceinline__ float block_max_n(float v, float* red, int nw)
 {
@@ -680,12 +705,41 @@ __device__ __forceinline__ int wave_sum_u32(int v)
     return v;
 }
 
-// Advance (u, s) by bd pairs: e -> e + bd with bd = du * S + ds, ds < S.
-__device__ __forceinline__ void pair_step(int& u, int& s, int du, int ds, int S)
+// Advance (q, r) of e = q * D + r by bd = dq * D + dr (dr < D).
+__device__ __forceinline__ void qr_step(int& q, int& r, int dq, int dr, int D)
 {
-    s += ds;
-    u += du;
-    if (s >= S) { s -= S; u++; }
+    r += dr;
+    q += dq;
+    if (r >= D) { r -= D; q++; }
+}
+
+// apply_parse for a window whose best spike (lab, b) and likelihood bl are
+// already at hand: the same decisions (parse_overlap_recheck_indices,
+// binary_pursuit.cl:657-815), with the item decoded in 32 bits and the four
+// values it needs read together.
+__device__ __forceinline__ void apply_parse_at(const TemplateSet& ts, const PassState& st, int n, uint32_t id, float v,
+                                               uint32_t k, float bl, uint32_t lab, int64_t b, int S)
+{
+    if (k == kNoItem || !(v > bl) || v <= 0.0f) { st.orc[id] = 0; return; }
+    const uint32_t U = (uint32_t)ts.U;
+    const uint32_t u = k % U, r = k / U;
+    const int64_t f = (int64_t)(r % (uint32_t)S), s = (int64_t)(r / (uint32_t)S);
+    const int64_t af = b + f - n, as = b + s - n;
+    const float A = st.LIK[(size_t)lab * st.N + af];
+    const float B = st.LIK[(size_t)u * st.N + as];
+    const float tl = ts.thr[lab], tu = ts.thr[u];
+    if (v < A || v < B || A <= 0.0f || B <= 0.0f) { st.orc[id] = 0; return; }
+    if (A >= B && A > tl) {
+        st.best_lik[id] = A; st.obest_lab[id] = lab; st.obest_idx[id] = (uint32_t)af;
+    } else if (B > A && B > tu) {
+        st.best_lik[id] = B; st.obest_lab[id] = u; st.obest_idx[id] = (uint32_t)as;
+    } else if (A > tl) {
+        st.best_lik[id] = A; st.obest_lab[id] = lab; st.obest_idx[id] = (uint32_t)af;
+    } else {
+        st.obest_idx[id] = (uint32_t)b;
+        st.obest_lab[id] = lab;
+        st.orc[id] = 0;
+    }
 }
 
 template <int LT>
@@ -709,9 +763,13 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     const int US = U * S;
     const int64_t N = st.N;
     const int bd = (int)blockDim.x, nw = bd >> 6;
+    const int KU = sel_slices(U);
+    const int UP = KU * kSelR;            // unit slots of the pair layout (>= U)
     // window rows at an odd stride: lanes rescoring different units read
     // different channels at one sample without sharing a bank
     const int VS = L | 1;
+    const int LP = L + 2 * n;             // zero-padded row: P[i] = T[i - n], i - n outside [0, L) -> 0
+    const int RG = tb.sel_region;         // floats of the phase-dependent region
     float* vW = sm;                       // [C][VS]
     float* vmax = vW + C * VS;            // [C]
     float* A = vmax + C;                  // [S]
@@ -722,15 +780,15 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     float* SMa = PMp + S;
     float* SMp = SMa + S;
     float* Q = SMp + S;                   // [U]
-    float* D = Q + U;                     // [U] per-unit bound delta
-    const int LP = L + 2 * n;             // zero-padded row: P[i] = T[i - n], i - n outside [0, L) -> 0
-    const int RG = tb.sel_region;         // floats of the phase-dependent region
-    float* TlP = D + U;                   // [CM][LP] padded rows of the window's label
+    const int dpo = (C * VS + C + 7 * S + U + 1) & ~1;
+    // [UP] (delta(l, u), pair id of (l, u) as bits, -1: no shared channel); slots u >= U hold (0, -1)
+    float2* Dp = reinterpret_cast<float2*>(sm + dpo);
+    float* TlP = sm + dpo + 2 * UP;       // [CM][LP] padded rows of the window's label
     float* zrow = TlP + CM * LP;          // [LP] zeros (a template channel that is absent)
     int* cl = reinterpret_cast<int*>(zrow + LP);  // [CM]
-    int* pidw = cl + CM;                  // [U] pair id of (label, u); -1: no shared channel
-    float* reg = reinterpret_cast<float*>(pidw + U);  // [RG], in turn:
+    float* reg = reinterpret_cast<float*>(cl + CM);  // [RG], in turn:
     float* gmB = reg;                     //   [npl][S] least G over the blocks of f with LIK[l] > 0
+    float* gmA = reg + RG / 2;            //   [npl][S] least G over every f
     uint32_t* surv_e = reinterpret_cast<uint32_t*>(reg);  //   survivor lists: pair ...
     float* surv_b = reg + RG / 2;                          //   ... and its B
     const uint32_t survcap = (uint32_t)(RG / 2);
@@ -743,48 +801,120 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     uint8_t* fin = reinterpret_cast<uint8_t*>(slot_off + U);  // [S] b+f-n in [0, N-L)
     uint8_t* posl = fin + S;              // [S] fin && LIK[l][b+f-n] > 0
     const int wv = threadIdx.x >> 6, lane = lane_id();
-    const int wbase = threadIdx.x & ~63;
     const float invS = 1.0f / (float)S;
-    const int du = bd / S, ds = bd - du * S;
-    int u_t, s_t;  // pair of i = 0
-    divmod_small((int)threadIdx.x, S, invS, u_t, s_t);
     const uint32_t nF = st.counters[CNT_F];
     // one window per block (the grid covers the batch)
     {
         const uint32_t wi = blockIdx.x;
         const uint32_t w = f0 + wi;
         if (wi >= nbatch || w >= nF) return;
+        const int tid = (int)threadIdx.x;
+        const bool owner = tid < KU * S;
+        int u0, s_t;  // the thread's first unit and its shift
+        divmod_small(tid, S, invS, u0, s_t);
+        if (!owner) u0 = 0;
         const uint32_t id = st.F[w];
         const uint32_t lab = st.best_lab[id];
         const int64_t b = st.best_idx[id];
         const float bl = st.best_lik[id];
-        if (threadIdx.x == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
+        if (tid == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
+        for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
         __syncthreads();
+        // a window whose best spike fell below its threshold or too near the
+        // chunk end takes no overlap recheck (binary_pursuit.cl:518-527)
         if (window_skipped(ts, st, id)) {
-            if (threadIdx.x == 0) apply_parse(ts, st, n, id, 0.0f, kNoItem);
+            if (tid == 0) st.orc[id] = 0;
             return;
         }
         const float* Bw = Bbuf + (size_t)wi * US;
-        for (int e = threadIdx.x; e < C * L; e += bd) {
-            const int j = e / C, c = e - j * C;
-            vW[c * VS + j] = st.V[(size_t)(b + j) * C + c];
+        // Staging: every global load below is independent of the others, and
+        // no barrier separates them, so they are in flight together.
+        // per-pair state: B, sign of LIK[u][b+s-n]; loads from clamped (always
+        // valid) addresses, B straight into the pair's register
+        const int64_t as_t = b + s_t - n;
+        const bool sin = owner && as_t >= 0 && as_t < N - L;
+        float bv[kSelR];
+        uint32_t pum = 0;
+        const int us = u0;
+#pragma unroll
+        for (int i0 = 0; i0 < kSelR; i0 += kSelLR) {
+            float lv[kSelLR];
+#pragma unroll
+            for (int j = 0; j < kSelLR; j++) {
+                const int u = us + KU * (i0 + j);
+                const bool ok = sin && u < U;
+                const int uc = ok ? u : 0;
+                lv[j] = st.LIK[(size_t)uc * N + (ok ? as_t : 0)];
+                bv[i0 + j] = Bw[ok ? uc * S + s_t : 0];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // the round's loads go out together
+#pragma unroll
+            for (int j = 0; j < kSelLR; j++) {
+                const int u = us + KU * (i0 + j);
+                const bool ok = sin && u < U;
+                if (!ok) bv[i0 + j] = -INFINITY;
+                pum |= (ok && lv[j] > 0.0f) ? 1u << (i0 + j) : 0u;
+            }
         }
+        // the window's samples, eight per thread in flight, and
+        // vmax[c] = max_j |V[b+j][c]| (non-negative floats order as their bit
+        // patterns; a NaN is skipped as fmaxf skips it)
+        {
+            int j = tid / C, c = tid - j * C;
+            const int dj = bd / C, dc = bd - dj * C;
+            for (int e0 = tid; e0 < C * L; e0 += 8 * bd) {
+                float v[8];
+                int dst[8], cq[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const bool ok = e0 + q * bd < C * L;
+                    dst[q] = ok ? c * VS + j : -1;
+                    cq[q] = c;
+                    v[q] = st.V[ok ? (size_t)(b + j) * C + c : (size_t)b * C];
+                    qr_step(j, c, dj, dc, C);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    if (dst[q] < 0) continue;
+                    vW[dst[q]] = v[q];
+                    const float a = fabsf(v[q]);
+                    if (a == a) atomicMax(reinterpret_cast<unsigned int*>(vmax) + cq[q], __float_as_uint(a));
+                }
+            }
+        }
+        // the label's zero-padded rows, eight per thread in flight
         const int nl = ts.nch[lab];
-        for (int e = threadIdx.x; e < nl * LP; e += bd) {
-            const int k = e / LP, i = e - k * LP;
-            TlP[e] = (i >= n && i < n + L) ? ts.rows[(size_t)(ts.off[lab] + k) * ts.Lp + (i - n)] : 0.0f;
+        const int offl = ts.off[lab];
+        {
+            int k = tid / LP, i = tid - k * LP;
+            const int dk = bd / LP, di = bd - dk * LP;
+            for (int e0 = tid; e0 < nl * LP; e0 += 8 * bd) {
+                float v[8];
+                int dst[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const int e = e0 + q * bd;
+                    const bool in = e < nl * LP && i >= n && i < n + L;
+                    dst[q] = e < nl * LP ? e : -1;
+                    v[q] = ts.rows[in ? (size_t)(offl + k) * ts.Lp + (i - n) : (size_t)offl * ts.Lp];
+                    v[q] = in ? v[q] : 0.0f;
+                    qr_step(k, i, dk, di, LP);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    if (dst[q] >= 0) TlP[dst[q]] = v[q];
+            }
         }
-        for (int i = threadIdx.x; i < LP; i += bd) zrow[i] = 0.0f;
-        for (int k = threadIdx.x; k < nl; k += bd) cl[k] = ts.chan[ts.off[lab] + k];
-        // the staging above, the pair ids, the label's likelihood row and the
-        // per-pair loads below are independent: no barrier between them, so
-        // their global loads are in flight together
-        for (int u = threadIdx.x; u < U; u += bd) {
-            const int p = tb.pid[(size_t)lab * U + u];
-            pidw[u] = p;
+        for (int i = tid; i < LP; i += bd) zrow[i] = 0.0f;
+        for (int k = tid; k < nl; k += bd) cl[k] = ts.chan[offl + k];
+        for (int u = tid; u < UP; u += bd) {
+            const int p = u < U ? tb.pid[(size_t)lab * U + u] : -1;
+            Dp[u] = make_float2(0.0f, __int_as_float(p));
             if (p >= 0) { atomicMin(&pfirst_sh, p); atomicAdd(&npl_sh, 1); }
         }
-        for (int f = threadIdx.x; f < S; f += bd) {
+        for (int f = tid; f < S; f += bd) {
             const int64_t af = b + f - n;
             const bool in = af >= 0 && af < N - L;
             const bool pl = in && st.LIK[(size_t)lab * N + af] > 0.0f;
@@ -793,64 +923,30 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             if (pl) { atomicMin(&plo_sh, f); atomicMax(&phi_sh, f); }
             A[f] = Bw[(int)lab * S + f];
         }
-        // per-pair state: B, sign of LIK[u][b+s-n], shared-channel bit.  Loads
-        // are issued kSelLR pairs at a time from clamped (always valid) addresses,
-        // B straight into the pair's register.
-        float bv[kSelR];
-        uint32_t pum = 0, shm = 0;
-        {
-            int u = u_t, s = s_t, t0 = (int)threadIdx.x;
-            asm volatile("" : "+v"(u), "+v"(s), "+v"(t0));  // recompute the pair walk, do not keep it live
-#pragma unroll
-            for (int i0 = 0; i0 < kSelR; i0 += kSelLR) {
-                float lv[kSelLR];
-                int pw[kSelLR];
-                bool ok[kSelLR];
-#pragma unroll
-                for (int j = 0; j < kSelLR; j++) {
-                    const int e = t0 + bd * (i0 + j);
-                    const int64_t as = b + s - n;
-                    ok[j] = e < US && as >= 0 && as < N - L;
-                    const int uc = ok[j] ? u : 0, ec = ok[j] ? e : 0;
-                    lv[j] = st.LIK[(size_t)uc * N + (ok[j] ? as : 0)];
-                    pw[j] = tb.pid[(size_t)lab * U + uc];  // global: pidw may not be written yet
-                    bv[i0 + j] = Bw[ec];
-                    pair_step(u, s, du, ds, S);
-                }
-                __builtin_amdgcn_sched_barrier(0);  // the round's loads go out together
-#pragma unroll
-                for (int j = 0; j < kSelLR; j++) {
-                    if (!ok[j]) bv[i0 + j] = -INFINITY;
-                    if (ok[j] && lv[j] > 0.0f) pum |= 1u << (i0 + j);
-                    if (ok[j] && pw[j] >= 0) shm |= 1u << (i0 + j);
-                }
-            }
-        }
-        __syncthreads();  // pidw, plo / phi, pfirst / npl are complete
-        // least G of every pair of the label over the blocks of f that hold
-        // the f with LIK[l][b+f-n] > 0 (the whole row when they span > 3 blocks)
+        __syncthreads();  // staging, pair ids, plo / phi, pfirst / npl are complete
+        // Gram minima, the range maxima of A and the per-unit bound terms are
+        // independent of each other: one phase
+        // least G of every pair of the label over every f (gmA) and over the
+        // blocks of f that hold the f with LIK[l][b+f-n] > 0 (gmB; the whole
+        // row when they span > 3 blocks, unset when there is no such f)
         {
             const int plo = plo_sh, phi = phi_sh, pf = pfirst_sh, np = npl_sh;
-            if (phi >= 0) {
-                const int j1 = plo / kGB, j3 = phi / kGB, j2 = min(j1 + 1, j3);
-                const bool wide = j3 - j1 > 2;
+            const int j1 = max(plo, 0) / kGB, j3 = max(phi, 0) / kGB, j2 = min(j1 + 1, j3);
+            const bool wide = j3 - j1 > 2, some = phi >= 0;
 #pragma unroll 4
-                for (int k = threadIdx.x; k < np * S; k += bd) {
-                    const size_t row = (size_t)pf * S + k;
+            for (int k = tid; k < np * S; k += bd) {
+                const size_t row = (size_t)pf * S + k;
+                const float ga = tb.minG[row];
+                gmA[k] = ga;
+                if (some) {
                     if (wide) {
-                        gmB[k] = tb.minG[row];
+                        gmB[k] = ga;
                     } else {
                         const float* g = tb.minGb + row * tb.nbg;
                         gmB[k] = fminf(fminf(g[j1], g[j2]), g[j3]);
                     }
                 }
             }
-        }
-        __syncthreads();
-        for (int c = threadIdx.x; c < C; c += bd) {
-            float m = 0.0f;
-            for (int j = 0; j < L; j++) m = fmaxf(m, fabsf(vW[c * VS + j]));
-            vmax[c] = m;
         }
         // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
         if (n <= L) {
@@ -879,14 +975,8 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     cp = __shfl(vp, 63, 64);
                 }
             }
-            __syncthreads();
-            for (int s = threadIdx.x; s < S; s += bd) {
-                const int fl = max(0, s - L), fh = min(S - 1, s + L);
-                RAall[s] = (fl == 0) ? PMa[fh] : SMa[fl];
-                RApos[s] = (fl == 0) ? PMp[fh] : SMp[fl];
-            }
         } else {
-            for (int s = threadIdx.x; s < S; s += bd) {
+            for (int s = tid; s < S; s += bd) {
                 float ma = -INFINITY, mp = -INFINITY;
                 const int fl = max(0, s - L), fh = min(S - 1, s + L);
                 for (int f = fl; f <= fh; f++) {
@@ -897,42 +987,76 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 RApos[s] = mp;
             }
         }
-        __syncthreads();
-        for (int u = threadIdx.x; u < U; u += bd) {
-            float q = 0.0f, t1 = 0.0f, vsum = 0.0f;
-            const int r0 = ts.off[u], nr = ts.nch[u];
-            for (int k = 0; k < nr; k++) {
-                const float vm = vmax[ts.chan[r0 + k]];
-                q += tb.rowL1[r0 + k] * vm;
-                t1 += tb.rowL1[r0 + k];
-                vsum += vm;
+        // the window maximum over every channel, in every wave
+        {
+            float mw = 0.0f;
+            for (int c = lane; c < C; c += 64) mw = fmaxf(mw, vmax[c]);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) mw = fmaxf(mw, __shfl_xor(mw, off, 64));
+            for (int u = tid; u < U; u += bd) {
+                float q = 0.0f, t1 = 0.0f, vsum = 0.0f;
+                const int r0 = ts.off[u], nr = ts.nch[u];
+                // eight rows at a time: their channel and L1 loads go out together
+                for (int k0 = 0; k0 < nr; k0 += 8) {
+                    int ch[8];
+                    float l1[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        const int r = r0 + min(k0 + j, nr - 1);
+                        ch[j] = ts.chan[r];
+                        l1[j] = tb.rowL1[r];
+                    }
+#pragma unroll
+                    for (int j = 0; j < 8; j++) {
+                        if (k0 + j < nr) {
+                            const float vm = vmax[ch[j]];
+                            q += l1[j] * vm;
+                            t1 += l1[j];
+                            vsum += vm;
+                        }
+                    }
+                }
+                float z = 0.0f;
+                if (tb.kappa3 != 0.0f)  // fp16 operands: absolute residue of the scaled split
+                    z = tb.kappa3 * mw * t1 + tb.kappa4 * vsum;
+                Q[u] = (tb.kappa + tb.kappa2) * q + 2.0f * tb.kappa * tb.SS[u] + z;
             }
-            float z = 0.0f;
-            if (tb.kappa3 != 0.0f) {  // fp16 operands: absolute residue of the scaled split
-                float mw = 0.0f;
-                for (int c = 0; c < C; c++) mw = fmaxf(mw, vmax[c]);
-                z = tb.kappa3 * mw * t1 + tb.kappa4 * vsum;
-            }
-            Q[u] = (tb.kappa + tb.kappa2) * q + 2.0f * tb.kappa * tb.SS[u] + z;
         }
         __syncthreads();
+        if (n <= L) {
+            for (int s = tid; s < S; s += bd) {
+                const int fl = max(0, s - L), fh = min(S - 1, s + L);
+                RAall[s] = (fl == 0) ? PMa[fh] : SMa[fl];
+                RApos[s] = (fl == 0) ? PMp[fh] : SMp[fl];
+            }
+        }
         // delta(l, u) = P_l + P_u: the bound is a sum of one term per template
-        for (int u = threadIdx.x; u < U; u += bd) D[u] = Q[lab] + Q[u];
+        {
+            const float ql = Q[lab];
+            for (int u = tid; u < U; u += bd) Dp[u].x = ql + Q[u];
+        }
         __syncthreads();
+        const float rall = RAall[s_t], rpos = RApos[s_t];
+        // pair i of the thread reads dpu[KU * i]; every phase recomputes these
+        // addresses from an opaque copy of u0 rather than keeping 24 of them live
+        auto pair_base = [&]() {
+            int ub = u0;
+            asm volatile("" : "+v"(ub));
+            return Dp + ub;
+        };
         // phase 1: pairs sharing no channel, E-hat = A(f) + B(u, s) separable;
         // M' = max(E-hat - delta_u) is a lower bound of the exact maximum
         float mns = -INFINITY;
-        {
-            int u = u_t, s = s_t;
-            asm volatile("" : "+v"(u), "+v"(s));  // recompute the pair walk, do not keep it live
+        uint32_t shm = 0;
+        const float2* dpu = pair_base();
 #pragma unroll
-            for (int i = 0; i < kSelR; i++) {
-                if (bv[i] != -INFINITY && !((shm >> i) & 1u)) {
-                    const float ra = ((pum >> i) & 1u) ? RAall[s] : RApos[s];
-                    if (ra != -INFINITY) mns = fmaxf(mns, (ra + bv[i]) - D[u]);
-                }
-                pair_step(u, s, du, ds, S);
-            }
+        for (int i = 0; i < kSelR; i++) {
+            const float2 dp = dpu[KU * i];
+            const bool sh = __float_as_int(dp.y) >= 0;
+            shm |= sh ? 1u << i : 0u;
+            const float ra = ((pum >> i) & 1u) ? rall : rpos;
+            const bool ok = bv[i] != -INFINITY && !sh && ra != -INFINITY;
+            mns = fmaxf(mns, ok ? (ra + bv[i]) - dp.x : -INFINITY);
         }
         const float Mns = block_max_n(mns, red, nw);
         // phase 2: pairs sharing a channel whose bound reaches
@@ -944,21 +1068,18 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         uint32_t sv = 0;
         {
             const int pf = pfirst_sh;
-            int u = u_t, s = s_t;
-            asm volatile("" : "+v"(u), "+v"(s));  // recompute the pair walk, do not keep it live
+            const float2* dpu = pair_base();
 #pragma unroll
             for (int i = 0; i < kSelR; i++) {
-                if ((shm >> i) & 1u) {
-                    const bool pu = (pum >> i) & 1u;
-                    const float ra = pu ? RAall[s] : RApos[s];
-                    if (ra != -INFINITY) {
-                        const int k = (pidw[u] - pf) * S + s;
-                        const float gmin = pu ? tb.minG[(size_t)pf * S + k] : gmB[k];
-                        const float tp = fmaxf(Mns - D[u], bl - D[u]);
-                        if (!((ra + bv[i]) - gmin < tp)) sv |= 1u << i;
-                    }
-                }
-                pair_step(u, s, du, ds, S);
+                const float2 dp = dpu[KU * i];
+                const int pw = __float_as_int(dp.y);
+                const bool pu = (pum >> i) & 1u;
+                const float ra = pu ? rall : rpos;
+                const int kk = pw >= 0 ? (pw - pf) * S + s_t : 0;  // row of (label, u) in the tables
+                const float gm = (pu ? gmA : gmB)[kk];
+                const float tp = fmaxf(Mns - dp.x, bl - dp.x);
+                const bool hit = pw >= 0 && ra != -INFINITY && !((ra + bv[i]) - gm < tp);
+                sv |= hit ? 1u << i : 0u;
             }
         }
         float msh = -INFINITY;
@@ -988,14 +1109,14 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         for (int ph = 2; ph <= 3; ph++) {
             const bool shared = ph == 2;
             const float Mref = shared ? Mns : fmaxf(Mns, Mp_sh);
-            auto thr_u = [&](int u) { return fmaxf(Mref - D[u], bl - D[u]); };
+            auto thr_u = [&](int u) { return fmaxf(Mref - Dp[u].x, bl - Dp[u].x); };
             auto gterm = [&](int u, int s, int f) -> float {
-                return shared ? tb.G[((size_t)pidw[u] * S + s) * tb.sg + f] : 0.0f;
+                return shared ? tb.G[((size_t)__float_as_int(Dp[u].y) * S + s) * tb.sg + f] : 0.0f;
             };
             auto eval_pl = [&](int e, float Bu) {
                 int u, s;
                 divmod_small(e, S, invS, u, s);
-                const float Du = D[u];
+                const float Du = Dp[u].x;
                 const float tp = thr_u(u);
                 const int fa = max(plo, s - L), fb = min(phi, s + L);
                 for (int f0 = fa; f0 <= fb; f0 += 4) {
@@ -1024,31 +1145,32 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 const float g = gterm(u, s, f);
                 if (!fin[f] || abs(f - s) > L) return;
                 const float eh = shared ? (A[f] + Bu) - g : A[f] + Bu;
-                if (shared) msh = fmaxf(msh, eh - D[u]);
+                if (shared) msh = fmaxf(msh, eh - Dp[u].x);
                 if (!(eh < thr_u(u))) push(eh, u, f, s);
             };
             if (!shared) {
                 // pairs sharing no channel whose exact best over f reaches theta_u
-                int u = u_t, s = s_t;
                 uint32_t pm = pum, sm2 = shm;
-                // recompute the pair walk and masks here, do not keep them live
-                asm volatile("" : "+v"(u), "+v"(s), "+v"(pm), "+v"(sm2));
+                asm volatile("" : "+v"(pm), "+v"(sm2));  // do not keep per-pair selections live across the phases
+                const float2* dpu = pair_base();
 #pragma unroll
                 for (int i = 0; i < kSelR; i++) {
-                    if (bv[i] != -INFINITY && !((sm2 >> i) & 1u)) {
-                        const float ra = ((pm >> i) & 1u) ? RAall[s] : RApos[s];
-                        if (!(ra == -INFINITY || ra + bv[i] < thr_u(u))) sv |= 1u << i;
-                    }
-                    pair_step(u, s, du, ds, S);
+                    const float2 dp = dpu[KU * i];
+                    const float ra = ((pm >> i) & 1u) ? rall : rpos;
+                    const float th = fmaxf(Mref - dp.x, bl - dp.x);
+                    const bool hit = bv[i] != -INFINITY && !((sm2 >> i) & 1u) &&
+                                     !(ra == -INFINITY || ra + bv[i] < th);
+                    sv |= hit ? 1u << i : 0u;
                 }
                 const int c = wave_sum_u32(__popc(sv));
                 if (lane == 0 && c) atomicAdd(st.counters + CNT_NSP, (uint32_t)c);
             }
-            if (threadIdx.x == 0) { nsurv_s = 0; nsurvp_s = 0; }
+            if (tid == 0) { nsurv_s = 0; nsurvp_s = 0; }
             __syncthreads();
             uint32_t spill = 0;
             uint32_t pm = pum;
-            asm volatile("" : "+v"(pm));  // do not keep per-pair selections live across the phases
+            int ub = u0;
+            asm volatile("" : "+v"(pm), "+v"(ub));  // do not keep per-pair selections live across the phases
 #pragma unroll
             for (int i = 0; i < kSelR; i++) {
                 if ((sv >> i) & 1u) {
@@ -1056,7 +1178,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                     const uint32_t slot = atomicAdd(pu ? &nsurvp_s : &nsurv_s, 1u);
                     if (slot < (pu ? capP : capN)) {
                         const uint32_t o = pu ? capN + slot : slot;
-                        surv_e[o] = threadIdx.x + (uint32_t)bd * i;
+                        surv_e[o] = (uint32_t)((ub + KU * i) * S + s_t);
                         surv_b[o] = bv[i];
                     } else {
                         spill |= 1u << i;
@@ -1078,7 +1200,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                 const int nq = (fhi - fo) / 4 + 1;
                 const uint32_t nI = np * (uint32_t)nq;
                 const float invq = 1.0f / (float)nq;
-                for (uint32_t j0 = threadIdx.x; j0 < nI; j0 += (uint32_t)kQR * (uint32_t)bd) {
+                for (uint32_t j0 = tid; j0 < nI; j0 += (uint32_t)kQR * (uint32_t)bd) {
                     int uq[kQR], sq[kQR], fq[kQR];
                     float bq[kQR];
                     float4 gq[kQR];
@@ -1096,7 +1218,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                         gq[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         if (okq[q] && shared)
                             gq[q] = *reinterpret_cast<const float4*>(
-                                tb.G + ((size_t)pidw[uq[q]] * S + sq[q]) * tb.sg + fq[q]);
+                                tb.G + ((size_t)__float_as_int(Dp[uq[q]].y) * S + sq[q]) * tb.sg + fq[q]);
                     }
                     __builtin_amdgcn_sched_barrier(0);  // the quads' G loads are in flight together
 #pragma unroll
@@ -1109,7 +1231,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
                             const int f = fq[q] + r;
                             if (f < flo || f > fhi || !(plm ? posl[f] : fin[f]) || abs(f - s) > L) continue;
                             const float eh = shared ? (A[f] + bq[q]) - gv[r] : A[f] + bq[q];
-                            if (shared) msh = fmaxf(msh, eh - D[u]);
+                            if (shared) msh = fmaxf(msh, eh - Dp[u].x);
                             if (!(eh < thr_u(u))) push(eh, u, f, s);
                         }
                     }
@@ -1121,9 +1243,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             run_items(capN, nP, 0, S - 1, false);
             for (uint32_t m = spill; m; m &= m - 1) {
                 const int i = __builtin_ctz(m);
-                const int e = (int)threadIdx.x + bd * i;
-                int u, s;
-                divmod_small(e, S, invS, u, s);
+                const int e = (u0 + KU * i) * S + s_t;
                 const float Bu = Bw[e];
                 if ((pum >> i) & 1u) {
                     for (int f = 0; f < S; f++) eval_all(e, Bu, f);
@@ -1133,7 +1253,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             }
             if (shared) {
                 const float m2 = block_max_n(msh, red, nw);
-                if (threadIdx.x == 0) Mp_sh = m2;
+                if (tid == 0) Mp_sh = m2;
                 __syncthreads();
             }
         }
@@ -1144,15 +1264,16 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         uint32_t bk = kNoItem;
         if (__syncthreads_or(overflow)) {
             // more candidates than the list holds: enumerate and rescore in place
-            for (int e = threadIdx.x; e < US; e += bd) {
+            for (int e = tid; e < US; e += bd) {
                 int u, s;
                 divmod_small(e, S, invS, u, s);
                 const int64_t as = b + s - n;
                 if (as < 0 || as >= N - L) continue;
                 const float Bu = Bw[e];
-                const int p = pidw[u];
+                const float2 dp = Dp[u];
+                const int p = __float_as_int(dp.y);
                 const bool pu = st.LIK[(size_t)u * N + as] > 0.0f;
-                const float theta = fmaxf(Mp - D[u], bl - D[u]);
+                const float theta = fmaxf(Mp - dp.x, bl - dp.x);
                 const float* Gr = p >= 0 ? tb.G + ((size_t)p * S + s) * tb.sg : nullptr;
                 for (int f = 0; f < S; f++) {
                     if (!fin[f] || abs(f - s) > L || !(posl[f] || pu)) continue;
@@ -1166,12 +1287,13 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             // candidates reaching theta_u, each rescored exactly by one thread;
             // the zero-padded rows of their units are staged in LDS, upr units
             // per round
-            for (int u = threadIdx.x; u < U; u += bd) slot_of[u] = -1;
-            if (threadIdx.x == 0) nrow_s = 0;
+            for (int u = tid; u < U; u += bd) slot_of[u] = -1;
+            if (tid == 0) nrow_s = 0;
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < nc; i += bd) {
+            for (uint32_t i = tid; i < nc; i += bd) {
                 const int u = (int)(cand_k[i] % (uint32_t)U);
-                if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                const float du = Dp[u].x;
+                if (cand_e[i] < fmaxf(Mp - du, bl - du)) continue;
                 if (atomicCAS(&slot_of[u], -1, -2) == -1) {
                     const int o = (int)atomicAdd(&nrow_s, 1u);
                     slot_nr[o] = ts.nch[u];
@@ -1182,44 +1304,47 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             __syncthreads();
             const int nun = (int)nrow_s;
             const int upr = max(1, srows / CM);
-            const int per = CM * LP;  // floats per staged unit
+            const float invCM = 1.0f / (float)CM;
             for (int r0u = 0; r0u < nun; r0u += upr) {
                 if (r0u > 0) __syncthreads();  // the previous round's chains are done
-                // the round's units staged by the whole block, eight loads in flight per thread
-                const int tot = min(upr, nun - r0u) * per;
-                for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * bd) {
+                // the round's units staged by the whole block, eight loads in flight per
+                // thread: element e = rr * LP + i of row rr = o * CM + k (unit o, row k)
+                const int tot = min(upr, nun - r0u) * CM * LP;
+                int rr = tid / LP, ii = tid - rr * LP;
+                const int drr = bd / LP, dii = bd - drr * LP;
+                for (int e0 = tid; e0 < tot; e0 += 8 * bd) {
                     float v[8];
                     int dst[8];
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         const int e = e0 + q * bd;
-                        v[q] = 0.0f;
-                        dst[q] = -1;
-                        if (e < tot) {
-                            const int o = e / per, rem = e - o * per;
-                            const int k = rem / LP, i = rem - k * LP;
-                            if (k < slot_nr[r0u + o]) {
-                                dst[q] = e;
-                                if (i >= n && i < n + L)
-                                    v[q] = ts.rows[(size_t)(slot_off[r0u + o] + k) * ts.Lp + (i - n)];
-                            }
-                        }
+                        int o, k;
+                        divmod_small(rr, CM, invCM, o, k);
+                        const bool row = e < tot && k < slot_nr[r0u + min(o, upr - 1)];
+                        const bool in = row && ii >= n && ii < n + L;
+                        dst[q] = row ? e : -1;
+                        v[q] = ts.rows[in ? (size_t)(slot_off[r0u + o] + k) * ts.Lp + (ii - n) : 0];
+                        v[q] = in ? v[q] : 0.0f;
+                        qr_step(rr, ii, drr, dii, LP);
                     }
                     __builtin_amdgcn_sched_barrier(0);  // the eight loads are in flight together
 #pragma unroll
                     for (int q = 0; q < 8; q++)
                         if (dst[q] >= 0) stgP[dst[q]] = v[q];
                 }
-                for (int e = threadIdx.x; e < min(upr, nun - r0u) * CM; e += bd) {
-                    const int o = e / CM, k = e - o * CM;
+                for (int e = tid; e < min(upr, nun - r0u) * CM; e += bd) {
+                    int o, k;
+                    divmod_small(e, CM, invCM, o, k);
                     if (k < slot_nr[r0u + o]) stg_ch[e] = ts.chan[slot_off[r0u + o] + k];
                 }
                 __syncthreads();
-                for (uint32_t i = threadIdx.x; i < nc; i += bd) {
+                // candidates packed into the first waves: a wave runs the longest chain of its lanes
+                for (uint32_t i = tid; i < nc; i += bd) {
                     const uint32_t k = cand_k[i];
                     const int u = (int)(k % (uint32_t)U);
                     const int f = (int)((k / (uint32_t)U) % (uint32_t)S), s = (int)(k / ((uint32_t)U * S));
-                    if (cand_e[i] < fmaxf(Mp - D[u], bl - D[u])) continue;
+                    const float du = Dp[u].x;
+                    if (cand_e[i] < fmaxf(Mp - du, bl - du)) continue;
                     const int o = slot_of[u];
                     if (o < r0u || o >= r0u + upr) continue;
                     const int rb = (o - r0u) * CM;
@@ -1233,11 +1358,11 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
         __syncthreads();
         if (lane == 0) { red[wv] = bv_best; redk[wv] = bk; }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             float v = red[0];
             uint32_t k = redk[0];
             for (int i = 1; i < nw; i++) argmax_merge(v, k, red[i], redk[i]);
-            apply_parse(ts, st, n, id, v, k);
+            apply_parse_at(ts, st, n, id, v, k, bl, lab, b, S);
             atomicAdd(st.counters + CNT_EVAL, ncand_s);
             if (ncand_s > (uint32_t)kCandCap) atomicAdd(st.counters + CNT_CAND, 1u);
         }
@@ -1278,12 +1403,13 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax)
            2 * (size_t)S + 16;
 }
 
-// Floats of the selection's phase-dependent region: the blocked Gram minima of
-// one label, the survivor lists, or the staged padded rows.
+// Floats of the selection's phase-dependent region: the two tables of Gram
+// minima of one label (one in each half), the survivor lists, or the staged
+// padded rows.
 int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max)
 {
     const int LP = ts.L + S - 1;
-    int rg = std::max(npl_max * S, kStageRegion);
+    int rg = std::max(2 * npl_max * S, kStageRegion);  // both tables of Gram minima
     rg = std::max(rg, cmax * LP);  // one unit's rows at least
     return (rg + 3) & ~3;
 }
@@ -1292,7 +1418,9 @@ size_t screen_select_reg_lds(const TemplateSet& ts, int S, int cmax, int npl_max
 {
     const int LP = ts.L + S - 1;
     const int rg = screen_select_region(ts, S, cmax, npl_max);
-    return (size_t)(ts.C * (ts.L | 1) + ts.C + 7 * S + 6 * ts.U + cmax * LP + LP + cmax + (size_t)rg + rg / LP) * 4 +
+    const size_t dpo = ((size_t)ts.C * (ts.L | 1) + ts.C + 7 * (size_t)S + ts.U + 1) & ~(size_t)1;
+    const size_t up = (size_t)sel_slices(ts.U) * kSelR;
+    return (dpo + 2 * up + (size_t)cmax * LP + LP + cmax + (size_t)rg + rg / LP + 3 * (size_t)ts.U) * 4 +
            2 * (size_t)S + 16;
 }
 
@@ -1305,31 +1433,28 @@ static void allow_lds(K kernel, size_t bytes)
                                   (int)bytes);
 }
 
+// The register-resident selection: one block per flagged window of the batch.
+template <class K>
+static void launch_select_reg(K kernel, int bd, size_t lds, uint32_t nbatch, hipStream_t s, const TemplateSet& ts,
+                              const PassState& st, const ScreenTables& tb, uint32_t f0, const float* Bbuf)
+{
+    allow_lds(kernel, lds);
+    hipLaunchKernelGGL(kernel, dim3(nbatch), dim3(bd), lds, s, ts, st, tb, f0, nbatch, Bbuf);
+}
 
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
                           uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s)
 {
     if (nbatch == 0) return;
-    const int64_t US = (int64_t)ts.U * tb.S;
-    if (!streaming && US <= (int64_t)kSelR * kSelMaxBlock &&
+    const int64_t owners = (int64_t)sel_slices(ts.U) * tb.S;  // threads owning pairs
+    if (!streaming && owners <= kSelMaxBlock &&
         screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max) <= kSelMaxLds) {
-        const int64_t need = (US + kSelR - 1) / kSelR;
-        const int bd = (int)std::max<int64_t>(kBlock, (need + 63) / 64 * 64);
-        const dim3 g(nbatch), b(bd);
+        const int bd = (int)std::max<int64_t>(kBlock, (owners + 63) / 64 * 64);
         const size_t lds = screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max);
-        if (ts.L == 48) {
-            allow_lds(k_screen_select_reg<48>, lds);
-            hipLaunchKernelGGL(k_screen_select_reg<48>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
-        } else if (ts.L == 64) {
-            allow_lds(k_screen_select_reg<64>, lds);
-            hipLaunchKernelGGL(k_screen_select_reg<64>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
-        } else if (ts.L == 32) {
-            allow_lds(k_screen_select_reg<32>, lds);
-            hipLaunchKernelGGL(k_screen_select_reg<32>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
-        } else {
-            allow_lds(k_screen_select_reg<0>, lds);
-            hipLaunchKernelGGL(k_screen_select_reg<0>, g, b, lds, s, ts, st, tb, f0, nbatch, Bbuf);
-        }
+        if (ts.L == 48) launch_select_reg(k_screen_select_reg<48>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
+        else if (ts.L == 64) launch_select_reg(k_screen_select_reg<64>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
+        else if (ts.L == 32) launch_select_reg(k_screen_select_reg<32>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
+        else launch_select_reg(k_screen_select_reg<0>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
         return;
     }
     const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);

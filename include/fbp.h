@@ -104,6 +104,22 @@ int fbp_set_templates(fbp_handle *h, const float *templates, int32_t U, int32_t 
 int fbp_run(fbp_handle *h, const float *V, int64_t ld, int64_t N, int32_t v_on_device,
             uint32_t n_max_shift_inds, uint32_t flags);
 
+/* Batched pursuit: the pass loops of nseg segments run together, each kernel
+ * launched once per pass over every segment still converging, with two host
+ * reads of all segments' counters per pass.  Segment g is the chunk V[g]
+ * (row stride ld[g], N[g] samples; device pointers when v_on_device != 0)
+ * pursued with the templates set on hs[g] and n_max_shift[g]; its spikes,
+ * residual, statistics and pass log are read back through hs[g] exactly as
+ * after fbp_run, and equal fbp_run's on the same segment (the likelihood and
+ * the screened overlap recheck run in FP32; precision flags are ignored).
+ * The handles are distinct, on one device, with one template length and at
+ * most 64 channels each; launches go on hs[0]'s stream.
+ * Replaces, for many independent segments, the per-chunk loop of
+ * binary_pursuit_parallel.py:508-1500 called once per segment
+ * (spikesorting_parallel.py:2161-2184). */
+int fbp_run_batch(fbp_handle *const *hs, int32_t nseg, const float *const *V, const int64_t *ld, const int64_t *N,
+                  int32_t v_on_device, const uint32_t *n_max_shift, uint32_t flags);
+
 /* Results of the last fbp_run: spike onsets within the chunk and labels in
  * pass order (ascending window within a pass). */
 int fbp_result_count(fbp_handle *h, int64_t *n);

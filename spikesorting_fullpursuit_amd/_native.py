@@ -34,7 +34,7 @@ FLAG_NO_RESCORE = 2048
 # Every symbol include/fbp.h declares.
 EXPORTS = (
     "fbp_create", "fbp_destroy", "fbp_last_error", "fbp_device_memory", "fbp_set_templates",
-    "fbp_run", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
+    "fbp_run", "fbp_run_batch", "fbp_result_count", "fbp_copy_result", "fbp_copy_clips", "fbp_copy_residual",
     "fbp_get_stats", "fbp_copy_likelihood", "fbp_get_pass_log", "fbp_refine_workspace_bytes", "fbp_refine_forward", "fbp_refine_forward_noalloc", "fbp_refine_backward",
     "fbp_refine_last_error", "fbp_get_clips", "fbp_clips_last_error", "fbp_covariance_workspace_bytes",
     "fbp_channel_covariance", "fbp_robust_template", "fbp_template_variances", "fbp_inputs_last_error",
@@ -104,6 +104,7 @@ def load():
     L.fbp_device_memory.argtypes = [vp, ctypes.POINTER(i64)]
     L.fbp_set_templates.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.fbp_run.argtypes = [vp, vp, i64, i64, i32, u32, u32]
+    L.fbp_run_batch.argtypes = [vp, i32, vp, vp, vp, i32, vp, u32]
     L.fbp_result_count.argtypes = [vp, ctypes.POINTER(i64)]
     L.fbp_copy_result.argtypes = [vp, vp, vp]
     L.fbp_copy_clips.argtypes = [vp, vp]
@@ -292,3 +293,37 @@ class Handle:
         rows = np.zeros((max(n.value, 1), 3), np.int64)
         self._check(self._lib.fbp_get_pass_log(self._h, rows.ctypes.data, n.value, ctypes.byref(n)))
         return rows[: n.value]
+
+
+def run_batch(handles, voltages, n_max_shift, flags):
+    """fbp_run_batch: pursue segment g (voltages[g], host float32 [C][N] arrays,
+    or (device_ptr, ld, n_samples) tuples) with handles[g]'s templates; read the
+    results back through each handle."""
+    n = len(handles)
+    if n == 0:
+        return
+    if len(voltages) != n:
+        raise ValueError("one voltage per handle")
+    lib = handles[0]._lib
+    hs = (ctypes.c_void_p * n)(*[h._h.value for h in handles])
+    ptrs = (ctypes.c_void_p * n)()
+    lds = (ctypes.c_int64 * n)()
+    ns = (ctypes.c_int64 * n)()
+    nms = np.ascontiguousarray(np.broadcast_to(np.asarray(n_max_shift, np.uint32), (n,)))
+    keep = []
+    on_device = isinstance(voltages[0], tuple)
+    for g, (h, V) in enumerate(zip(handles, voltages)):
+        if isinstance(V, tuple) != on_device:
+            raise ValueError("voltages must be all host arrays or all device pointers")
+        if on_device:
+            ptr, ld, nsmp = V
+            ptrs[g], lds[g], ns[g] = int(ptr), int(ld), int(nsmp)
+        else:
+            V = np.asarray(V)
+            if V.dtype != np.float32 or V.strides[1] != 4:
+                V = np.ascontiguousarray(V, dtype=np.float32)
+            if V.shape[0] != h.C:
+                raise ValueError(f"segment {g}: voltage channels do not match templates")
+            keep.append(V)
+            ptrs[g], lds[g], ns[g] = V.ctypes.data, V.strides[0] // 4, V.shape[1]
+    handles[0]._check(lib.fbp_run_batch(hs, n, ptrs, lds, ns, 1 if on_device else 0, nms.ctypes.data, int(flags)))

@@ -49,6 +49,16 @@ void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, con
                            uint64_t nsp, float* clips, hipStream_t s);
 void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t npairs, hipStream_t s);
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
+void seg_likelihood_lds(const TemplateSet& ts, int* wb, int* sv, size_t* lds);
+void launch_seg_pass_kernel(int kind, int L, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
+                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s);
+void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
+                        uint32_t nblocks, uint16_t* eob, hipStream_t s);
+void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s);
+bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* bd, size_t* lds);
+void launch_seg_screen(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks_corr,
+                       uint16_t* eob_corr, size_t lds_corr, uint32_t nblocks_sel, uint16_t* eob_sel, int bd,
+                       size_t lds_sel, float* Bbuf, size_t bstride, hipStream_t s);
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -769,12 +779,13 @@ static void pool_copy_rows(CopyPool& pool, float* dst, int64_t dld, const float*
     });
 }
 
-// Host voltage [C][N] (row stride ld) into the sample-major device copy dst [N][C].
-static int stage_in(fbp_handle* h, const float* V, int64_t ld, int64_t N, float* dst)
+// Host voltage [C][N] (row stride ld) into the sample-major device copy dst [N][C],
+// through h's staging buffers on h's stream (C: h's channel count unless given).
+static int stage_in(fbp_handle* h, const float* V, int64_t ld, int64_t N, float* dst, int64_t Cg = -1)
 {
     int rc = ensure_staging(h);
     if (rc != FBP_OK) return rc;
-    const int64_t C = h->C, P = stage_piece(C);
+    const int64_t C = Cg > 0 ? Cg : h->C, P = stage_piece(C);
     hipStream_t s = h->stream;
     int64_t k = 0;
     for (int64_t x0 = 0; x0 < N; x0 += P, k++) {
@@ -824,6 +835,146 @@ static int stage_out(fbp_handle* h, const float* src, int64_t N, float* out, int
         pool_copy_rows(*h->pool, out + x0, ldo, h->pinned[b], np, C, np);
     }
     return FBP_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched pursuit over many segments (fbp_run_batch).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSegBlockCap = 1024;  // blocks per segment and launch at most (bodies grid-stride)
+
+// Allocations, sample-major voltage and initial state of one segment, as the
+// FP32 path of fbp_run sets them up (binary_pursuit_parallel.py:528-655), on
+// the segment's own stream (synchronized before return).
+static int batch_prepare(fbp_handle* h, fbp_handle* stager, const float* Vsrc, int64_t ld, int64_t N,
+                         int32_t v_on_device, uint32_t n_max_shift, bool recheck)
+{
+    if (!h->have_templates) return set_err(h, FBP_E_STATE, "fbp_set_templates must be called first");
+    if (!Vsrc || N <= 0 || ld < N) return set_err(h, FBP_E_INVALID, "voltage must be non-empty with ld >= N");
+    const int64_t L = h->L, C = h->C, U = h->U;
+    if (N >= (int64_t)0xffffffffLL) return set_err(h, FBP_E_UNSUPPORTED, "segment longer than 2^32-1 samples");
+    if (C > 64) return set_err(h, FBP_E_UNSUPPORTED, "batched segments hold at most 64 channels");
+    const uint64_t S = 2ull * n_max_shift + 1ull;
+    if (recheck && S * S * (uint64_t)U >= 0xffffffffull)
+        return set_err(h, FBP_E_UNSUPPORTED, "(2*n_max_shift_inds+1)^2 * n_templates must be < 2^32-1");
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    TemplateSet ts = template_view(h);
+    if (!likelihood_fits(ts))
+        return set_err(h, FBP_E_UNSUPPORTED, "channels x template length exceed the likelihood tile (64 KiB LDS)");
+    const int64_t nwin = (N + L - 1) / L;
+    h->N = N;
+    h->nwin = nwin;
+    h->have_result = false;
+    h->have_clips = false;
+    HIP_TRY(h->V.ensure((size_t)C * N * 4), "alloc voltage");
+    HIP_TRY(h->LIK.ensure((size_t)U * N * 4), "alloc likelihood");
+    HIP_TRY(h->wmax_v.ensure((size_t)U * nwin * 4), "alloc window maxima");
+    HIP_TRY(h->wmax_p.ensure((size_t)U * nwin * 2), "alloc window maxima");
+    HIP_TRY(h->best_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->best_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->best_lik.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->obest_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->obest_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->check.ensure(nwin), "alloc");
+    HIP_TRY(h->check_snap.ensure(nwin), "alloc");
+    HIP_TRY(h->orc.ensure(nwin), "alloc");
+    HIP_TRY(h->giveup.ensure(nwin), "alloc");
+    HIP_TRY(h->has.ensure(nwin), "alloc");
+    HIP_TRY(h->W.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->F.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->cand_idx.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->cand_lab.ensure(nwin * 4), "alloc");
+    HIP_TRY(h->counters.ensure(CNT_N * 4), "alloc");
+    HIP_TRY(h->blockbuf.ensure(((nwin + kCompactTile - 1) / kCompactTile + 1) * 4), "alloc");
+    HIP_TRY(h->dirty.ensure(nwin * 8), "alloc");
+    HIP_TRY(grow_keep(h->sp_idx, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
+    HIP_TRY(grow_keep(h->sp_lab, (size_t)std::max<int64_t>(nwin, 1024) * 4, 0, s), "alloc spikes");
+    if (recheck && h->tab_n != (int64_t)n_max_shift) {
+        bool ok = false;
+        int rc = build_screen_tables(h, n_max_shift, &ok);
+        if (rc != FBP_OK) return rc;
+    }
+    if (recheck && !h->screen_ok)
+        return set_err(h, FBP_E_UNSUPPORTED, "the screened recheck's tables do not fit this segment");
+    PassState st = pass_view(h);
+    if (v_on_device) {
+        launch_to_sample_major(Vsrc, ld, st.V, C, N, s);
+    } else {
+        // one set of pinned staging buffers (the stager's) serves every segment
+        int rc = stage_in(stager, Vsrc, ld, N, st.V, C);
+        if (rc != FBP_OK) return rc;
+        if (stager != h) HIP_TRY(hipStreamSynchronize(stager->stream), "sync");
+    }
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
+    launch_fill_u32(st.best_idx, 0u, nwin, s);
+    launch_fill_u32(st.best_lab, 0u, nwin, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.best_lik), 0u, nwin, s);
+    launch_fill_u32(st.obest_idx, 0u, nwin, s);
+    launch_fill_u32(st.obest_lab, 0u, nwin, s);
+    launch_fill_u8(st.check, 0, nwin, s);
+    launch_fill_u8(st.orc, 0, nwin, s);
+    launch_fill_u8(st.giveup, 0, nwin, s);
+    launch_fill_u32(st.counters, 0u, CNT_N, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.dirty), 0u, 2 * nwin, s);
+    launch_iota(st.W, nwin, s);
+    HIP_TRY(hipStreamSynchronize(s), "sync");
+    return FBP_OK;
+}
+
+// Launch tables of one upload, built on the host in pinned memory and copied
+// to the device in one transfer.
+struct SegTables {
+    SegLaunch* host = nullptr;  // pinned
+    SegLaunch* dev = nullptr;
+    size_t cap = 0, used = 0;
+    // entries of one launch start at the returned offset
+    size_t begin() const { return used; }
+};
+
+struct BatchBufs {
+    std::vector<SegDesc> desc;
+    SegDesc* d_desc = nullptr;
+    SegDyn* h_dyn = nullptr;  // pinned
+    SegDyn* d_dyn = nullptr;
+    SegTables tabs[3];
+    uint16_t* eob[2] = {};
+    uint32_t* d_cnt = nullptr;
+    uint32_t* h_cnt = nullptr;  // pinned
+    float* Bbuf = nullptr;
+    ~BatchBufs()
+    {
+        if (d_desc) (void)hipFree(d_desc);
+        if (h_dyn) (void)hipHostFree(h_dyn);
+        if (d_dyn) (void)hipFree(d_dyn);
+        for (auto& t : tabs) {
+            if (t.host) (void)hipHostFree(t.host);
+            if (t.dev) (void)hipFree(t.dev);
+        }
+        for (auto& e : eob)
+            if (e) (void)hipFree(e);
+        if (d_cnt) (void)hipFree(d_cnt);
+        if (h_cnt) (void)hipHostFree(h_cnt);
+        if (Bbuf) (void)hipFree(Bbuf);
+    }
+};
+
+// Append one launch's entries: segment g gets nblk(g) blocks for n(g) items
+// (segments with no items are left out).  Returns the entry count.
+template <class F>
+static uint32_t seg_table(SegTables& t, const std::vector<uint32_t>& segs, F&& f, uint32_t* nblocks)
+{
+    uint32_t b = 0, k = 0;
+    for (uint32_t g : segs) {
+        SegLaunch e{};
+        if (!f(g, e) || e.n == 0 || e.nblk == 0) continue;
+        e.seg = g;
+        e.blk0 = b;
+        b += e.nblk;
+        t.host[t.used + k++] = e;
+    }
+    *nblocks = b;
+    return k;
 }
 
 extern "C" {
@@ -1273,6 +1424,339 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     h->stats = stats;
     h->n_spikes = total;
     h->have_result = true;
+    return FBP_OK;
+}
+
+int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, const int64_t* ld, const int64_t* N,
+                  int32_t v_on_device, const uint32_t* n_max_shift, uint32_t flags)
+{
+    if (!hs || nseg <= 0 || !V || !ld || !N || !n_max_shift) return FBP_E_INVALID;
+    fbp_handle* h = hs[0];
+    if (!h) return FBP_E_INVALID;
+    if (nseg > 65535) return set_err(h, FBP_E_UNSUPPORTED, "at most 65535 segments per batch");
+    for (int32_t g = 0; g < nseg; g++) {
+        if (!hs[g]) return set_err(h, FBP_E_INVALID, "null segment handle");
+        if (hs[g]->device != h->device) return set_err(h, FBP_E_INVALID, "segment handles on different devices");
+        if (!hs[g]->have_templates) return set_err(hs[g], FBP_E_STATE, "fbp_set_templates must be called first");
+        if (hs[g]->L != h->L) return set_err(h, FBP_E_UNSUPPORTED, "segments of one batch share the template length");
+        for (int32_t k = 0; k < g; k++)
+            if (hs[k] == hs[g]) return set_err(h, FBP_E_INVALID, "a handle appears twice in the batch");
+    }
+    const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
+    const int L = h->L;
+    for (int32_t g = 0; g < nseg; g++) {
+        int rc = batch_prepare(hs[g], h, V[g], ld[g], N[g], v_on_device, n_max_shift[g], recheck);
+        if (rc != FBP_OK) {
+            if (hs[g] != h) h->err = hs[g]->err;
+            return rc;
+        }
+    }
+    HIP_TRY(hipSetDevice(h->device), "hipSetDevice");
+    hipStream_t s = h->stream;
+    EventPair evp;
+    HIP_TRY(hipEventCreate(&evp.a), "event");
+    HIP_TRY(hipEventCreate(&evp.b), "event");
+    HIP_TRY(hipEventRecord(evp.a, s), "event");
+
+    // per-segment descriptors and launch plans
+    BatchBufs bb;
+    bb.desc.resize(nseg);
+    size_t lds_lik = 0, lds_corr = 0, lds_sel = 0, bstride = 1;
+    int bd_sel = 0;
+    std::vector<int64_t> nwin(nseg);
+    for (int32_t g = 0; g < nseg; g++) {
+        fbp_handle* hg = hs[g];
+        SegDesc& d = bb.desc[g];
+        d.ts = template_view(hg);
+        d.st = pass_view(hg);
+        d.tb = recheck ? tables_view(hg) : ScreenTables{};
+        d.blockbuf = hg->blockbuf.as<uint32_t>();
+        size_t ll = 0;
+        seg_likelihood_lds(d.ts, &d.wb, &d.sv, &ll);
+        lds_lik = std::max(lds_lik, ll);
+        d.reach = (int)(3 + (2 * (int64_t)n_max_shift[g] + L) / L);
+        d.rp = 0;
+        if (recheck) {
+            lds_corr = std::max(lds_corr, screen_corr_lds(d.ts, d.tb.S, &d.rp));
+            int bd = 0;
+            size_t ls = 0;
+            if (!screen_select_reg_plan(d.ts, d.tb, &bd, &ls))
+                return set_err(h, FBP_E_UNSUPPORTED, "a segment is too large for the register-resident selection");
+            bd_sel = std::max(bd_sel, bd);
+            lds_sel = std::max(lds_sel, ls);
+            bstride = std::max(bstride, (size_t)d.ts.U * d.tb.S);
+        }
+        nwin[g] = hg->nwin;
+    }
+    if (lds_corr > kMaxLds) return set_err(h, FBP_E_UNSUPPORTED, "a segment's window exceeds the screen tile");
+    HIP_TRY(hipMalloc((void**)&bb.d_desc, sizeof(SegDesc) * nseg), "alloc batch");
+    HIP_TRY(hipMemcpy(bb.d_desc, bb.desc.data(), sizeof(SegDesc) * nseg, hipMemcpyHostToDevice), "copy batch");
+    HIP_TRY(hipHostMalloc((void**)&bb.h_dyn, sizeof(SegDyn) * nseg, hipHostMallocDefault), "alloc batch");
+    HIP_TRY(hipMalloc((void**)&bb.d_dyn, sizeof(SegDyn) * nseg), "alloc batch");
+    for (auto& t : bb.tabs) {
+        t.cap = (size_t)nseg * 16 + 64;
+        HIP_TRY(hipHostMalloc((void**)&t.host, sizeof(SegLaunch) * t.cap, hipHostMallocDefault), "alloc batch");
+        HIP_TRY(hipMalloc((void**)&t.dev, sizeof(SegLaunch) * t.cap), "alloc batch");
+    }
+    // screened recheck rows: flagged windows per screen round
+    int64_t sum_nwin = 0;
+    for (int64_t w : nwin) sum_nwin += w;
+    const uint32_t rows_cap =
+        recheck ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(sum_nwin, 1 << 20),
+                                                                  (int64_t)(kMaxBbufBytes / (bstride * 4))))
+                : 0;
+    int64_t max_tiles = 0;
+    for (int64_t w : nwin) max_tiles = std::max(max_tiles, (w + kCompactTile - 1) / kCompactTile);
+    const size_t eob_n = std::max<size_t>({(size_t)nseg * kSegBlockCap, (size_t)rows_cap, (size_t)nseg * max_tiles, 1});
+    HIP_TRY(hipMalloc((void**)&bb.eob[0], eob_n * 2), "alloc batch");
+    HIP_TRY(hipMalloc((void**)&bb.eob[1], eob_n * 2), "alloc batch");
+    HIP_TRY(hipMalloc((void**)&bb.d_cnt, (size_t)nseg * CNT_N * 4), "alloc batch");
+    HIP_TRY(hipHostMalloc((void**)&bb.h_cnt, (size_t)nseg * CNT_N * 4, hipHostMallocDefault), "alloc batch");
+    if (recheck) HIP_TRY(hipMalloc((void**)&bb.Bbuf, (size_t)rows_cap * bstride * 4), "alloc screen buffer");
+
+    std::vector<uint32_t> nW(nseg), nF(nseg, 0), nadd(nseg, 0), active;
+    std::vector<uint64_t> total(nseg, 0);
+    std::vector<int> passes(nseg, 0);
+    std::vector<fbp_stats> stats(nseg);
+    for (int32_t g = 0; g < nseg; g++) {
+        nW[g] = (uint32_t)nwin[g];
+        active.push_back((uint32_t)g);
+        hs[g]->pass_log.clear();
+    }
+    auto cdiv = [](uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); };
+    auto capb = [&](uint64_t blocks) { return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)blocks, kSegBlockCap)); };
+    auto upload = [&](SegTables& t) -> hipError_t {
+        if (t.used == 0) return hipSuccess;
+        return hipMemcpyAsync(t.dev, t.host, sizeof(SegLaunch) * t.used, hipMemcpyHostToDevice, s);
+    };
+    auto read_counters = [&]() -> hipError_t {
+        launch_seg_counters(bb.d_desc, (uint32_t)nseg, bb.d_cnt, s);
+        hipError_t e = hipMemcpyAsync(bb.h_cnt, bb.d_cnt, (size_t)nseg * CNT_N * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    };
+    int pass = 0;
+    while (!active.empty()) {
+        // ---- likelihood, window maxima, flagged windows ----
+        SegTables& A = bb.tabs[0];
+        A.used = 0;
+        uint32_t nb_lik, nb_clr, nb_max, nb_flag = 0;
+        const size_t o_lik = A.begin();
+        const uint32_t n_lik = seg_table(A, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.nblk = capb(cdiv(nW[g], (uint32_t)bb.desc[g].wb));
+            return true;
+        }, &nb_lik);
+        A.used += n_lik;
+        const size_t o_clr = A.begin();
+        const uint32_t n_clr = seg_table(A, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.nblk = capb(cdiv(nW[g], kBlock));
+            return true;
+        }, &nb_clr);
+        A.used += n_clr;
+        const size_t o_max = A.begin();
+        const uint32_t n_max = seg_table(A, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.nblk = capb(cdiv(nW[g], kBlock / 64));
+            return true;
+        }, &nb_max);
+        A.used += n_max;
+        const size_t o_flag = A.begin();
+        uint32_t n_flag = 0;
+        if (recheck) {
+            n_flag = seg_table(A, active, [&](uint32_t g, SegLaunch& e) {
+                e.n = nW[g];
+                e.nblk = cdiv(nW[g], kCompactTile);
+                return true;
+            }, &nb_flag);
+            A.used += n_flag;
+        }
+        HIP_TRY(upload(A), "copy batch tables");
+        launch_seg_counters(bb.d_desc, (uint32_t)nseg, nullptr, s);  // CNT_PAIRS = 0
+        launch_seg_pass_kernel(SEG_LIKELIHOOD, L, bb.d_desc, bb.d_dyn, A.dev + o_lik, n_lik, nb_lik, bb.eob[0], lds_lik,
+                               pass > 0 ? 1 : 0, s);
+        launch_seg_pass_kernel(SEG_CLEAR_DIRTY, L, bb.d_desc, bb.d_dyn, A.dev + o_clr, n_clr, nb_clr, bb.eob[0], 0, 0, s);
+        launch_seg_pass_kernel(SEG_WINDOW_MAX, L, bb.d_desc, bb.d_dyn, A.dev + o_max, n_max, nb_max, bb.eob[0], 0, 0, s);
+        if (recheck) {
+            launch_seg_compact(0, bb.d_desc, bb.d_dyn, A.dev + o_flag, n_flag, nb_flag, bb.eob[0], s);
+            HIP_TRY(read_counters(), "read counters");
+            for (uint32_t g : active) nF[g] = bb.h_cnt[(size_t)g * CNT_N + CNT_F];
+        }
+        // ---- screened recheck, reassignment, pursuit, compactions ----
+        SegTables& B = bb.tabs[1];
+        B.used = 0;
+        // spike lists: room for one more spike per listed window
+        for (uint32_t g : active) {
+            fbp_handle* hg = hs[g];
+            const size_t need = (size_t)(total[g] + nW[g]) * 4;
+            HIP_TRY(grow_keep(hg->sp_idx, need, (size_t)total[g] * 4, s), "grow spikes");
+            HIP_TRY(grow_keep(hg->sp_lab, need, (size_t)total[g] * 4, s), "grow spikes");
+        }
+        for (int32_t g = 0; g < nseg; g++)
+            bb.h_dyn[g] = SegDyn{hs[g]->sp_idx.as<uint32_t>(), hs[g]->sp_lab.as<uint32_t>(), total[g]};
+        HIP_TRY(hipMemcpyAsync(bb.d_dyn, bb.h_dyn, sizeof(SegDyn) * nseg, hipMemcpyHostToDevice, s), "copy batch");
+        // screen rounds: the flagged windows of the active segments, rows_cap at a time
+        struct Round { size_t o; uint32_t n, nb_corr, nb_sel; };
+        std::vector<Round> rounds;
+        if (recheck) {
+            std::vector<uint32_t> done(nseg, 0);
+            for (;;) {
+                uint32_t rows = 0;
+                Round r{B.begin(), 0, 0, 0};
+                // corr entries, then select entries (same items, one block per window)
+                for (uint32_t g : active) {
+                    if (done[g] >= nF[g] || rows >= rows_cap) continue;
+                    const uint32_t take = std::min(nF[g] - done[g], rows_cap - rows);
+                    SegLaunch e{};
+                    e.seg = g;
+                    e.n = take;
+                    e.a0 = done[g];
+                    e.g0 = rows;
+                    e.blk0 = r.nb_corr;
+                    e.nblk = capb(take);
+                    r.nb_corr += e.nblk;
+                    B.host[B.used + r.n++] = e;
+                    done[g] += take;
+                    rows += take;
+                }
+                if (r.n == 0) break;
+                if (B.used + 2 * (size_t)r.n > B.cap) return set_err(h, FBP_E_UNSUPPORTED, "batch table overflow");
+                for (uint32_t k = 0; k < r.n; k++) {
+                    SegLaunch e = B.host[B.used + k];
+                    e.blk0 = r.nb_sel;
+                    e.nblk = e.n;
+                    r.nb_sel += e.n;
+                    B.host[B.used + r.n + k] = e;
+                }
+                B.used += 2 * (size_t)r.n;
+                rounds.push_back(r);
+            }
+        }
+        uint32_t nb_re = 0, nb_dec, nb_snap, nb_pr, nb_sp, nb_nx;
+        const size_t o_re = B.begin();
+        uint32_t n_re = 0;
+        if (recheck) {
+            n_re = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+                e.n = nF[g];
+                e.nblk = capb(cdiv(nF[g], kBlock));
+                return true;
+            }, &nb_re);
+            B.used += n_re;
+        }
+        const size_t o_dec = B.begin();
+        const uint32_t n_dec = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.a0 = (recheck && nF[g] > 0) ? 1u : 0u;
+            e.nblk = capb(cdiv(nW[g], kBlock));
+            return true;
+        }, &nb_dec);
+        B.used += n_dec;
+        const size_t o_snap = B.begin();
+        const uint32_t n_snap = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = (uint32_t)nwin[g];
+            e.nblk = capb(cdiv(nwin[g], 4 * kBlock));
+            return true;
+        }, &nb_snap);
+        B.used += n_snap;
+        const size_t o_pr = B.begin();
+        const uint32_t n_pr = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.nblk = capb(cdiv(nW[g], kBlock));
+            return true;
+        }, &nb_pr);
+        B.used += n_pr;
+        const size_t o_sp = B.begin();
+        const uint32_t n_sp = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = nW[g];
+            e.nblk = cdiv(nW[g], kCompactTile);
+            return true;
+        }, &nb_sp);
+        B.used += n_sp;
+        const size_t o_nx = B.begin();
+        const uint32_t n_nx = seg_table(B, active, [&](uint32_t g, SegLaunch& e) {
+            e.n = (uint32_t)nwin[g];
+            e.nblk = cdiv(nwin[g], kCompactTile);
+            return true;
+        }, &nb_nx);
+        B.used += n_nx;
+        if (B.used > B.cap) return set_err(h, FBP_E_UNSUPPORTED, "batch table overflow");
+        HIP_TRY(upload(B), "copy batch tables");
+        for (const Round& r : rounds)
+            launch_seg_screen(L, bb.d_desc, B.dev + r.o, r.n, r.nb_corr, bb.eob[0], lds_corr, r.nb_sel, bb.eob[1], bd_sel,
+                              lds_sel, bb.Bbuf, bstride, s);
+        if (recheck)
+            launch_seg_pass_kernel(SEG_REASSIGN, L, bb.d_desc, bb.d_dyn, B.dev + o_re, n_re, nb_re, bb.eob[0], 0, 0, s);
+        launch_seg_pass_kernel(SEG_DECIDE, L, bb.d_desc, bb.d_dyn, B.dev + o_dec, n_dec, nb_dec, bb.eob[0], 0, 0, s);
+        launch_seg_pass_kernel(SEG_SNAPSHOT, L, bb.d_desc, bb.d_dyn, B.dev + o_snap, n_snap, nb_snap, bb.eob[0], 0, 0, s);
+        launch_seg_pass_kernel(SEG_PRUNE, L, bb.d_desc, bb.d_dyn, B.dev + o_pr, n_pr, nb_pr, bb.eob[0], 0, 0, s);
+        launch_seg_compact(1, bb.d_desc, bb.d_dyn, B.dev + o_sp, n_sp, nb_sp, bb.eob[0], s);
+        launch_seg_compact(2, bb.d_desc, bb.d_dyn, B.dev + o_nx, n_nx, nb_nx, bb.eob[0], s);
+        HIP_TRY(read_counters(), "read counters");
+        // ---- residual of this pass's spikes; converged segments leave ----
+        std::vector<uint32_t> next;
+        for (uint32_t g : active) {
+            const uint32_t* c = bb.h_cnt + (size_t)g * CNT_N;
+            nadd[g] = c[CNT_SPIKES];
+            fbp_stats& sg = stats[g];
+            sg.windows_listed += nW[g];
+            sg.windows_flagged += nF[g];
+            sg.likelihood_positions += (int64_t)nW[g] * L * hs[g]->U;
+            sg.likelihood_pairs_computed += c[CNT_PAIRS];
+            passes[g]++;
+            hs[g]->pass_log.push_back(nadd[g]);
+            hs[g]->pass_log.push_back(nW[g]);
+            hs[g]->pass_log.push_back(nF[g]);
+            if (nadd[g] > 0) next.push_back(g);
+        }
+        SegTables& Ct = bb.tabs[2];
+        Ct.used = 0;
+        uint32_t nb_res, nb_mk;
+        const size_t o_res = Ct.begin();
+        const uint32_t n_res = seg_table(Ct, next, [&](uint32_t g, SegLaunch& e) {
+            e.n = nadd[g];
+            e.nblk = capb(nadd[g]);
+            return true;
+        }, &nb_res);
+        Ct.used += n_res;
+        const size_t o_mk = Ct.begin();
+        const uint32_t n_mk = seg_table(Ct, next, [&](uint32_t g, SegLaunch& e) {
+            e.n = nadd[g];
+            e.nblk = capb(cdiv(nadd[g], kBlock));
+            return true;
+        }, &nb_mk);
+        Ct.used += n_mk;
+        HIP_TRY(upload(Ct), "copy batch tables");
+        launch_seg_pass_kernel(SEG_RESIDUAL, L, bb.d_desc, bb.d_dyn, Ct.dev + o_res, n_res, nb_res, bb.eob[0], 0, 0, s);
+        launch_seg_pass_kernel(SEG_MARK_DIRTY, L, bb.d_desc, bb.d_dyn, Ct.dev + o_mk, n_mk, nb_mk, bb.eob[0], 0, 0, s);
+        for (uint32_t g : next) {
+            total[g] += nadd[g];
+            nW[g] = bb.h_cnt[(size_t)g * CNT_N + CNT_W];
+        }
+        active.swap(next);
+        pass++;
+    }
+    HIP_TRY(hipEventRecord(evp.b, s), "event");
+    HIP_TRY(hipGetLastError(), "kernel launch");
+    HIP_TRY(hipEventSynchronize(evp.b), "sync");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, evp.a, evp.b);
+    HIP_TRY(read_counters(), "read counters");
+    for (int32_t g = 0; g < nseg; g++) {
+        fbp_handle* hg = hs[g];
+        const uint32_t* c = bb.h_cnt + (size_t)g * CNT_N;
+        fbp_stats& sg = stats[g];
+        sg.ms_total = ms;
+        sg.passes = passes[g];
+        sg.spikes = (int64_t)total[g];
+        sg.recheck_items = c[CNT_EVAL];
+        sg.recheck_overflow = c[CNT_CAND];
+        sg.screen_shared_pairs = c[CNT_SURV];
+        sg.screen_separable_pairs = c[CNT_NSP];
+        hg->stats = sg;
+        hg->n_spikes = (int64_t)total[g];
+        hg->have_result = true;
+    }
     return FBP_OK;
 }
 

@@ -105,6 +105,36 @@ struct MmaVoltage {
     int64_t ldq, nbq;
 };
 
+// One segment of a batched pursuit, as the segmented kernels read it.
+struct SegDesc {
+    TemplateSet ts;
+    PassState st;
+    ScreenTables tb;
+    uint32_t* blockbuf;  // per-tile counts of the ordered compactions
+    int wb, sv;          // likelihood: windows per block, staged voltage row length
+    int rp;              // screen correlation: staged window row length
+    int reach;           // residual: list entries whose samples a spike can share
+};
+
+// Per-pass values of a segment: its spike list (it grows) and the spikes
+// listed before this pass (where this pass's spikes start).
+struct SegDyn {
+    uint32_t* sp_idx;
+    uint32_t* sp_lab;
+    uint64_t total;
+};
+
+// One entry of a segmented launch: blocks [blk0, blk0 + nblk) of the grid work
+// on segment `seg`, on its n items (windows, flagged windows or spikes) from
+// a0; g0 is the entry's first row in a buffer shared by the launch's entries.
+struct SegLaunch {
+    uint32_t seg, blk0, nblk, n, a0, g0;
+};
+
+// Kernels of a segmented pass launch (launch_seg_pass_kernel).
+enum SegKind { SEG_LIKELIHOOD = 0, SEG_CLEAR_DIRTY, SEG_WINDOW_MAX, SEG_REASSIGN, SEG_DECIDE, SEG_SNAPSHOT, SEG_PRUNE,
+               SEG_RESIDUAL, SEG_MARK_DIRTY };
+
 enum Counter { CNT_F = 0, CNT_SPIKES = 1, CNT_W = 2, CNT_CAND = 3, CNT_EVAL = 4, CNT_LIKX = 5, CNT_SURV = 6, CNT_NSP = 7,
                CNT_PAIRS = 8, CNT_N = 9 };
 

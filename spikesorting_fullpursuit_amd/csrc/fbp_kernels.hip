@@ -21,13 +21,13 @@ namespace fbp {
 constexpr int kLikMaxWb = 16;  // windows per likelihood block at most
 
 template <int LT>
-__global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
+__device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __restrict__ V, int64_t N,
                                                        const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
                                                        float* __restrict__ LIK, float* __restrict__ wmax_v,
                                                        uint16_t* __restrict__ wmax_p,
                                                        const uint64_t* __restrict__ dirty,
                                                        uint32_t* __restrict__ cnt_pairs,
-                                                       const uint8_t* __restrict__ gmask)
+                                                       const uint8_t* __restrict__ gmask, uint32_t bx, uint32_t gx)
 {
     // [wb][C][sv] voltage rows, then per-item maxima [wb][U][ngr] (value, position),
     // then the block's (window, template) pairs to compute [wb][U], then the
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     uint32_t* pp = reinterpret_cast<uint32_t*>(pv + wb * per_win);
     uint32_t* plist = pp + wb * per_win;
     int* clist = reinterpret_cast<int*>(plist + wb * U);
-    for (uint32_t bt = blockIdx.x; bt < nbatch; bt += gridDim.x) {
+    for (uint32_t bt = bx; bt < nbatch; bt += gx) {
         const uint32_t w0 = bt * wb;
         const int nb = (int)min((uint32_t)wb, nW - w0);
         __syncthreads();
@@ -208,17 +208,29 @@ __global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const flo
     }
 }
 
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_likelihood(TemplateSet ts, const float* __restrict__ V, int64_t N,
+                                                       const uint32_t* __restrict__ W, uint32_t nW, int wb, int sv,
+                                                       float* __restrict__ LIK, float* __restrict__ wmax_v,
+                                                       uint16_t* __restrict__ wmax_p,
+                                                       const uint64_t* __restrict__ dirty,
+                                                       uint32_t* __restrict__ cnt_pairs,
+                                                       const uint8_t* __restrict__ gmask)
+{
+    likelihood_body<LT>(ts, V, N, W, nW, wb, sv, LIK, wmax_v, wmax_p, dirty, cnt_pairs, gmask, blockIdx.x, gridDim.x);
+}
+
 // Marks, for every spike subtracted this pass, the windows whose likelihood
 // positions read any of its samples with the spike's channels: window w
 // computes positions [wL, wL + L), which read samples [wL, wL + 2L - 1), and
 // that range meets the spike's [p, p + L) exactly for
 // w in [ceil((p - 2L + 2) / L), floor((p + L - 1) / L)].  Integer OR is order
 // independent.
-__global__ __launch_bounds__(kBlock) void k_mark_dirty(TemplateSet ts, PassState st, const uint32_t* __restrict__ sidx,
-                                                       const uint32_t* __restrict__ slab, uint32_t nsp)
+__device__ __forceinline__ void mark_dirty_body(TemplateSet ts, PassState st, const uint32_t* __restrict__ sidx,
+                                                       const uint32_t* __restrict__ slab, uint32_t nsp, uint32_t bx, uint32_t gx)
 {
     const int64_t L = ts.L;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nsp; k += gridDim.x * blockDim.x) {
+    for (uint32_t k = bx * blockDim.x + threadIdx.x; k < nsp; k += gx * blockDim.x) {
         const uint32_t u = slab[k];
         if (u >= (uint32_t)ts.U) continue;
         const int64_t p = sidx[k];
@@ -228,11 +240,22 @@ __global__ __launch_bounds__(kBlock) void k_mark_dirty(TemplateSet ts, PassState
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_mark_dirty(TemplateSet ts, PassState st, const uint32_t* __restrict__ sidx,
+                                                       const uint32_t* __restrict__ slab, uint32_t nsp)
+{
+    mark_dirty_body(ts, st, sidx, slab, nsp, blockIdx.x, gridDim.x);
+}
+
 // The listed windows' likelihoods are current: clear their dirty channels.
+__device__ __forceinline__ void clear_dirty_body(PassState st, uint32_t nW, uint32_t bx, uint32_t gx)
+{
+    for (uint32_t i = bx * blockDim.x + threadIdx.x; i < nW; i += gx * blockDim.x)
+        st.dirty[st.W[i]] = 0ull;
+}
+
 __global__ __launch_bounds__(kBlock) void k_clear_dirty(PassState st, uint32_t nW)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nW; i += gridDim.x * blockDim.x)
-        st.dirty[st.W[i]] = 0ull;
+    clear_dirty_body(st, nW, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -240,12 +263,12 @@ __global__ __launch_bounds__(kBlock) void k_clear_dirty(PassState st, uint32_t n
 // template in order, binary_pursuit.cl:350-441.  One wave per listed window;
 // lexicographic arg-max (value, then template, then position).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, const float* __restrict__ wmax_v,
+__device__ __forceinline__ void window_max_body(int U, int L, int64_t N, const float* __restrict__ wmax_v,
                                                        const uint16_t* __restrict__ wmax_p,
-                                                       const uint32_t* __restrict__ W, uint32_t nW, PassState st)
+                                                       const uint32_t* __restrict__ W, uint32_t nW, PassState st, uint32_t bx, uint32_t gx)
 {
     const uint32_t wpb = kBlock / 64;
-    for (uint32_t w = blockIdx.x * wpb + (threadIdx.x >> 6); w < nW; w += gridDim.x * wpb) {
+    for (uint32_t w = bx * wpb + (threadIdx.x >> 6); w < nW; w += gx * wpb) {
         const int64_t id = W[w];
         const int64_t sw = id * L, ew = (id + 1) * L;
         if (ew >= N - L) continue;  // binary_pursuit.cl:380-383
@@ -288,6 +311,13 @@ __global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, 
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_window_max(int U, int L, int64_t N, const float* __restrict__ wmax_v,
+                                                       const uint16_t* __restrict__ wmax_p,
+                                                       const uint32_t* __restrict__ W, uint32_t nW, PassState st)
+{
+    window_max_body(U, L, N, wmax_v, wmax_p, W, nW, st, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Ordered stream compaction (three kernels: count, scan, scatter).  Output
 // order equals input order, so lists stay sorted by window.
@@ -312,10 +342,10 @@ struct SelNextWindows {  // W' = nonzero(check); resets check and overlap flags
 };
 
 template <class Sel>
-__global__ __launch_bounds__(kBlock) void k_compact_count(Sel sel, uint32_t n, uint32_t* blockcnt)
+__device__ __forceinline__ void compact_count_body(Sel sel, uint32_t n, uint32_t* blockcnt, uint32_t bx, uint32_t gx)
 {
     __shared__ uint32_t lds[kBlock / 64];
-    const uint32_t base = blockIdx.x * kCompactTile;
+    const uint32_t base = bx * kCompactTile;
     uint32_t tot_all = 0;
     for (int j = 0; j < kCompactTile / kBlock; j++) {
         const uint32_t e = base + j * kBlock + threadIdx.x;
@@ -324,11 +354,17 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(Sel sel, uint32_t n, u
         block_excl_scan_flag(f, lds, tot);
         tot_all += tot;
     }
-    if (threadIdx.x == 0) blockcnt[blockIdx.x] = tot_all;
+    if (threadIdx.x == 0) blockcnt[bx] = tot_all;
+}
+
+template <class Sel>
+__global__ __launch_bounds__(kBlock) void k_compact_count(Sel sel, uint32_t n, uint32_t* blockcnt)
+{
+    compact_count_body<Sel>(sel, n, blockcnt, blockIdx.x, gridDim.x);
 }
 
 // Exclusive scan of nb block counts in place; total into *out_total.
-__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* blockcnt, uint32_t nb, uint32_t* out_total)
+__device__ __forceinline__ void compact_scan_body(uint32_t* blockcnt, uint32_t nb, uint32_t* out_total)
 {
     __shared__ uint32_t part[1024];
     uint32_t carry = 0;
@@ -351,12 +387,17 @@ __global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* blockcnt, uint3
     if (threadIdx.x == 0) *out_total = carry;
 }
 
+__global__ __launch_bounds__(1024) void k_compact_scan(uint32_t* blockcnt, uint32_t nb, uint32_t* out_total)
+{
+    compact_scan_body(blockcnt, nb, out_total);
+}
+
 template <class Sel>
-__global__ __launch_bounds__(kBlock) void k_compact_scatter(Sel sel, uint32_t n, const uint32_t* blockoff)
+__device__ __forceinline__ void compact_scatter_body(Sel sel, uint32_t n, const uint32_t* blockoff, uint32_t bx, uint32_t gx)
 {
     __shared__ uint32_t lds[kBlock / 64];
-    const uint32_t base = blockIdx.x * kCompactTile;
-    uint32_t run = blockoff[blockIdx.x];
+    const uint32_t base = bx * kCompactTile;
+    uint32_t run = blockoff[bx];
     for (int j = 0; j < kCompactTile / kBlock; j++) {
         const uint32_t e = base + j * kBlock + threadIdx.x;
         const bool inb = e < n;
@@ -367,6 +408,12 @@ __global__ __launch_bounds__(kBlock) void k_compact_scatter(Sel sel, uint32_t n,
         if (inb) sel.always(e);
         run += tot;
     }
+}
+
+template <class Sel>
+__global__ __launch_bounds__(kBlock) void k_compact_scatter(Sel sel, uint32_t n, const uint32_t* blockoff)
+{
+    compact_scatter_body<Sel>(sel, n, blockoff, blockIdx.x, gridDim.x);
 }
 
 template <class Sel>
@@ -474,11 +521,11 @@ __global__ __launch_bounds__(kBlock) void k_recheck_parse(TemplateSet ts, PassSt
 
 // check_overlap_reassignments, binary_pursuit.cl:822-875.  orc is only read
 // here (the post-parse snapshot); giving up is recorded in `giveup`.
-__global__ __launch_bounds__(kBlock) void k_reassign(int L, PassState st)
+__device__ __forceinline__ void reassign_body(int L, PassState st, uint32_t bx, uint32_t gx)
 {
     const uint32_t nF = st.counters[CNT_F];
     const int64_t nwf = st.N / L;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nF; w += gridDim.x * blockDim.x) {
+    for (uint32_t w = bx * blockDim.x + threadIdx.x; w < nF; w += gx * blockDim.x) {
         const int64_t id = st.F[w];
         uint8_t gu = 0;
         if (st.orc[id] != 0) {
@@ -503,13 +550,18 @@ __global__ __launch_bounds__(kBlock) void k_reassign(int L, PassState st)
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_reassign(int L, PassState st)
+{
+    reassign_body(L, st, blockIdx.x, gridDim.x);
+}
+
 // binary_pursuit kernel phase A (binary_pursuit.cl:960-1004): add decisions
 // and the "=3" flags.  `use_giveup` is 0 when no recheck ran this pass.
-__global__ __launch_bounds__(kBlock) void k_pursuit_decide(int L, const float* __restrict__ thr, PassState st,
-                                                           uint32_t nW, int use_giveup)
+__device__ __forceinline__ void pursuit_decide_body(int L, const float* __restrict__ thr, PassState st,
+                                                           uint32_t nW, int use_giveup, uint32_t bx, uint32_t gx)
 {
     const int64_t N = st.N, nwf = N / L;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nW; w += gridDim.x * blockDim.x) {
+    for (uint32_t w = bx * blockDim.x + threadIdx.x; w < nW; w += gx * blockDim.x) {
         const int64_t id = st.W[w];
         const int64_t sw = id * L, ew = (id + 1) * L;
         float ml = 0.0f;
@@ -531,18 +583,29 @@ __global__ __launch_bounds__(kBlock) void k_pursuit_decide(int L, const float* _
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pursuit_decide(int L, const float* __restrict__ thr, PassState st,
+                                                           uint32_t nW, int use_giveup)
+{
+    pursuit_decide_body(L, thr, st, nW, use_giveup, blockIdx.x, gridDim.x);
+}
+
+__device__ __forceinline__ void copy_bytes_body(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n, uint32_t bx, uint32_t gx)
+{
+    for (int64_t i = bx * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gx * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(kBlock) void k_copy_bytes(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n)
 {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    copy_bytes_body(src, dst, n, blockIdx.x, gridDim.x);
 }
 
 // binary_pursuit kernel phase B (binary_pursuit.cl:1009-1045): pruning of
 // windows that added nothing, reading neighbour flags from the snapshot.
-__global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __restrict__ thr, PassState st, uint32_t nW)
+__device__ __forceinline__ void pursuit_prune_body(int L, const float* __restrict__ thr, PassState st, uint32_t nW, uint32_t bx, uint32_t gx)
 {
     const int64_t N = st.N, nwf = N / L;
-    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nW; w += gridDim.x * blockDim.x) {
+    for (uint32_t w = bx * blockDim.x + threadIdx.x; w < nW; w += gx * blockDim.x) {
         const int64_t id = st.W[w];
         const int64_t sw = id * L, ew = (id + 1) * L;
         if (!(ew < N - L) || st.has[w]) continue;
@@ -557,6 +620,11 @@ __global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __restrict__ thr, PassState st, uint32_t nW)
+{
+    pursuit_prune_body(L, thr, st, nW, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Residual: compute_residual, binary_pursuit.cl:57-102, without atomics.
 // A spike changes only the channels its template holds (a channel that is
@@ -565,12 +633,12 @@ __global__ __launch_bounds__(kBlock) void k_pursuit_prune(int L, const float* __
 // once, by the first of them in list order, which subtracts all covering
 // templates in list order.  Covering spikes lie within `reach` list entries.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __restrict__ V, int64_t N,
+__device__ __forceinline__ void residual_body(TemplateSet ts, float* __restrict__ V, int64_t N,
                                                      const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ slab,
-                                                     uint32_t nsp, int reach)
+                                                     uint32_t nsp, int reach, uint32_t bx, uint32_t gx)
 {
     const int C = ts.C, L = ts.L;
-    for (uint32_t k = blockIdx.x; k < nsp; k += gridDim.x) {
+    for (uint32_t k = bx; k < nsp; k += gx) {
         const int64_t pk = sidx[k];
         const uint32_t lk = slab[k];
         if (lk >= (uint32_t)ts.U) continue;
@@ -604,6 +672,13 @@ __global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __re
             V[(size_t)x * C + c] = v;
         }
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual(TemplateSet ts, float* __restrict__ V, int64_t N,
+                                                     const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ slab,
+                                                     uint32_t nsp, int reach)
+{
+    residual_body(ts, V, N, sidx, slab, nsp, reach, blockIdx.x, gridDim.x);
 }
 
 // get_adjusted_clips, binary_pursuit.cl:1088-1130.
@@ -866,6 +941,235 @@ void launch_adjusted_clips(const TemplateSet& ts, const float* V, int64_t N, con
     if (nsp == 0) return;
     hipLaunchKernelGGL(k_adjusted_clips, dim3(grid_for((int64_t)(nsp * ts.C * ts.L), kBlock * 4)), dim3(kBlock), 0, s,
                        ts, V, N, sidx, slab, nsp, clips);
+}
+
+
+// ---------------------------------------------------------------------------
+// Segmented launches of the pass kernels: one grid over every active segment
+// of a batch.  Block b works on entry eob[b] of the launch table with the
+// entry's blocks as its grid; each body is the single-segment kernel's.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_fill_eob(const SegLaunch* __restrict__ tab, uint32_t ntab, uint16_t* eob)
+{
+    for (uint32_t e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const uint32_t b0 = tab[e].blk0, nb = tab[e].nblk;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) eob[b0 + i] = (uint16_t)e;
+    }
+}
+
+#define SEG_ENTRY()                                      \
+    const SegLaunch e = tab[eob[blockIdx.x]];            \
+    const SegDesc& d = sd[e.seg];                        \
+    const uint32_t bx = blockIdx.x - e.blk0, gx = e.nblk
+
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_likelihood_seg(const SegDesc* __restrict__ sd, const SegLaunch* __restrict__ tab,
+                                                           const uint16_t* __restrict__ eob, int use_dirty)
+{
+    SEG_ENTRY();
+    likelihood_body<LT>(d.ts, d.st.V, d.st.N, d.st.W, e.n, d.wb, d.sv, d.st.LIK, d.st.wmax_v, d.st.wmax_p,
+                        use_dirty ? d.st.dirty : nullptr, d.st.counters + CNT_PAIRS, nullptr, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_clear_dirty_seg(const SegDesc* __restrict__ sd,
+                                                            const SegLaunch* __restrict__ tab,
+                                                            const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    clear_dirty_body(d.st, e.n, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_window_max_seg(const SegDesc* __restrict__ sd,
+                                                           const SegLaunch* __restrict__ tab,
+                                                           const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    window_max_body(d.ts.U, d.ts.L, d.st.N, d.st.wmax_v, d.st.wmax_p, d.st.W, e.n, d.st, bx, gx);
+}
+
+// The three ordered compactions of a pass, by kind: 0 flagged windows (into F,
+// count CNT_F), 1 this pass's spikes (appended to the segment's list, count
+// CNT_SPIKES), 2 the next window list (into W, count CNT_W).
+template <int K>
+struct SegSel;
+template <>
+struct SegSel<0> {
+    static __device__ SelFlagged make(const SegDesc& d, const SegDyn&) { return SelFlagged{d.st.W, d.st.orc, d.st.F}; }
+    static constexpr int cnt = CNT_F;
+};
+template <>
+struct SegSel<1> {
+    static __device__ SelSpikes make(const SegDesc& d, const SegDyn& y)
+    {
+        return SelSpikes{d.st.has, d.st.cand_idx, d.st.cand_lab, y.sp_idx, y.sp_lab, y.total};
+    }
+    static constexpr int cnt = CNT_SPIKES;
+};
+template <>
+struct SegSel<2> {
+    static __device__ SelNextWindows make(const SegDesc& d, const SegDyn&) { return SelNextWindows{d.st.check, d.st.orc, d.st.W}; }
+    static constexpr int cnt = CNT_W;
+};
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_compact_count_seg(const SegDesc* __restrict__ sd, const SegDyn* __restrict__ dy,
+                                                              const SegLaunch* __restrict__ tab,
+                                                              const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    (void)gx;
+    compact_count_body(SegSel<K>::make(d, dy[e.seg]), e.n, d.blockbuf, bx, gx);
+}
+
+template <int K>
+__global__ __launch_bounds__(1024) void k_compact_scan_seg(const SegDesc* __restrict__ sd, const SegLaunch* __restrict__ tab)
+{
+    const SegLaunch e = tab[blockIdx.x];
+    const SegDesc& d = sd[e.seg];
+    compact_scan_body(d.blockbuf, (e.n + kCompactTile - 1) / kCompactTile, d.st.counters + SegSel<K>::cnt);
+}
+
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_compact_scatter_seg(const SegDesc* __restrict__ sd,
+                                                                const SegDyn* __restrict__ dy,
+                                                                const SegLaunch* __restrict__ tab,
+                                                                const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    compact_scatter_body(SegSel<K>::make(d, dy[e.seg]), e.n, d.blockbuf, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reassign_seg(const SegDesc* __restrict__ sd, const SegLaunch* __restrict__ tab,
+                                                         const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    reassign_body(d.ts.L, d.st, bx, gx);
+}
+
+// a0: 1 when the segment's recheck ran this pass (its give-up flags are current)
+__global__ __launch_bounds__(kBlock) void k_pursuit_decide_seg(const SegDesc* __restrict__ sd,
+                                                               const SegLaunch* __restrict__ tab,
+                                                               const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    pursuit_decide_body(d.ts.L, d.ts.thr, d.st, e.n, (int)e.a0, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_check_snapshot_seg(const SegDesc* __restrict__ sd,
+                                                               const SegLaunch* __restrict__ tab,
+                                                               const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    copy_bytes_body(d.st.check, d.st.check_snap, d.st.nwin, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pursuit_prune_seg(const SegDesc* __restrict__ sd,
+                                                              const SegLaunch* __restrict__ tab,
+                                                              const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    pursuit_prune_body(d.ts.L, d.ts.thr, d.st, e.n, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_seg(const SegDesc* __restrict__ sd, const SegDyn* __restrict__ dy,
+                                                         const SegLaunch* __restrict__ tab,
+                                                         const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    const SegDyn y = dy[e.seg];
+    residual_body(d.ts, d.st.V, d.st.N, y.sp_idx + y.total, y.sp_lab + y.total, e.n, d.reach, bx, gx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_mark_dirty_seg(const SegDesc* __restrict__ sd, const SegDyn* __restrict__ dy,
+                                                           const SegLaunch* __restrict__ tab,
+                                                           const uint16_t* __restrict__ eob)
+{
+    SEG_ENTRY();
+    const SegDyn y = dy[e.seg];
+    mark_dirty_body(d.ts, d.st, y.sp_idx + y.total, y.sp_lab + y.total, e.n, bx, gx);
+}
+
+// Per segment: zero the pass's computed-pair count, or copy the counters out.
+__global__ __launch_bounds__(kBlock) void k_seg_counters(const SegDesc* __restrict__ sd, uint32_t nseg,
+                                                         uint32_t* __restrict__ out)
+{
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nseg; g += gridDim.x * blockDim.x) {
+        uint32_t* c = sd[g].st.counters;
+        if (out == nullptr) {
+            c[CNT_PAIRS] = 0u;
+        } else {
+            for (int k = 0; k < CNT_N; k++) out[(size_t)g * CNT_N + k] = c[k];
+        }
+    }
+}
+#undef SEG_ENTRY
+
+// Launchers of the segmented kernels.  `tab` (device) holds ntab entries whose
+// blocks tile [0, nblocks); `eob` has room for nblocks entries.
+static void fill_eob(const SegLaunch* tab, uint32_t ntab, uint16_t* eob, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_eob, dim3(std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024))), dim3(kBlock), 0, s, tab,
+                       ntab, eob);
+}
+
+void seg_likelihood_lds(const TemplateSet& ts, int* wb, int* sv, size_t* lds)
+{
+    likelihood_plan(ts, wb, sv);
+    *lds = (size_t)(*wb) * likelihood_lds_per_window(ts, *sv);
+}
+
+void launch_seg_pass_kernel(int kind, int L, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
+                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s)
+{
+    if (ntab == 0 || nblocks == 0) return;
+    fill_eob(tab, ntab, eob, s);
+    const dim3 g(nblocks), b(kBlock);
+    switch (kind) {
+    case SEG_LIKELIHOOD:
+        if (L == 48) hipLaunchKernelGGL(k_likelihood_seg<48>, g, b, lds, s, sd, tab, eob, flag);
+        else if (L == 64) hipLaunchKernelGGL(k_likelihood_seg<64>, g, b, lds, s, sd, tab, eob, flag);
+        else if (L == 32) hipLaunchKernelGGL(k_likelihood_seg<32>, g, b, lds, s, sd, tab, eob, flag);
+        else hipLaunchKernelGGL(k_likelihood_seg<0>, g, b, lds, s, sd, tab, eob, flag);
+        break;
+    case SEG_CLEAR_DIRTY: hipLaunchKernelGGL(k_clear_dirty_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_WINDOW_MAX: hipLaunchKernelGGL(k_window_max_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_REASSIGN: hipLaunchKernelGGL(k_reassign_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_DECIDE: hipLaunchKernelGGL(k_pursuit_decide_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_SNAPSHOT: hipLaunchKernelGGL(k_check_snapshot_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_PRUNE: hipLaunchKernelGGL(k_pursuit_prune_seg, g, b, 0, s, sd, tab, eob); break;
+    case SEG_RESIDUAL: hipLaunchKernelGGL(k_residual_seg, g, b, 0, s, sd, dy, tab, eob); break;
+    case SEG_MARK_DIRTY: hipLaunchKernelGGL(k_mark_dirty_seg, g, b, 0, s, sd, dy, tab, eob); break;
+    default: break;
+    }
+}
+
+// Ordered compaction of kind K over every entry: one block per tile of
+// kCompactTile items (the entries' nblk), one scan block per entry.
+void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
+                        uint32_t nblocks, uint16_t* eob, hipStream_t s)
+{
+    if (ntab == 0) return;
+    if (nblocks > 0) fill_eob(tab, ntab, eob, s);
+    const dim3 g(nblocks), b(kBlock);
+    if (kind == 0) {
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<0>, g, b, 0, s, sd, dy, tab, eob);
+        hipLaunchKernelGGL(k_compact_scan_seg<0>, dim3(ntab), dim3(1024), 0, s, sd, tab);
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<0>, g, b, 0, s, sd, dy, tab, eob);
+    } else if (kind == 1) {
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<1>, g, b, 0, s, sd, dy, tab, eob);
+        hipLaunchKernelGGL(k_compact_scan_seg<1>, dim3(ntab), dim3(1024), 0, s, sd, tab);
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<1>, g, b, 0, s, sd, dy, tab, eob);
+    } else {
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<2>, g, b, 0, s, sd, dy, tab, eob);
+        hipLaunchKernelGGL(k_compact_scan_seg<2>, dim3(ntab), dim3(1024), 0, s, sd, tab);
+        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<2>, g, b, 0, s, sd, dy, tab, eob);
+    }
+}
+
+void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s)
+{
+    if (nseg == 0) return;
+    hipLaunchKernelGGL(k_seg_counters, dim3(grid_for(nseg, kBlock, 1024)), dim3(kBlock), 0, s, sd, nseg, out);
 }
 
 }  // namespace fbp

@@ -115,8 +115,8 @@ __device__ __forceinline__ void corr_row_range(const float* __restrict__ trow, c
 // One block per window; the window's voltage (zero outside [b, b+L)) is staged
 // in LDS as rows of rp samples, sample p <-> v[p - n].
 template <int LT>
-__global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
-                                                        uint32_t nbatch, int rp, float* __restrict__ Bbuf)
+__device__ __forceinline__ void screen_corr_body(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
+                                                        uint32_t nbatch, int rp, float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx)
 {
     extern __shared__ float sP[];
     const int C = ts.C, U = ts.U;
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassStat
     const int ngr = (S + kPos - 1) / kPos;
     const int64_t N = st.N;
     const uint32_t nF = st.counters[CNT_F];
-    for (uint32_t wi = blockIdx.x; wi < nbatch; wi += gridDim.x) {
+    for (uint32_t wi = bx; wi < nbatch; wi += gx) {
         const uint32_t w = f0 + wi;
         if (w >= nF) break;
         const uint32_t id = st.F[w];
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassStat
             }
         __syncthreads();
         if (skip) continue;
-        float* Bw = Bbuf + (size_t)wi * U * S;
+        float* Bw = Bbuf + (size_t)wi * bstride;
         // item = (shift group g, unit u) with units padded to a multiple of 64,
         // so every wave works on one shift group and skips the same taps
         const int Up = (U + 63) & ~63;
@@ -167,6 +167,13 @@ __global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassStat
             }
         }
     }
+}
+
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_screen_corr(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
+                                                        uint32_t nbatch, int rp, float* __restrict__ Bbuf, size_t bstride)
+{
+    screen_corr_body<LT>(ts, st, tb, f0, nbatch, rp, Bbuf, bstride, blockIdx.x, gridDim.x);
 }
 
 __device__ __forceinline__ float block_max(float v, float* red)
@@ -743,9 +750,9 @@ __device__ __forceinline__ void apply_parse_at(const TemplateSet& ts, const Pass
 }
 
 template <int LT>
-__global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet ts, PassState st, ScreenTables tb,
+__device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState st, ScreenTables tb,
                                                                     uint32_t f0, uint32_t nbatch,
-                                                                    const float* __restrict__ Bbuf)
+                                                                    const float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx)
 {
     extern __shared__ float sm[];
     __shared__ float red[kSelMaxBlock / 64];
@@ -805,7 +812,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     const uint32_t nF = st.counters[CNT_F];
     // one window per block (the grid covers the batch)
     {
-        const uint32_t wi = blockIdx.x;
+        const uint32_t wi = bx;
         const uint32_t w = f0 + wi;
         if (wi >= nbatch || w >= nF) return;
         const int tid = (int)threadIdx.x;
@@ -826,7 +833,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
             if (tid == 0) st.orc[id] = 0;
             return;
         }
-        const float* Bw = Bbuf + (size_t)wi * US;
+        const float* Bw = Bbuf + (size_t)wi * bstride;
         // Staging: every global load below is independent of the others, and
         // no barrier separates them, so they are in flight together.
         // per-pair state: B, sign of LIK[u][b+s-n]; loads from clamped (always
@@ -1369,6 +1376,14 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet 
     }
 }
 
+template <int LT>
+__global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet ts, PassState st, ScreenTables tb,
+                                                                    uint32_t f0, uint32_t nbatch,
+                                                                    const float* __restrict__ Bbuf, size_t bstride)
+{
+    screen_select_reg_body<LT>(ts, st, tb, f0, nbatch, Bbuf, bstride, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -1439,7 +1454,7 @@ static void launch_select_reg(K kernel, int bd, size_t lds, uint32_t nbatch, hip
                               const PassState& st, const ScreenTables& tb, uint32_t f0, const float* Bbuf)
 {
     allow_lds(kernel, lds);
-    hipLaunchKernelGGL(kernel, dim3(nbatch), dim3(bd), lds, s, ts, st, tb, f0, nbatch, Bbuf);
+    hipLaunchKernelGGL(kernel, dim3(nbatch), dim3(bd), lds, s, ts, st, tb, f0, nbatch, Bbuf, (size_t)ts.U * tb.S);
 }
 
 void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
@@ -1481,10 +1496,89 @@ void launch_screen_corr(const TemplateSet& ts, const PassState& st, const Screen
     int rp;
     const size_t lds1 = screen_corr_lds(ts, tb.S, &rp);
     const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
-    if (ts.L == 48) hipLaunchKernelGGL(k_screen_corr<48>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
-    else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf);
+    if (ts.L == 48) hipLaunchKernelGGL(k_screen_corr<48>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf, (size_t)ts.U * tb.S);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_corr<64>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf, (size_t)ts.U * tb.S);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_corr<32>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf, (size_t)ts.U * tb.S);
+    else hipLaunchKernelGGL(k_screen_corr<0>, g, b, lds1, s, ts, st, tb, f0, nbatch, rp, Bbuf, (size_t)ts.U * tb.S);
+}
+
+
+// ---------------------------------------------------------------------------
+// Segmented launches of the screened recheck: the flagged windows of every
+// active segment in one grid.  Entry e covers the segment's flagged windows
+// [a0, a0 + n), whose B rows are rows [g0, g0 + n) of Bbuf (bstride floats each).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_fill_eob_scr(const SegLaunch* __restrict__ tab, uint32_t ntab, uint16_t* eob)
+{
+    for (uint32_t e = blockIdx.x; e < ntab; e += gridDim.x) {
+        const uint32_t b0 = tab[e].blk0, nb = tab[e].nblk;
+        for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) eob[b0 + i] = (uint16_t)e;
+    }
+}
+
+template <int LT>
+__global__ __launch_bounds__(kBlock) void k_screen_corr_seg(const SegDesc* __restrict__ sd, const SegLaunch* __restrict__ tab,
+                                                            const uint16_t* __restrict__ eob, float* __restrict__ Bbuf,
+                                                            size_t bstride)
+{
+    const SegLaunch e = tab[eob[blockIdx.x]];
+    const SegDesc& d = sd[e.seg];
+    screen_corr_body<LT>(d.ts, d.st, d.tb, e.a0, e.n, d.rp, Bbuf + (size_t)e.g0 * bstride, bstride, blockIdx.x - e.blk0,
+                         e.nblk);
+}
+
+template <int LT>
+__global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg_seg(const SegDesc* __restrict__ sd,
+                                                                        const SegLaunch* __restrict__ tab,
+                                                                        const uint16_t* __restrict__ eob,
+                                                                        const float* __restrict__ Bbuf, size_t bstride)
+{
+    const SegLaunch e = tab[eob[blockIdx.x]];
+    const SegDesc& d = sd[e.seg];
+    screen_select_reg_body<LT>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
+                               blockIdx.x - e.blk0, e.nblk);
+}
+
+// Whether the register-resident selection takes this segment, and its block
+// size and dynamic LDS.
+bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* bd, size_t* lds)
+{
+    const int64_t owners = (int64_t)sel_slices(ts.U) * tb.S;
+    *lds = screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max);
+    *bd = (int)std::max<int64_t>(kBlock, (owners + 63) / 64 * 64);
+    return owners <= kSelMaxBlock && *lds <= kSelMaxLds;
+}
+
+// corr: entries' nblk blocks each (grid-stride over their windows); select: one
+// block per window (nblk == n), bd threads.
+void launch_seg_screen(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks_corr,
+                       uint16_t* eob_corr, size_t lds_corr, uint32_t nblocks_sel, uint16_t* eob_sel, int bd,
+                       size_t lds_sel, float* Bbuf, size_t bstride, hipStream_t s)
+{
+    if (ntab == 0) return;
+    const unsigned ge = std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024));
+    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob_corr);
+    // the select table: the same entries with one block per window
+    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab + ntab, ntab, eob_sel);
+    const dim3 gc(nblocks_corr), gs(nblocks_sel), bc(kBlock), bs(bd);
+    const SegLaunch* ts2 = tab + ntab;
+    if (L == 48) {
+        hipLaunchKernelGGL(k_screen_corr_seg<48>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<48>, lds_sel);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<48>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+    } else if (L == 64) {
+        hipLaunchKernelGGL(k_screen_corr_seg<64>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<64>, lds_sel);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<64>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+    } else if (L == 32) {
+        hipLaunchKernelGGL(k_screen_corr_seg<32>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<32>, lds_sel);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<32>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+    } else {
+        hipLaunchKernelGGL(k_screen_corr_seg<0>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<0>, lds_sel);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<0>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+    }
 }
 
 }  // namespace fbp

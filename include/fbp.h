@@ -111,14 +111,19 @@ int fbp_run(fbp_handle *h, const float *V, int64_t ld, int64_t N, int32_t v_on_d
  * pursued with the templates set on hs[g] and n_max_shift[g]; its spikes,
  * residual, statistics and pass log are read back through hs[g] exactly as
  * after fbp_run, and equal fbp_run's on the same segment (the likelihood and
- * the screened overlap recheck run in FP32; precision flags are ignored).
+ * the screened overlap recheck run in FP32; precision flags are ignored;
+ * FBP_FLAG_ADJUSTED_CLIPS gives every segment its clips).
  * The handles are distinct, on one device, with one template length and at
- * most 64 channels each; launches go on hs[0]'s stream.
+ * most 64 channels each; launches go on hs[0]'s stream.  Tile shape (results
+ * do not depend on it; 0 = default): block_cap = blocks per segment and
+ * launch at most (1024), lik_windows = windows per likelihood block (the
+ * single-segment plan's).
  * Replaces, for many independent segments, the per-chunk loop of
  * binary_pursuit_parallel.py:508-1500 called once per segment
  * (spikesorting_parallel.py:2161-2184). */
 int fbp_run_batch(fbp_handle *const *hs, int32_t nseg, const float *const *V, const int64_t *ld, const int64_t *N,
-                  int32_t v_on_device, const uint32_t *n_max_shift, uint32_t flags);
+                  int32_t v_on_device, const uint32_t *n_max_shift, uint32_t flags, int32_t block_cap,
+                  int32_t lik_windows);
 
 /* Results of the last fbp_run: spike onsets within the chunk and labels in
  * pass order (ascending window within a pass). */

@@ -104,7 +104,7 @@ def load():
     L.fbp_device_memory.argtypes = [vp, ctypes.POINTER(i64)]
     L.fbp_set_templates.argtypes = [vp, vp, i32, i32, i32, vp, vp]
     L.fbp_run.argtypes = [vp, vp, i64, i64, i32, u32, u32]
-    L.fbp_run_batch.argtypes = [vp, i32, vp, vp, vp, i32, vp, u32]
+    L.fbp_run_batch.argtypes = [vp, i32, vp, vp, vp, i32, vp, u32, i32, i32]
     L.fbp_result_count.argtypes = [vp, ctypes.POINTER(i64)]
     L.fbp_copy_result.argtypes = [vp, vp, vp]
     L.fbp_copy_clips.argtypes = [vp, vp]
@@ -295,10 +295,11 @@ class Handle:
         return rows[: n.value]
 
 
-def run_batch(handles, voltages, n_max_shift, flags):
+def run_batch(handles, voltages, n_max_shift, flags, block_cap=0, lik_windows=0):
     """fbp_run_batch: pursue segment g (voltages[g], host float32 [C][N] arrays,
     or (device_ptr, ld, n_samples) tuples) with handles[g]'s templates; read the
-    results back through each handle."""
+    results back through each handle.  block_cap / lik_windows: tile shape (0:
+    the library's default)."""
     n = len(handles)
     if n == 0:
         return
@@ -326,4 +327,5 @@ def run_batch(handles, voltages, n_max_shift, flags):
                 raise ValueError(f"segment {g}: voltage channels do not match templates")
             keep.append(V)
             ptrs[g], lds[g], ns[g] = V.ctypes.data, V.strides[0] // 4, V.shape[1]
-    handles[0]._check(lib.fbp_run_batch(hs, n, ptrs, lds, ns, 1 if on_device else 0, nms.ctypes.data, int(flags)))
+    handles[0]._check(lib.fbp_run_batch(hs, n, ptrs, lds, ns, 1 if on_device else 0, nms.ctypes.data, int(flags),
+                                        int(block_cap), int(lik_windows)))

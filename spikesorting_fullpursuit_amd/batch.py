@@ -7,16 +7,22 @@ segments of uneven length, channel count and unit count is the workload of a
 lab that sorts many short sessions or tetrodes; one small segment alone leaves
 most of the 256 compute units idle.
 
-Here a pool of worker threads, each owning one library handle (one HIP stream
-and its device buffers), takes segments from a queue ordered largest first (by
-samples x channels x units, so the long ones start early) and runs the drop-in
-binary_pursuit on each.  The library calls release the GIL, so the pass loops
-of different segments interleave on the device.  Every segment's pass loop is
-the same deterministic sequence of kernels on its own stream, so the results
-equal separate calls bit for bit.
+Two ways to run a batch:
 
-autotune_workers() picks the number of concurrent streams for a batch by
-timing a sample of its segments at each candidate count.
+- "fused" (default): the pass loops of all segments run together in the
+  library (fbp_run_batch): every kernel of a pass is launched once over all
+  segments still converging, with two host reads of all segments' counters per
+  pass.  Segments are grouped so that each group's buffers fit the device.  The
+  likelihood and the screened recheck run in FP32 (the same spikes as every
+  other precision).
+- "streams": a pool of worker threads, each owning one library handle (one HIP
+  stream and its device buffers), takes segments from a queue ordered largest
+  first and runs the drop-in binary_pursuit on each; the pass loops of
+  different segments interleave on the device.
+
+Either way every segment gets exactly the spikes of a separate drop-in call.
+autotune_workers() picks the stream count of the "streams" mode;
+autotune_block_cap() picks the fused mode's blocks per segment and launch.
 """
 import threading
 import time
@@ -70,8 +76,97 @@ def release_device_buffers():
     _pools.clear()
 
 
-def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", stats=None):
-    """Run binary_pursuit on every segment with `n_workers` concurrent streams.
+def _batch_fused(segments, device, stats, tiles):
+    """Fused mode of binary_pursuit_batch: the drop-in's argument handling,
+    chunk plan and output assembly per segment, one fbp_run_batch per group."""
+    t0 = time.perf_counter()
+    prepared = []
+    for i, seg in enumerate(segments):
+        try:
+            prepared.append(bpp.prepare_inputs(seg["voltage"], seg["sort_info"], seg["separability_metrics"],
+                                               seg.get("n_max_shift_inds"), precision="fp32"))
+        except Exception as e:
+            raise RuntimeError(f"segment {i} failed: {e}") from e
+    pool = _pool(device)
+    dev_mem = pool.get(1)[0].device_memory()
+    # work items: (segment, chunk start, chunk stop, bytes), chunks as the drop-in plans them
+    items = []
+    for i, p in enumerate(prepared):
+        per_chunk, onsets = bpp.chunk_plan(
+            p["n_chans"], p["n_samples"], p["sampling_rate"], p["n_templates"], p["template_samples_per_chan"],
+            p["n_max_shift_inds"], dev_mem, None, template_bytes=p["template_bytes"], resident_bytes=p["resident"],
+            shared_pairs=p["shared_pairs"])
+        const, per_sample = bpp.library_device_bytes(p["n_chans"], per_chunk, p["n_templates"],
+                                                     p["template_samples_per_chan"], p["n_max_shift_inds"],
+                                                     shared_pairs=p["shared_pairs"])
+        for a in onsets:
+            b = min(p["n_samples"], a + per_chunk)
+            items.append((i, int(a), int(b), const + per_sample * (b - a)))
+    # groups that fit the device together, one kind of input and one flag set each
+    budget = 0.8 * dev_mem - sum(p["resident"] for p in prepared)
+    groups, cur, used, key = [], [], 0.0, None
+    for it in items:
+        p = prepared[it[0]]
+        k = (p["on_device"], p["flags"] & (_native.FLAG_OVERLAP_RECHECK | _native.FLAG_ADJUSTED_CLIPS))
+        if cur and (k != key or used + it[3] > budget or len(cur) >= 65535):
+            groups.append((key, cur))
+            cur, used = [], 0.0
+        cur.append(it)
+        used += it[3]
+        key = k
+    if cur:
+        groups.append((key, cur))
+    per_seg = [([], [], [], []) for _ in segments]
+    dev_views = {}
+    for (on_device, flags), group in groups:
+        hs = pool.get(len(group))
+        volts, nms = [], []
+        for h, (i, a, b, _) in zip(hs, group):
+            p = prepared[i]
+            h.set_templates(p["T3"], p["bias"], p["thr"])
+            V = segments[i]["voltage"]
+            if on_device:
+                import torch
+                if i not in dev_views:
+                    if V.dtype != torch.float32 or V.stride(1) != 1:
+                        V = V.to(torch.float32).contiguous()
+                    torch.cuda.current_stream(V.device).synchronize()
+                    dev_views[i] = V
+                V = dev_views[i]
+                volts.append((V.data_ptr() + 4 * a, V.stride(0), b - a))
+            else:
+                chunk = V[:, a:b]
+                if (not isinstance(chunk, np.ndarray) or chunk.dtype != np.float32 or chunk.strides[1] != 4
+                        or chunk.strides[0] % 4 != 0):
+                    chunk = np.float32(chunk)
+                volts.append(chunk)
+            nms.append(int(p["n_max_shift_inds"]))
+        _native.run_batch(hs, volts, nms, flags, block_cap=tiles[0], lik_windows=tiles[1])
+        for h, (i, a, b, _) in zip(hs, group):
+            idx, lab = h.result()
+            if idx.shape[0] > 0:
+                out = per_seg[i]
+                out[0].append(idx.astype(np.int64) + a)
+                out[1].append(lab)
+                out[2].append(np.ones(idx.shape[0], dtype="bool"))
+                out[3].append(h.clips(idx.shape[0]) if prepared[i]["get_adjusted_clips"] else [])
+    results = []
+    for seg, p, (si, sl, sb, sc) in zip(segments, prepared, per_seg):
+        r = bpp.assemble_outputs(si, sl, sb, sc, seg.get("v_dtype", np.float32), p["clip_init_samples"])
+        bpp.neuron_separability.add_n_spikes(seg["separability_metrics"], r[1])
+        results.append(r)
+    if stats is not None:
+        stats.update({"seconds": time.perf_counter() - t0, "mode": "fused", "groups": len(groups),
+                      "chunks": len(items), "tiles": tuple(tiles)})
+    return results
+
+
+def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", stats=None, mode="fused",
+                         tiles=(0, 0)):
+    """Run binary_pursuit on every segment: all pass loops together ("fused",
+    tile shape `tiles` = (blocks per segment and launch, windows per likelihood
+    block), 0 = the library's default) or on `n_workers` concurrent streams
+    ("streams", `precision` as the drop-in's).
 
     segments: list of dicts with the drop-in's arguments: "voltage" ([C][N]
     array or float32 device tensor), "sort_info", "separability_metrics" and
@@ -79,12 +174,15 @@ def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", sta
     Returns the list of drop-in results (event_indices, neuron_labels,
     binary_pursuit_spike_bool, adjusted_clips), in input order.  Each
     separability_metrics dict gets its "n_spikes" as from the drop-in.
-    `stats` (a dict) receives the wall time, the worker count and the segment
-    order.
+    `stats` (a dict) receives the wall time and the mode's parameters.
     """
     n = len(segments)
     if n == 0:
         return []
+    if mode == "fused":
+        return _batch_fused(segments, device, stats, tiles)
+    if mode != "streams":
+        raise ValueError("mode must be 'fused' or 'streams'")
     n_workers = max(1, min(int(n_workers), n))
     order = sorted(range(n), key=lambda i: -segment_cost(segments[i]))
     handles = _pool(device).get(n_workers)
@@ -142,11 +240,39 @@ def autotune_workers(segments, candidates=(1, 2, 4, 8), sample=16, device=0, pre
     pick = [order[int(j)] for j in np.linspace(0, len(order) - 1, k).round().astype(int)]
     sub = [segments[i] for i in pick]
     work = sum(int(s["voltage"].shape[0]) * int(s["voltage"].shape[1]) for s in sub)
-    binary_pursuit_batch(sub, n_workers=max(candidates), device=device, precision=precision)
+    binary_pursuit_batch(sub, n_workers=max(candidates), device=device, precision=precision, mode="streams")
     rates = {}
     for c in candidates:
         st = {}
-        binary_pursuit_batch(sub, n_workers=c, device=device, precision=precision, stats=st)
+        binary_pursuit_batch(sub, n_workers=c, device=device, precision=precision, stats=st, mode="streams")
         rates[c] = work / st["seconds"] / 1e6
     best = max(rates, key=lambda c: rates[c])
+    return best, rates
+
+
+def _sample(segments, sample):
+    order = sorted(range(len(segments)), key=lambda i: -segment_cost(segments[i]))
+    k = min(sample, len(segments))
+    pick = [order[int(j)] for j in np.linspace(0, len(order) - 1, k).round().astype(int)]
+    return [segments[i] for i in pick]
+
+
+def autotune_tiles(segments, block_caps=(256, 1024, 4096), lik_windows=(0, 1, 4), sample=32, device=0):
+    """Pick the fused mode's tile shape with the highest throughput on a sample
+    of the batch: every (blocks per segment and launch, windows per likelihood
+    block) pair is timed on `sample` segments drawn evenly from the batch's cost
+    order, after one untimed warm-up run.  Returns (best, {shape: Msamples*ch/s}).
+    Results do not depend on the shape."""
+    if not segments:
+        return (0, 0), {}
+    sub = _sample(segments, sample)
+    work = sum(int(s["voltage"].shape[0]) * int(s["voltage"].shape[1]) for s in sub)
+    binary_pursuit_batch(sub, device=device, mode="fused")
+    rates = {}
+    for cap in block_caps:
+        for wb in lik_windows:
+            st = {}
+            binary_pursuit_batch(sub, device=device, stats=st, mode="fused", tiles=(cap, wb))
+            rates[(cap, wb)] = work / st["seconds"] / 1e6
+    best = max(rates, key=lambda k: rates[k])
     return best, rates

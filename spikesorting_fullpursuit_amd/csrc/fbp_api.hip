@@ -841,7 +841,8 @@ static int stage_out(fbp_handle* h, const float* src, int64_t N, float* out, int
 // ---------------------------------------------------------------------------
 // Batched pursuit over many segments (fbp_run_batch).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kSegBlockCap = 1024;  // blocks per segment and launch at most (bodies grid-stride)
+constexpr int kLikMaxWbHost = 16;         // windows per likelihood block at most (the kernel's limit)
+constexpr uint32_t kSegBlockCap = 1024;  // default blocks per segment and launch at most (bodies grid-stride)
 
 // Allocations, sample-major voltage and initial state of one segment, as the
 // FP32 path of fbp_run sets them up (binary_pursuit_parallel.py:528-655), on
@@ -1428,7 +1429,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
 }
 
 int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, const int64_t* ld, const int64_t* N,
-                  int32_t v_on_device, const uint32_t* n_max_shift, uint32_t flags)
+                  int32_t v_on_device, const uint32_t* n_max_shift, uint32_t flags, int32_t block_cap,
+                  int32_t lik_windows)
 {
     if (!hs || nseg <= 0 || !V || !ld || !N || !n_max_shift) return FBP_E_INVALID;
     fbp_handle* h = hs[0];
@@ -1473,6 +1475,13 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         d.blockbuf = hg->blockbuf.as<uint32_t>();
         size_t ll = 0;
         seg_likelihood_lds(d.ts, &d.wb, &d.sv, &ll);
+        if (lik_windows > 0 && lik_windows != d.wb) {
+            // windows per likelihood block as asked, within the block's limit and 64 KiB of LDS
+            const size_t per = ll / (size_t)d.wb;
+            d.wb = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)lik_windows, (int64_t)kLikMaxWbHost,
+                                                                 (int64_t)(kMaxLds / per)}));
+            ll = per * (size_t)d.wb;
+        }
         lds_lik = std::max(lds_lik, ll);
         d.reach = (int)(3 + (2 * (int64_t)n_max_shift[g] + L) / L);
         d.rp = 0;
@@ -1507,7 +1516,8 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
                 : 0;
     int64_t max_tiles = 0;
     for (int64_t w : nwin) max_tiles = std::max(max_tiles, (w + kCompactTile - 1) / kCompactTile);
-    const size_t eob_n = std::max<size_t>({(size_t)nseg * kSegBlockCap, (size_t)rows_cap, (size_t)nseg * max_tiles, 1});
+    const uint32_t seg_cap = block_cap > 0 ? (uint32_t)std::min<int32_t>(block_cap, 1 << 16) : kSegBlockCap;
+    const size_t eob_n = std::max<size_t>({(size_t)nseg * seg_cap, (size_t)rows_cap, (size_t)nseg * max_tiles, 1});
     HIP_TRY(hipMalloc((void**)&bb.eob[0], eob_n * 2), "alloc batch");
     HIP_TRY(hipMalloc((void**)&bb.eob[1], eob_n * 2), "alloc batch");
     HIP_TRY(hipMalloc((void**)&bb.d_cnt, (size_t)nseg * CNT_N * 4), "alloc batch");
@@ -1524,7 +1534,9 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         hs[g]->pass_log.clear();
     }
     auto cdiv = [](uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); };
-    auto capb = [&](uint64_t blocks) { return std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)blocks, kSegBlockCap)); };
+    auto capb = [&](uint64_t blocks) {
+        return std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>(blocks, seg_cap));
+    };
     auto upload = [&](SegTables& t) -> hipError_t {
         if (t.used == 0) return hipSuccess;
         return hipMemcpyAsync(t.dev, t.host, sizeof(SegLaunch) * t.used, hipMemcpyHostToDevice, s);
@@ -1741,6 +1753,15 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
     HIP_TRY(hipEventSynchronize(evp.b), "sync");
     float ms = 0.0f;
     (void)hipEventElapsedTime(&ms, evp.a, evp.b);
+    if (flags & FBP_FLAG_ADJUSTED_CLIPS) {
+        for (int32_t g = 0; g < nseg; g++) {
+            fbp_handle* hg = hs[g];
+            HIP_TRY(hg->clips.ensure((size_t)std::max<uint64_t>(total[g], 1) * hg->C * L * 4), "alloc clips");
+            launch_adjusted_clips(template_view(hg), hg->V.as<float>(), hg->N, hg->sp_idx.as<uint32_t>(),
+                                  hg->sp_lab.as<uint32_t>(), total[g], hg->clips.as<float>(), s);
+            hg->have_clips = true;
+        }
+    }
     HIP_TRY(read_counters(), "read counters");
     for (int32_t g = 0; g < nseg; g++) {
         fbp_handle* hg = hs[g];

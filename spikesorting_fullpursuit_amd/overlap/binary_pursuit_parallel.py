@@ -122,6 +122,84 @@ def _is_device_tensor(x):
     return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
 
 
+def prepare_inputs(voltage, sort_info, separability_metrics, n_max_shift_inds=None, *, recheck_literal=False,
+                   precision="mixed"):
+    """The drop-in's reading of its arguments (binary_pursuit_parallel.py:175-300,
+    :480-507): kernel inputs (templates [U][C][L], bias, thresholds), the
+    library flags, the shift range and what the chunk plan needs."""
+    n_chans = int(voltage.shape[0])
+    n_samples = int(voltage.shape[1])
+    sampling_rate = sort_info["sampling_rate"]
+    chan_win, _ = time_window_to_samples(sort_info["clip_width"], sampling_rate)
+    template_samples_per_chan = int(sort_info["n_samples_per_chan"])
+
+    templates = separability_metrics["templates"]
+    if templates.ndim == 1:
+        templates = np.reshape(templates, (1, -1))
+    templates = np.float32(templates)
+    n_templates = templates.shape[0]
+    if templates.shape[1] != n_chans * template_samples_per_chan:
+        raise ValueError("templates must have n_channels * n_samples_per_chan columns")
+    clip_init_samples = int(np.abs(chan_win[0]))
+    template_sum_squared = np.float32(-0.5 * separability_metrics["template_SS"])
+    _ = np.float32(separability_metrics["template_SS_by_chan"].flatten(order="C"))
+    neuron_lower_thresholds = np.float32(separability_metrics["neuron_lower_thresholds"])
+    do_overlap_recheck = sort_info["do_overlap_recheck"]
+    get_adjusted_clips = sort_info["get_adjusted_clips"]
+    if n_max_shift_inds is None or n_max_shift_inds < 0:
+        n_max_shift_inds = template_samples_per_chan // 2
+    n_max_shift_inds = np.uint32(n_max_shift_inds)
+
+    on_device = _is_device_tensor(voltage)
+    resident = 0
+    if on_device:
+        import torch
+        resident = voltage.numel() * voltage.element_size()
+        if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
+            resident += voltage.numel() * 4  # the float32 copy made below
+    nz = np.any(templates.reshape(n_templates, n_chans, template_samples_per_chan) != 0, axis=2)
+    shared_pairs = int(np.count_nonzero((nz.astype(np.int64) @ nz.T.astype(np.int64)) > 0))
+    flags = 0
+    if do_overlap_recheck:
+        flags |= _native.FLAG_OVERLAP_RECHECK
+    if get_adjusted_clips:
+        flags |= _native.FLAG_ADJUSTED_CLIPS
+    if recheck_literal:
+        flags |= _native.FLAG_RECHECK_LITERAL
+    if precision == "mixed":
+        flags |= _native.FLAG_PRECISION_MIXED
+    elif precision == "bf16":
+        flags |= _native.FLAG_PRECISION_BF16
+    elif precision == "fp8":
+        flags |= _native.FLAG_PRECISION_FP8
+    else:
+        flags |= _native.FLAG_SCREEN_FP32
+    return {
+        "n_chans": n_chans, "n_samples": n_samples, "sampling_rate": sampling_rate,
+        "template_samples_per_chan": template_samples_per_chan, "n_templates": n_templates,
+        "T3": templates.reshape(n_templates, n_chans, template_samples_per_chan),
+        "bias": np.atleast_1d(template_sum_squared), "thr": np.atleast_1d(neuron_lower_thresholds),
+        "template_bytes": templates.nbytes, "clip_init_samples": clip_init_samples,
+        "n_max_shift_inds": n_max_shift_inds, "flags": flags, "get_adjusted_clips": get_adjusted_clips,
+        "on_device": on_device, "resident": resident, "shared_pairs": shared_pairs,
+    }
+
+
+def assemble_outputs(secret_spike_indices, secret_spike_labels, secret_spike_bool, adjusted_spike_clips, v_dtype,
+                     clip_init_samples):
+    """The drop-in's return arrays from its per-chunk lists
+    (binary_pursuit_parallel.py:1506-1532)."""
+    if len(secret_spike_indices) > 0:
+        event_indices = np.int64(np.hstack(secret_spike_indices))
+        neuron_labels = np.int64(np.hstack(secret_spike_labels))
+        adjusted_clips = (np.vstack(adjusted_spike_clips)).astype(v_dtype)
+        binary_pursuit_spike_bool = np.hstack(secret_spike_bool)
+        event_indices += clip_init_samples
+        return event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips
+    return [], [], [], []
+
+
+
 def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shift_inds=None,
                    kernels_path=None, max_gpu_memory=None, *, device=0, return_residual=False,
                    recheck_literal=False, precision="mixed", stats=None, handle=None):
@@ -144,64 +222,21 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
     """
     if precision not in ("mixed", "fp32", "bf16", "fp8"):
         raise ValueError("precision must be 'mixed', 'fp32', 'bf16' or 'fp8'")
-    n_chans = int(voltage.shape[0])
-    n_samples = int(voltage.shape[1])
-    sampling_rate = sort_info["sampling_rate"]
-    chan_win, _ = time_window_to_samples(sort_info["clip_width"], sampling_rate)
-    template_samples_per_chan = int(sort_info["n_samples_per_chan"])
-
-    templates = separability_metrics["templates"]
-    if templates.ndim == 1:
-        templates = np.reshape(templates, (1, -1))
-    templates = np.float32(templates)
-    n_templates = templates.shape[0]
-    if templates.shape[1] != n_chans * template_samples_per_chan:
-        raise ValueError("templates must have n_channels * n_samples_per_chan columns")
-    clip_init_samples = int(np.abs(chan_win[0]))
-    template_sum_squared = np.float32(-0.5 * separability_metrics["template_SS"])
-    _ = np.float32(separability_metrics["template_SS_by_chan"].flatten(order="C"))
-    neuron_lower_thresholds = np.float32(separability_metrics["neuron_lower_thresholds"])
-    do_overlap_recheck = sort_info["do_overlap_recheck"]
-    get_adjusted_clips = sort_info["get_adjusted_clips"]
-
+    p = prepare_inputs(voltage, sort_info, separability_metrics, n_max_shift_inds, recheck_literal=recheck_literal,
+                       precision=precision)
     h = handle if handle is not None else _handle(device)
-    if n_max_shift_inds is None or n_max_shift_inds < 0:
-        n_max_shift_inds = template_samples_per_chan // 2
-    n_max_shift_inds = np.uint32(n_max_shift_inds)
-
-    on_device = _is_device_tensor(voltage)
-    resident = 0
+    on_device = p["on_device"]
+    n_samples = p["n_samples"]
+    n_max_shift_inds = p["n_max_shift_inds"]
+    flags = p["flags"]
+    get_adjusted_clips = p["get_adjusted_clips"]
+    num_indices_per_chunk, chunk_onsets = chunk_plan(
+        p["n_chans"], n_samples, p["sampling_rate"], p["n_templates"], p["template_samples_per_chan"],
+        n_max_shift_inds, h.device_memory(), max_gpu_memory, template_bytes=p["template_bytes"],
+        resident_bytes=p["resident"], shared_pairs=p["shared_pairs"])
+    h.set_templates(p["T3"], p["bias"], p["thr"])
     if on_device:
         import torch
-        resident = voltage.numel() * voltage.element_size()
-        if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
-            resident += voltage.numel() * 4  # the float32 copy made below
-    nz = np.any(templates.reshape(n_templates, n_chans, template_samples_per_chan) != 0, axis=2)
-    shared_pairs = int(np.count_nonzero((nz.astype(np.int64) @ nz.T.astype(np.int64)) > 0))
-    num_indices_per_chunk, chunk_onsets = chunk_plan(
-        n_chans, n_samples, sampling_rate, n_templates, template_samples_per_chan, n_max_shift_inds,
-        h.device_memory(), max_gpu_memory, template_bytes=templates.nbytes, resident_bytes=resident,
-        shared_pairs=shared_pairs)
-
-    T3 = templates.reshape(n_templates, n_chans, template_samples_per_chan)
-    h.set_templates(T3, np.atleast_1d(template_sum_squared), np.atleast_1d(neuron_lower_thresholds))
-    flags = 0
-    if do_overlap_recheck:
-        flags |= _native.FLAG_OVERLAP_RECHECK
-    if get_adjusted_clips:
-        flags |= _native.FLAG_ADJUSTED_CLIPS
-    if recheck_literal:
-        flags |= _native.FLAG_RECHECK_LITERAL
-    if precision == "mixed":
-        flags |= _native.FLAG_PRECISION_MIXED
-    elif precision == "bf16":
-        flags |= _native.FLAG_PRECISION_BF16
-    elif precision == "fp8":
-        flags |= _native.FLAG_PRECISION_FP8
-    else:
-        flags |= _native.FLAG_SCREEN_FP32
-
-    if on_device:
         if voltage.dtype != torch.float32 or voltage.stride(1) != 1:
             voltage = voltage.to(torch.float32).contiguous()
         # the library works on its own stream: the producer's writes must be done
@@ -234,15 +269,9 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
             adjusted_spike_clips.append(clips)
             secret_spike_bool.append(np.ones(idx.shape[0], dtype="bool"))
 
-    if len(secret_spike_indices) > 0:
-        event_indices = np.int64(np.hstack(secret_spike_indices))
-        neuron_labels = np.int64(np.hstack(secret_spike_labels))
-        adjusted_clips = (np.vstack(adjusted_spike_clips)).astype(v_dtype)
-        binary_pursuit_spike_bool = np.hstack(secret_spike_bool)
-        event_indices += clip_init_samples
-    else:
-        event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips = [], [], [], []
-
+    event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips = assemble_outputs(
+        secret_spike_indices, secret_spike_labels, secret_spike_bool, adjusted_spike_clips, v_dtype,
+        p["clip_init_samples"])
     neuron_separability.add_n_spikes(separability_metrics, neuron_labels)
     out = (event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips)
     if return_residual:

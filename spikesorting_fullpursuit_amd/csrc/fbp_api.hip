@@ -252,7 +252,7 @@ struct fbp_handle {
     int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, rowL1, SS, Bbuf;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf;
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
@@ -417,6 +417,8 @@ static ScreenTables tables_view(const fbp_handle* h)
     tb.nbg = (int)((2 * h->tab_n + 1 + kGB - 1) / kGB);
     tb.npl_max = h->npl_max;
     tb.Ntr = h->Ntr.as<float>();
+    tb.Til = h->Til.as<float>();
+    tb.upad = (h->U + 63) & ~63;
     tb.rowL1 = h->rowL1.as<float>();
     tb.SS = h->SS.as<float>();
     tb.n = (int)h->tab_n;
@@ -736,6 +738,7 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     HIP_TRY(h->minG.ensure((size_t)std::max<int64_t>(np, 1) * S * 4), "alloc gram minima");
     HIP_TRY(h->minGb.ensure((size_t)std::max<int64_t>(np, 1) * S * ((S + kGB - 1) / kGB) * 4), "alloc gram minima");
     HIP_TRY(h->Ntr.ensure((size_t)U * S * 4), "alloc energies");
+    HIP_TRY(h->Til.ensure((size_t)std::max(h->Cmax, 1) * h->Lp * ((U + 63) & ~63) * 4), "alloc interleaved rows");
 
     HIP_TRY(hipMemcpy(h->pid.p, pid.data(), pid.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->pair_l.p, pl.data(), pl.size() * 4, hipMemcpyHostToDevice), "copy");

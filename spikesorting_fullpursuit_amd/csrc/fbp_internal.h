@@ -69,6 +69,12 @@ struct ScreenTables {
     int npl_max;          // most units sharing a channel with one unit
     int sel_region;       // floats of the register-resident selection's phase-dependent LDS region
     float* Ntr;           // [U][S]          energy of the window-truncated template
+    // Template rows interleaved by unit for the FP32 screen correlation:
+    // Til[((k * Lp / 4 + i / 4) * upad + u) * 4 + i % 4] = row k of unit u, tap i
+    // (zero for k >= nch[u] or u >= U), so that lanes on consecutive units read
+    // consecutive 16-byte words.
+    float* Til;           // [cmax][Lp / 4][upad][4]
+    int upad;             // U rounded up to a multiple of 64
     const float* rowL1;   // [R] L1 norm of each compact template row (rounded up)
     const float* SS;      // [U] sum of squares of each template (rounded up)
     int S;

@@ -87,19 +87,45 @@ __global__ __launch_bounds__(kBlock) void k_tab_min_ntr(TemplateSet ts, ScreenTa
     }
 }
 
-// Taps [ilo, ihi) of a sliding correlation (4-tap chunks outside are skipped;
-// they would only add products with the zero padding).
-template <int LT>
-__device__ __forceinline__ void corr_row_range(const float* __restrict__ trow, const float* vrow, float* acc, int ilo,
-                                               int ihi)
+// The unit-interleaved copy of the template rows (ScreenTables::Til).
+__global__ __launch_bounds__(kBlock) void k_tab_interleave(TemplateSet ts, ScreenTables tb)
 {
-    float r[LT + kPos - 1];
+    const int Lp = ts.Lp, up = tb.upad;
+    const int64_t total = (int64_t)tb.cmax * Lp * up;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e & 3);
+        const int64_t r = e >> 2;  // (k * Lp / 4 + i / 4) * up + u
+        const int u = (int)(r % up);
+        const int64_t kq = r / up;
+        const int k = (int)(kq / (Lp / 4)), iq = (int)(kq - (int64_t)k * (Lp / 4));
+        float v = 0.0f;
+        if (u < ts.U && k < ts.nch[u]) v = ts.rows[(size_t)(ts.off[u] + k) * Lp + 4 * iq + i];
+        tb.Til[e] = v;
+    }
+}
+
+// Taps [ilo, ihi) of a sliding correlation (4-tap chunks outside are skipped;
+// they would only add products with the zero padding).  t4 steps over the
+// unit-interleaved rows (ustride float4 per 4 taps); vrow is 16-byte aligned
+// with LT + kPos readable samples.
+template <int LT>
+__device__ __forceinline__ void corr_row_range(const float4* __restrict__ t4, int ustride, const float* vrow, float* acc,
+                                               int ilo, int ihi)
+{
+    static_assert(LT % 4 == 0, "whole 4-tap chunks");
+    float r[LT + kPos];
 #pragma unroll
-    for (int j = 0; j < LT + kPos - 1; j++) r[j] = vrow[j];
+    for (int j = 0; j < LT + kPos; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(vrow + j);
+        r[j] = v.x;
+        r[j + 1] = v.y;
+        r[j + 2] = v.z;
+        r[j + 3] = v.w;
+    }
 #pragma unroll
     for (int i = 0; i < LT; i += 4) {
         if (i + 3 < ilo || i >= ihi) continue;
-        const float4 tv = *reinterpret_cast<const float4*>(trow + i);
+        const float4 tv = t4[(size_t)(i >> 2) * ustride];
 #pragma unroll
         for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv.x, r[i + q], acc[q]);
 #pragma unroll
@@ -111,6 +137,23 @@ __device__ __forceinline__ void corr_row_range(const float* __restrict__ trow, c
     }
 }
 
+// Runtime L: the same taps with a sliding window of kPos registers.
+__device__ __forceinline__ void corr_row_generic(const float* __restrict__ tf, int ustride, const float* vrow, int L,
+                                                 float* acc)
+{
+    float r[kPos];
+#pragma unroll
+    for (int q = 0; q < kPos; q++) r[q] = vrow[q];
+    for (int i = 0; i < L; i++) {
+        const float tv = tf[(size_t)(i >> 2) * ustride * 4 + (i & 3)];
+#pragma unroll
+        for (int q = 0; q < kPos; q++) acc[q] = fmaf(tv, r[q], acc[q]);
+#pragma unroll
+        for (int q = 0; q < kPos - 1; q++) r[q] = r[q + 1];
+        r[kPos - 1] = vrow[i + kPos];
+    }
+}
+
 // B(u, s) for every unit and shift of the flagged windows F[f0 .. f0+nbatch).
 // One block per window; the window's voltage (zero outside [b, b+L)) is staged
 // in LDS as rows of rp samples, sample p <-> v[p - n].
@@ -118,13 +161,17 @@ template <int LT>
 __device__ __forceinline__ void screen_corr_body(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
                                                         uint32_t nbatch, int rp, float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx)
 {
-    extern __shared__ float sP[];
+    extern __shared__ __align__(16) float sP[];
     const int C = ts.C, U = ts.U;
     const int L = (LT > 0) ? LT : ts.L;
     const int S = tb.S, n = tb.n;
     const int ngr = (S + kPos - 1) / kPos;
     const int64_t N = st.N;
     const uint32_t nF = st.counters[CNT_F];
+    // every window of the block fills the same samples [n, n + L) of each row:
+    // the zero padding around them is written once
+    if (bx < nbatch)
+        for (int e = threadIdx.x; e < C * rp; e += blockDim.x) sP[e] = 0.0f;
     for (uint32_t wi = bx; wi < nbatch; wi += gx) {
         const uint32_t w = f0 + wi;
         if (w >= nF) break;
@@ -132,12 +179,39 @@ __device__ __forceinline__ void screen_corr_body(TemplateSet ts, PassState st, S
         const bool skip = window_skipped(ts, st, id);
         const int64_t b = st.best_idx[id];
         __syncthreads();
-        if (!skip)
-            for (int e = threadIdx.x; e < C * rp; e += blockDim.x) {
-                const int p = e / C, c = e - p * C;
-                const int j = p - n;
-                sP[c * rp + p] = (j >= 0 && j < L) ? st.V[(size_t)(b + j) * C + c] : 0.0f;
+        // the window's C x L samples are one contiguous run of the sample-major
+        // voltage (b + L <= N for a window that is not skipped): four float4
+        // loads in flight per thread, then the transposed LDS stores
+        if (!skip) {
+            if (C % 4 == 0) {
+                const float4* src = reinterpret_cast<const float4*>(st.V + (size_t)b * C);
+                const int n4 = C * L / 4;
+                for (int e0 = threadIdx.x; e0 < n4; e0 += 4 * blockDim.x) {
+                    float4 v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int e = e0 + q * (int)blockDim.x;
+                        v[q] = src[e < n4 ? e : 0];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const int e = e0 + q * (int)blockDim.x;
+                        if (e >= n4) continue;
+                        const int j = (4 * e) / C, c = 4 * e - j * C;
+                        float* d = sP + c * rp + n + j;
+                        d[0] = v[q].x;
+                        d[rp] = v[q].y;
+                        d[2 * rp] = v[q].z;
+                        d[3 * rp] = v[q].w;
+                    }
+                }
+            } else {
+                for (int e = threadIdx.x; e < C * L; e += blockDim.x) {
+                    const int j = e / C, c = e - j * C;
+                    sP[c * rp + n + j] = st.V[(size_t)b * C + e];
+                }
             }
+        }
         __syncthreads();
         if (skip) continue;
         float* Bw = Bbuf + (size_t)wi * bstride;
@@ -153,12 +227,13 @@ __device__ __forceinline__ void screen_corr_body(TemplateSet ts, PassState st, S
             for (int q = 0; q < kPos; q++) acc[q] = 0.0f;
             const int ilo = n - s0 - (kPos - 1), ihi = n + L - s0;
             const int r0 = ts.off[u], nr = ts.nch[u];
+            const int lq = ts.Lp >> 2;
             for (int k = 0; k < nr; k++) {
                 const int c = ts.chan[r0 + k];
-                const float* trow = ts.rows + (size_t)(r0 + k) * ts.Lp;
+                const float4* t4 = reinterpret_cast<const float4*>(tb.Til) + (size_t)k * lq * tb.upad + u;
                 const float* vrow = sP + c * rp + s0;
-                if (LT > 0) corr_row_range<(LT > 0 ? LT : 4)>(trow, vrow, acc, ilo, ihi);
-                else lik_row_generic(trow, vrow, L, acc);
+                if (LT > 0) corr_row_range<(LT > 0 ? LT : 4)>(t4, tb.upad, vrow, acc, ilo, ihi);
+                else corr_row_generic(reinterpret_cast<const float*>(t4), tb.upad, vrow, L, acc);
             }
 #pragma unroll
             for (int q = 0; q < kPos; q++) {
@@ -1402,12 +1477,18 @@ void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t
                            npairs);
     hipLaunchKernelGGL(k_tab_min_ntr, dim3(sgrid(npairs * tb.S + (int64_t)ts.U * tb.S, kBlock, 1u << 20)),
                        dim3(kBlock), 0, s, ts, tb, npairs);
+    hipLaunchKernelGGL(k_tab_interleave, dim3(sgrid((int64_t)tb.cmax * ts.Lp * tb.upad, kBlock, 1u << 20)), dim3(kBlock),
+                       0, s, ts, tb);
 }
 
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp)
 {
     const int ngr = (S + kPos - 1) / kPos;
-    *rp = ngr * kPos + ts.L + kPos + 1;
+    // rows of 4k floats with k odd: float4 reads, and lanes on different
+    // channels start in different banks; (ngr - 1) kPos + L + kPos samples read
+    int q = (ngr * kPos + ts.L + 3) / 4;
+    if (q % 2 == 0) q++;
+    *rp = 4 * q;
     return (size_t)ts.C * (*rp) * sizeof(float);
 }
 

@@ -899,104 +899,155 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         const uint32_t lab = st.best_lab[id];
         const int64_t b = st.best_idx[id];
         const float bl = st.best_lik[id];
+        // a window whose best spike fell below its threshold or too near the
+        // chunk end takes no overlap recheck (binary_pursuit.cl:518-527); the
+        // label's row count and first row are read with the test's values
+        const float lik_b = st.LIK[(size_t)lab * N + b];
+        const float thr_l = ts.thr[lab];
+        const int nl = ts.nch[lab];
+        const int offl = ts.off[lab];
+        // first row and row count of unit u = tid (the bound terms below)
+        const int q_r0 = tid < U ? ts.off[tid] : 0;
+        const int q_nr = tid < U ? ts.nch[tid] : 0;
         if (tid == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
         for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
         __syncthreads();
-        // a window whose best spike fell below its threshold or too near the
-        // chunk end takes no overlap recheck (binary_pursuit.cl:518-527)
-        if (window_skipped(ts, st, id)) {
+        if (lik_b < thr_l || b >= N - L) {
             if (tid == 0) st.orc[id] = 0;
             return;
         }
         const float* Bw = Bbuf + (size_t)wi * bstride;
         // Staging: every global load below is independent of the others, and
-        // no barrier separates them, so they are in flight together.
+        // each load's first round is issued before any value is used, so they
+        // are all in flight together.
         // per-pair state: B, sign of LIK[u][b+s-n]; loads from clamped (always
         // valid) addresses, B straight into the pair's register
         const int64_t as_t = b + s_t - n;
         const bool sin = owner && as_t >= 0 && as_t < N - L;
         float bv[kSelR];
+        float lv[kSelR];
         uint32_t pum = 0;
         const int us = u0;
 #pragma unroll
-        for (int i0 = 0; i0 < kSelR; i0 += kSelLR) {
-            float lv[kSelLR];
-#pragma unroll
-            for (int j = 0; j < kSelLR; j++) {
-                const int u = us + KU * (i0 + j);
-                const bool ok = sin && u < U;
-                const int uc = ok ? u : 0;
-                lv[j] = st.LIK[(size_t)uc * N + (ok ? as_t : 0)];
-                bv[i0 + j] = Bw[ok ? uc * S + s_t : 0];
-            }
-            __builtin_amdgcn_sched_barrier(0);  // the round's loads go out together
-#pragma unroll
-            for (int j = 0; j < kSelLR; j++) {
-                const int u = us + KU * (i0 + j);
-                const bool ok = sin && u < U;
-                if (!ok) bv[i0 + j] = -INFINITY;
-                pum |= (ok && lv[j] > 0.0f) ? 1u << (i0 + j) : 0u;
-            }
+        for (int i = 0; i < kSelR; i++) {
+            const int u = us + KU * i;
+            const bool ok = sin && u < U;
+            const int uc = ok ? u : 0;
+            lv[i] = st.LIK[(size_t)uc * N + (ok ? as_t : 0)];
+            bv[i] = Bw[ok ? uc * S + s_t : 0];
         }
-        // the window's samples, eight per thread in flight, and
+        // first round of the window's samples (eight per thread), of the
+        // label's zero-padded rows (eight per thread), its pair ids and its
+        // A row with the sign of LIK[l][b+f-n]
+        int wj = tid / C, wc = tid - wj * C;
+        const int wdj = bd / C, wdc = bd - wdj * C;
+        float wvv[8];
+        int wdst[8], wcq[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const bool ok = tid + q * bd < C * L;
+            wdst[q] = ok ? wc * VS + wj : -1;
+            wcq[q] = wc;
+            wvv[q] = st.V[ok ? (size_t)(b + wj) * C + wc : (size_t)b * C];
+            qr_step(wj, wc, wdj, wdc, C);
+        }
+        int rk = tid / LP, ri = tid - rk * LP;
+        const int rdk = bd / LP, rdi = bd - rdk * LP;
+        float rvv[8];
+        int rdst[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int e = tid + q * bd;
+            const bool in = e < nl * LP && ri >= n && ri < n + L;
+            rdst[q] = e < nl * LP ? e : -1;
+            rvv[q] = ts.rows[in ? (size_t)(offl + rk) * ts.Lp + (ri - n) : (size_t)offl * ts.Lp];
+            rvv[q] = in ? rvv[q] : 0.0f;
+            qr_step(rk, ri, rdk, rdi, LP);
+        }
+        const int pid0 = tid < U ? tb.pid[(size_t)lab * U + tid] : -1;
+        const int64_t af0 = b + tid - n;
+        const bool fin0 = tid < S && af0 >= 0 && af0 < N - L;
+        const float la0 = st.LIK[(size_t)lab * N + (fin0 ? af0 : b)];
+        const float A0 = Bw[(int)lab * S + (tid < S ? tid : 0)];
+        const int cl0 = tid < nl ? ts.chan[offl + tid] : 0;
+        __builtin_amdgcn_sched_barrier(0);  // every first-round load goes out before any is used
+#pragma unroll
+        for (int i = 0; i < kSelR; i++) {
+            const int u = us + KU * i;
+            const bool ok = sin && u < U;
+            if (!ok) bv[i] = -INFINITY;
+            pum |= (ok && lv[i] > 0.0f) ? 1u << i : 0u;
+        }
         // vmax[c] = max_j |V[b+j][c]| (non-negative floats order as their bit
         // patterns; a NaN is skipped as fmaxf skips it)
-        {
-            int j = tid / C, c = tid - j * C;
-            const int dj = bd / C, dc = bd - dj * C;
-            for (int e0 = tid; e0 < C * L; e0 += 8 * bd) {
-                float v[8];
-                int dst[8], cq[8];
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const bool ok = e0 + q * bd < C * L;
-                    dst[q] = ok ? c * VS + j : -1;
-                    cq[q] = c;
-                    v[q] = st.V[ok ? (size_t)(b + j) * C + c : (size_t)b * C];
-                    qr_step(j, c, dj, dc, C);
-                }
-                __builtin_amdgcn_sched_barrier(0);
+        for (int q = 0; q < 8; q++) {
+            if (wdst[q] < 0) continue;
+            vW[wdst[q]] = wvv[q];
+            const float a = fabsf(wvv[q]);
+            if (a == a) atomicMax(reinterpret_cast<unsigned int*>(vmax) + wcq[q], __float_as_uint(a));
+        }
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    if (dst[q] < 0) continue;
-                    vW[dst[q]] = v[q];
-                    const float a = fabsf(v[q]);
-                    if (a == a) atomicMax(reinterpret_cast<unsigned int*>(vmax) + cq[q], __float_as_uint(a));
-                }
+        for (int q = 0; q < 8; q++)
+            if (rdst[q] >= 0) TlP[rdst[q]] = rvv[q];
+        if (tid < UP) {
+            Dp[tid] = make_float2(0.0f, __int_as_float(pid0));
+            if (pid0 >= 0) { atomicMin(&pfirst_sh, pid0); atomicAdd(&npl_sh, 1); }
+        }
+        if (tid < S) {
+            const bool pl = fin0 && la0 > 0.0f;
+            fin[tid] = fin0;
+            posl[tid] = pl;
+            if (pl) { atomicMin(&plo_sh, tid); atomicMax(&phi_sh, tid); }
+            A[tid] = A0;
+        }
+        if (tid < nl) cl[tid] = cl0;
+        // later rounds (large windows, label rows, unit counts or shift ranges only)
+        for (int e0 = tid + 8 * bd; e0 < C * L; e0 += 8 * bd) {
+            float v[8];
+            int dst[8], cq[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const bool ok = e0 + q * bd < C * L;
+                dst[q] = ok ? wc * VS + wj : -1;
+                cq[q] = wc;
+                v[q] = st.V[ok ? (size_t)(b + wj) * C + wc : (size_t)b * C];
+                qr_step(wj, wc, wdj, wdc, C);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if (dst[q] < 0) continue;
+                vW[dst[q]] = v[q];
+                const float a = fabsf(v[q]);
+                if (a == a) atomicMax(reinterpret_cast<unsigned int*>(vmax) + cq[q], __float_as_uint(a));
             }
         }
-        // the label's zero-padded rows, eight per thread in flight
-        const int nl = ts.nch[lab];
-        const int offl = ts.off[lab];
-        {
-            int k = tid / LP, i = tid - k * LP;
-            const int dk = bd / LP, di = bd - dk * LP;
-            for (int e0 = tid; e0 < nl * LP; e0 += 8 * bd) {
-                float v[8];
-                int dst[8];
+        for (int e0 = tid + 8 * bd; e0 < nl * LP; e0 += 8 * bd) {
+            float v[8];
+            int dst[8];
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const int e = e0 + q * bd;
-                    const bool in = e < nl * LP && i >= n && i < n + L;
-                    dst[q] = e < nl * LP ? e : -1;
-                    v[q] = ts.rows[in ? (size_t)(offl + k) * ts.Lp + (i - n) : (size_t)offl * ts.Lp];
-                    v[q] = in ? v[q] : 0.0f;
-                    qr_step(k, i, dk, di, LP);
-                }
-                __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int q = 0; q < 8; q++)
-                    if (dst[q] >= 0) TlP[dst[q]] = v[q];
+            for (int q = 0; q < 8; q++) {
+                const int e = e0 + q * bd;
+                const bool in = e < nl * LP && ri >= n && ri < n + L;
+                dst[q] = e < nl * LP ? e : -1;
+                v[q] = ts.rows[in ? (size_t)(offl + rk) * ts.Lp + (ri - n) : (size_t)offl * ts.Lp];
+                v[q] = in ? v[q] : 0.0f;
+                qr_step(rk, ri, rdk, rdi, LP);
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (dst[q] >= 0) TlP[dst[q]] = v[q];
         }
         for (int i = tid; i < LP; i += bd) zrow[i] = 0.0f;
-        for (int k = tid; k < nl; k += bd) cl[k] = ts.chan[offl + k];
-        for (int u = tid; u < UP; u += bd) {
+        for (int k = tid + bd; k < nl; k += bd) cl[k] = ts.chan[offl + k];
+        for (int u = tid + bd; u < UP; u += bd) {
             const int p = u < U ? tb.pid[(size_t)lab * U + u] : -1;
             Dp[u] = make_float2(0.0f, __int_as_float(p));
             if (p >= 0) { atomicMin(&pfirst_sh, p); atomicAdd(&npl_sh, 1); }
         }
-        for (int f = tid; f < S; f += bd) {
+        for (int f = tid + bd; f < S; f += bd) {
             const int64_t af = b + f - n;
             const bool in = af >= 0 && af < N - L;
             const bool pl = in && st.LIK[(size_t)lab * N + af] > 0.0f;
@@ -1077,7 +1128,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             for (int off = 32; off > 0; off >>= 1) mw = fmaxf(mw, __shfl_xor(mw, off, 64));
             for (int u = tid; u < U; u += bd) {
                 float q = 0.0f, t1 = 0.0f, vsum = 0.0f;
-                const int r0 = ts.off[u], nr = ts.nch[u];
+                const int r0 = u == tid ? q_r0 : ts.off[u], nr = u == tid ? q_nr : ts.nch[u];
                 // eight rows at a time: their channel and L1 loads go out together
                 for (int k0 = 0; k0 < nr; k0 += 8) {
                     int ch[8];

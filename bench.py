@@ -403,13 +403,15 @@ def run_batch(args):
 
     tiles, trates = batch.autotune_tiles(segs, sample=32)
     log(f"autotune: tiles {tiles}, Msamples*ch/s by (blocks per segment, windows per block) {trates}")
+    lanes, lrates = batch.autotune_lanes(segs, sample=64, tiles=tiles)
+    log(f"autotune: lanes {lanes}, Msamples*ch/s by lanes {lrates}")
     workers, wrates = batch.autotune_workers(segs, candidates=(2, 4, 8), sample=12)
     log(f"autotune: workers {workers}, Msamples*ch/s by workers {wrates}")
     for _ in range(args.warmup):
-        batch.binary_pursuit_batch(segs[:16], tiles=tiles)
+        batch.binary_pursuit_batch(segs[:16], tiles=tiles, lanes=lanes)
     stf, sts, st1 = {}, {}, {}
     torch.cuda.synchronize()
-    fused = batch.binary_pursuit_batch(segs, tiles=tiles, stats=stf)
+    fused = batch.binary_pursuit_batch(segs, tiles=tiles, stats=stf, lanes=lanes)
     torch.cuda.synchronize()
     log(f"fused batch: {stf['seconds']:.2f}s in {stf['groups']} group(s)")
     streams = batch.binary_pursuit_batch(segs, n_workers=workers, stats=sts, mode="streams")
@@ -445,6 +447,8 @@ def run_batch(args):
             "workload": f"{len(segs)} segments, 5-120 s, channels {{1,4,8,16,32}}, 4-60 units, L=64, 30 kHz, "
                         f"{work / 1e6:.1f} Msamples·ch in all, voltage resident in HBM",
             "tiles": {"blocks_per_segment": tiles[0], "likelihood_windows_per_block": tiles[1]},
+            "lanes": lanes,
+            "autotune_lanes_msamples_ch_per_s": {str(k): round(v, 2) for k, v in lrates.items()},
             "autotune_tiles_msamples_ch_per_s": {f"{k[0]}x{k[1]}": round(v, 2) for k, v in trates.items()},
             "groups": stf["groups"],
             "streams_value": round(work / sts["seconds"] / 1e6, 3),

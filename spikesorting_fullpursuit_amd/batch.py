@@ -62,11 +62,13 @@ class _Pool:
 _pools = {}
 
 
-def _pool(device):
-    p = _pools.get(device)
+def _pool(device, lane=0):
+    """The handles of `device` for lane `lane` of a fused batch (lane 0 also
+    serves the streams mode)."""
+    p = _pools.get((device, lane))
     if p is None:
         p = _Pool(device)
-        _pools[device] = p
+        _pools[(device, lane)] = p
     return p
 
 
@@ -76,9 +78,12 @@ def release_device_buffers():
     _pools.clear()
 
 
-def _batch_fused(segments, device, stats, tiles):
+def _batch_fused(segments, device, stats, tiles, lanes=1):
     """Fused mode of binary_pursuit_batch: the drop-in's argument handling,
-    chunk plan and output assembly per segment, one fbp_run_batch per group."""
+    chunk plan and output assembly per segment, one fbp_run_batch per group.
+    The chunks are shared out over `lanes` lanes (largest first, each to the
+    lane with the least work so far); the lanes run side by side, each its
+    groups in turn on its own handles and stream."""
     t0 = time.perf_counter()
     prepared = []
     for i, seg in enumerate(segments):
@@ -87,8 +92,7 @@ def _batch_fused(segments, device, stats, tiles):
                                                seg.get("n_max_shift_inds"), precision="fp32"))
         except Exception as e:
             raise RuntimeError(f"segment {i} failed: {e}") from e
-    pool = _pool(device)
-    dev_mem = pool.get(1)[0].device_memory()
+    dev_mem = _pool(device).get(1)[0].device_memory()
     # work items: (segment, chunk start, chunk stop, bytes), chunks as the drop-in plans them
     items = []
     for i, p in enumerate(prepared):
@@ -102,71 +106,109 @@ def _batch_fused(segments, device, stats, tiles):
         for a in onsets:
             b = min(p["n_samples"], a + per_chunk)
             items.append((i, int(a), int(b), const + per_sample * (b - a)))
-    # groups that fit the device together, one kind of input and one flag set each
-    budget = 0.8 * dev_mem - sum(p["resident"] for p in prepared)
-    groups, cur, used, key = [], [], 0.0, None
-    for it in items:
-        p = prepared[it[0]]
-        k = (p["on_device"], p["flags"] & (_native.FLAG_OVERLAP_RECHECK | _native.FLAG_ADJUSTED_CLIPS))
-        if cur and (k != key or used + it[3] > budget or len(cur) >= 65535):
+    lanes = max(1, min(int(lanes), len(items)))
+    # lanes: largest chunk first, each to the lane with the least work so far
+    cost = [(b - a) * prepared[i]["n_chans"] * prepared[i]["n_templates"] for (i, a, b, _) in items]
+    load = [0] * lanes
+    lane_items = [[] for _ in range(lanes)]
+    for j in sorted(range(len(items)), key=lambda j: -cost[j]):
+        k = min(range(lanes), key=lambda k: load[k])
+        lane_items[k].append(items[j])
+        load[k] += cost[j]
+    # groups that fit the device together (the lanes share it), one kind of input and one flag set each
+    budget = (0.8 * dev_mem - sum(p["resident"] for p in prepared)) / lanes
+    lane_groups = []
+    for its in lane_items:
+        groups, cur, used, key = [], [], 0.0, None
+        for it in its:
+            p = prepared[it[0]]
+            k = (p["on_device"], p["flags"] & (_native.FLAG_OVERLAP_RECHECK | _native.FLAG_ADJUSTED_CLIPS))
+            if cur and (k != key or used + it[3] > budget or len(cur) >= 65535):
+                groups.append((key, cur))
+                cur, used = [], 0.0
+            cur.append(it)
+            used += it[3]
+            key = k
+        if cur:
             groups.append((key, cur))
-            cur, used = [], 0.0
-        cur.append(it)
-        used += it[3]
-        key = k
-    if cur:
-        groups.append((key, cur))
-    per_seg = [([], [], [], []) for _ in segments]
+        lane_groups.append(groups)
+    chunks = {}  # (segment, chunk start) -> (indices, labels, spike bools, clips)
     dev_views = {}
-    for (on_device, flags), group in groups:
-        hs = pool.get(len(group))
-        volts, nms = [], []
-        for h, (i, a, b, _) in zip(hs, group):
-            p = prepared[i]
-            h.set_templates(p["T3"], p["bias"], p["thr"])
-            V = segments[i]["voltage"]
-            if on_device:
-                import torch
-                if i not in dev_views:
-                    if V.dtype != torch.float32 or V.stride(1) != 1:
-                        V = V.to(torch.float32).contiguous()
-                    torch.cuda.current_stream(V.device).synchronize()
-                    dev_views[i] = V
-                V = dev_views[i]
-                volts.append((V.data_ptr() + 4 * a, V.stride(0), b - a))
-            else:
-                chunk = V[:, a:b]
-                if (not isinstance(chunk, np.ndarray) or chunk.dtype != np.float32 or chunk.strides[1] != 4
-                        or chunk.strides[0] % 4 != 0):
-                    chunk = np.float32(chunk)
-                volts.append(chunk)
-            nms.append(int(p["n_max_shift_inds"]))
-        _native.run_batch(hs, volts, nms, flags, block_cap=tiles[0], lik_windows=tiles[1])
-        for h, (i, a, b, _) in zip(hs, group):
-            idx, lab = h.result()
-            if idx.shape[0] > 0:
-                out = per_seg[i]
-                out[0].append(idx.astype(np.int64) + a)
-                out[1].append(lab)
-                out[2].append(np.ones(idx.shape[0], dtype="bool"))
-                out[3].append(h.clips(idx.shape[0]) if prepared[i]["get_adjusted_clips"] else [])
+    if any(prepared[i]["on_device"] for i, _, _, _ in items):
+        import torch
+        for i, p in enumerate(prepared):
+            if p["on_device"]:
+                V = segments[i]["voltage"]
+                if V.dtype != torch.float32 or V.stride(1) != 1:
+                    V = V.to(torch.float32).contiguous()
+                torch.cuda.current_stream(V.device).synchronize()
+                dev_views[i] = V
+
+    def run_lane(lane, groups):
+        pool = _pool(device, lane)
+        for (on_device, flags), group in groups:
+            hs = pool.get(len(group))
+            volts, nms = [], []
+            for h, (i, a, b, _) in zip(hs, group):
+                p = prepared[i]
+                h.set_templates(p["T3"], p["bias"], p["thr"])
+                if on_device:
+                    V = dev_views[i]
+                    volts.append((V.data_ptr() + 4 * a, V.stride(0), b - a))
+                else:
+                    chunk = segments[i]["voltage"][:, a:b]
+                    if (not isinstance(chunk, np.ndarray) or chunk.dtype != np.float32 or chunk.strides[1] != 4
+                            or chunk.strides[0] % 4 != 0):
+                        chunk = np.float32(chunk)
+                    volts.append(chunk)
+                nms.append(int(p["n_max_shift_inds"]))
+            _native.run_batch(hs, volts, nms, flags, block_cap=tiles[0], lik_windows=tiles[1])
+            for h, (i, a, b, _) in zip(hs, group):
+                idx, lab = h.result()
+                if idx.shape[0] > 0:
+                    clips = h.clips(idx.shape[0]) if prepared[i]["get_adjusted_clips"] else []
+                    chunks[(i, a)] = (idx.astype(np.int64) + a, lab, np.ones(idx.shape[0], dtype="bool"), clips)
+
+    errors = []
+
+    def guarded(lane, groups):
+        try:
+            run_lane(lane, groups)
+        except BaseException as e:  # re-raised in the caller's thread
+            errors.append(e)
+
+    if lanes == 1:
+        run_lane(0, lane_groups[0])
+    else:
+        threads = [threading.Thread(target=guarded, args=(k, g), daemon=True) for k, g in enumerate(lane_groups)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if errors:
+            raise errors[0]
+    per_seg = [([], [], [], []) for _ in segments]
+    for (i, a) in sorted(chunks):
+        for dst, v in zip(per_seg[i], chunks[(i, a)]):
+            dst.append(v)
     results = []
     for seg, p, (si, sl, sb, sc) in zip(segments, prepared, per_seg):
         r = bpp.assemble_outputs(si, sl, sb, sc, seg.get("v_dtype", np.float32), p["clip_init_samples"])
         bpp.neuron_separability.add_n_spikes(seg["separability_metrics"], r[1])
         results.append(r)
     if stats is not None:
-        stats.update({"seconds": time.perf_counter() - t0, "mode": "fused", "groups": len(groups),
-                      "chunks": len(items), "tiles": tuple(tiles)})
+        stats.update({"seconds": time.perf_counter() - t0, "mode": "fused",
+                      "groups": sum(len(g) for g in lane_groups), "lanes": lanes, "chunks": len(items),
+                      "tiles": tuple(tiles)})
     return results
 
 
 def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", stats=None, mode="fused",
-                         tiles=(0, 0)):
+                         tiles=(0, 0), lanes=2):
     """Run binary_pursuit on every segment: all pass loops together ("fused",
     tile shape `tiles` = (blocks per segment and launch, windows per likelihood
-    block), 0 = the library's default) or on `n_workers` concurrent streams
-    ("streams", `precision` as the drop-in's).
+    block), 0 = the library's default; `lanes` fused loops side by side) or on
+    `n_workers` concurrent streams ("streams", `precision` as the drop-in's).
 
     segments: list of dicts with the drop-in's arguments: "voltage" ([C][N]
     array or float32 device tensor), "sort_info", "separability_metrics" and
@@ -180,7 +222,7 @@ def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", sta
     if n == 0:
         return []
     if mode == "fused":
-        return _batch_fused(segments, device, stats, tiles)
+        return _batch_fused(segments, device, stats, tiles, lanes)
     if mode != "streams":
         raise ValueError("mode must be 'fused' or 'streams'")
     n_workers = max(1, min(int(n_workers), n))
@@ -274,5 +316,24 @@ def autotune_tiles(segments, block_caps=(256, 1024, 4096), lik_windows=(0, 1, 4)
             st = {}
             binary_pursuit_batch(sub, device=device, stats=st, mode="fused", tiles=(cap, wb))
             rates[(cap, wb)] = work / st["seconds"] / 1e6
+    best = max(rates, key=lambda k: rates[k])
+    return best, rates
+
+
+def autotune_lanes(segments, candidates=(1, 2, 3, 4), sample=64, tiles=(0, 0), device=0):
+    """Pick the fused mode's lane count (fused loops side by side) with the
+    highest throughput on `sample` segments drawn evenly from the batch's cost
+    order, after one untimed warm-up run.  Returns (best, {lanes: Msamples*ch/s}).
+    Results do not depend on the lane count."""
+    if not segments:
+        return 1, {}
+    sub = _sample(segments, sample)
+    work = sum(int(s["voltage"].shape[0]) * int(s["voltage"].shape[1]) for s in sub)
+    binary_pursuit_batch(sub, device=device, mode="fused", tiles=tiles, lanes=max(candidates))
+    rates = {}
+    for k in candidates:
+        st = {}
+        binary_pursuit_batch(sub, device=device, stats=st, mode="fused", tiles=tiles, lanes=k)
+        rates[k] = work / st["seconds"] / 1e6
     best = max(rates, key=lambda k: rates[k])
     return best, rates

@@ -56,9 +56,10 @@ void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const Seg
                         uint32_t nblocks, uint16_t* eob, hipStream_t s);
 void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s);
 bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* bd, size_t* lds);
-void launch_seg_screen(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks_corr,
-                       uint16_t* eob_corr, size_t lds_corr, uint32_t nblocks_sel, uint16_t* eob_sel, int bd,
-                       size_t lds_sel, float* Bbuf, size_t bstride, hipStream_t s);
+void launch_seg_screen_corr(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
+                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s);
+void launch_seg_select(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks, uint16_t* eob,
+                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s);
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -926,6 +927,22 @@ static int batch_prepare(fbp_handle* h, fbp_handle* stager, const float* Vsrc, i
     return FBP_OK;
 }
 
+// Side streams (default priority) and their events, owned for the length of a
+// batch: the selection classes of one screen round run on them side by side.
+struct SideStreams {
+    hipStream_t st[4] = {};
+    hipEvent_t ev[4] = {};
+    hipEvent_t fork = nullptr;
+    ~SideStreams()
+    {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& x : st)
+            if (x) (void)hipStreamDestroy(x);
+        if (fork) (void)hipEventDestroy(fork);
+    }
+};
+
 // Launch tables of one upload, built on the host in pinned memory and copied
 // to the device in one transfer.
 struct SegTables {
@@ -1466,8 +1483,15 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
     // per-segment descriptors and launch plans
     BatchBufs bb;
     bb.desc.resize(nseg);
-    size_t lds_lik = 0, lds_corr = 0, lds_sel = 0, bstride = 1;
-    int bd_sel = 0;
+    size_t lds_lik = 0, lds_corr = 0, bstride = 1;
+    // selection classes by dynamic LDS (at most 4, 3, 2 and 1 blocks per CU of
+    // 160 KiB): each class launches with its own largest block and LDS, so the
+    // small segments are not held to the largest one's occupancy
+    constexpr int kSelClasses = 4;
+    const size_t cls_cap[kSelClasses] = {40 * 1024, 53 * 1024, 80 * 1024, SIZE_MAX};
+    int cls_bd[kSelClasses] = {};
+    size_t cls_lds[kSelClasses] = {};
+    std::vector<int> scls(nseg, 0);
     std::vector<int64_t> nwin(nseg);
     for (int32_t g = 0; g < nseg; g++) {
         fbp_handle* hg = hs[g];
@@ -1494,8 +1518,11 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
             size_t ls = 0;
             if (!screen_select_reg_plan(d.ts, d.tb, &bd, &ls))
                 return set_err(h, FBP_E_UNSUPPORTED, "a segment is too large for the register-resident selection");
-            bd_sel = std::max(bd_sel, bd);
-            lds_sel = std::max(lds_sel, ls);
+            int k = 0;
+            while (ls > cls_cap[k]) k++;
+            scls[g] = k;
+            cls_bd[k] = std::max(cls_bd[k], bd);
+            cls_lds[k] = std::max(cls_lds[k], ls);
             bstride = std::max(bstride, (size_t)d.ts.U * d.tb.S);
         }
         nwin[g] = hg->nwin;
@@ -1526,6 +1553,14 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
     HIP_TRY(hipMalloc((void**)&bb.d_cnt, (size_t)nseg * CNT_N * 4), "alloc batch");
     HIP_TRY(hipHostMalloc((void**)&bb.h_cnt, (size_t)nseg * CNT_N * 4, hipHostMallocDefault), "alloc batch");
     if (recheck) HIP_TRY(hipMalloc((void**)&bb.Bbuf, (size_t)rows_cap * bstride * 4), "alloc screen buffer");
+    SideStreams side;
+    if (recheck) {
+        HIP_TRY(hipEventCreateWithFlags(&side.fork, hipEventDisableTiming), "event");
+        for (int c = 1; c < kSelClasses; c++) {
+            HIP_TRY(hipStreamCreateWithFlags(&side.st[c], hipStreamNonBlocking), "stream");
+            HIP_TRY(hipEventCreateWithFlags(&side.ev[c], hipEventDisableTiming), "event");
+        }
+    }
 
     std::vector<uint32_t> nW(nseg), nF(nseg, 0), nadd(nseg, 0), active;
     std::vector<uint64_t> total(nseg, 0);
@@ -1612,13 +1647,19 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
             bb.h_dyn[g] = SegDyn{hs[g]->sp_idx.as<uint32_t>(), hs[g]->sp_lab.as<uint32_t>(), total[g]};
         HIP_TRY(hipMemcpyAsync(bb.d_dyn, bb.h_dyn, sizeof(SegDyn) * nseg, hipMemcpyHostToDevice, s), "copy batch");
         // screen rounds: the flagged windows of the active segments, rows_cap at a time
-        struct Round { size_t o; uint32_t n, nb_corr, nb_sel; };
+        struct Round {
+            size_t o;
+            uint32_t n, nb_corr;
+            size_t so[kSelClasses];  // select entries of each class: table offset, count, blocks
+            uint32_t sn[kSelClasses], snb[kSelClasses];
+        };
         std::vector<Round> rounds;
         if (recheck) {
             std::vector<uint32_t> done(nseg, 0);
             for (;;) {
                 uint32_t rows = 0;
-                Round r{B.begin(), 0, 0, 0};
+                Round r{};
+                r.o = B.begin();
                 // corr entries, then select entries (same items, one block per window)
                 for (uint32_t g : active) {
                     if (done[g] >= nF[g] || rows >= rows_cap) continue;
@@ -1637,12 +1678,19 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
                 }
                 if (r.n == 0) break;
                 if (B.used + 2 * (size_t)r.n > B.cap) return set_err(h, FBP_E_UNSUPPORTED, "batch table overflow");
-                for (uint32_t k = 0; k < r.n; k++) {
-                    SegLaunch e = B.host[B.used + k];
-                    e.blk0 = r.nb_sel;
-                    e.nblk = e.n;
-                    r.nb_sel += e.n;
-                    B.host[B.used + r.n + k] = e;
+                // the same items, one block per window, grouped by selection class
+                size_t o = B.used + r.n;
+                for (int c = 0; c < kSelClasses; c++) {
+                    r.so[c] = o;
+                    for (uint32_t k = 0; k < r.n; k++) {
+                        SegLaunch e = B.host[B.used + k];
+                        if (scls[e.seg] != c) continue;
+                        e.blk0 = r.snb[c];
+                        e.nblk = e.n;
+                        r.snb[c] += e.n;
+                        r.sn[c]++;
+                        B.host[o++] = e;
+                    }
                 }
                 B.used += 2 * (size_t)r.n;
                 rounds.push_back(r);
@@ -1697,9 +1745,28 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         B.used += n_nx;
         if (B.used > B.cap) return set_err(h, FBP_E_UNSUPPORTED, "batch table overflow");
         HIP_TRY(upload(B), "copy batch tables");
-        for (const Round& r : rounds)
-            launch_seg_screen(L, bb.d_desc, B.dev + r.o, r.n, r.nb_corr, bb.eob[0], lds_corr, r.nb_sel, bb.eob[1], bd_sel,
-                              lds_sel, bb.Bbuf, bstride, s);
+        for (const Round& r : rounds) {
+            launch_seg_screen_corr(L, bb.d_desc, B.dev + r.o, r.n, r.nb_corr, bb.eob[0], lds_corr, bb.Bbuf, bstride, s);
+            // the classes side by side (class 0 on the batch's stream, the others on
+            // side streams), joined before the next round's correlations
+            HIP_TRY(hipEventRecord(side.fork, s), "event");
+            size_t eo = 0;  // each class's block-to-entry map in its own part of eob[1]
+            for (int c = 0; c < kSelClasses; c++) {
+                if (r.sn[c] == 0) continue;
+                hipStream_t sc = s;
+                if (c > 0) {
+                    sc = side.st[c];
+                    HIP_TRY(hipStreamWaitEvent(sc, side.fork, 0), "stream wait");
+                }
+                launch_seg_select(L, bb.d_desc, B.dev + r.so[c], r.sn[c], r.snb[c], bb.eob[1] + eo, cls_bd[c], cls_lds[c],
+                                  bb.Bbuf, bstride, sc);
+                eo += r.snb[c];
+                if (c > 0) {
+                    HIP_TRY(hipEventRecord(side.ev[c], sc), "event");
+                    HIP_TRY(hipStreamWaitEvent(s, side.ev[c], 0), "stream wait");
+                }
+            }
+        }
         if (recheck)
             launch_seg_pass_kernel(SEG_REASSIGN, L, bb.d_desc, bb.d_dyn, B.dev + o_re, n_re, nb_re, bb.eob[0], 0, 0, s);
         launch_seg_pass_kernel(SEG_DECIDE, L, bb.d_desc, bb.d_dyn, B.dev + o_dec, n_dec, nb_dec, bb.eob[0], 0, 0, s);

@@ -1683,33 +1683,40 @@ bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* 
 
 // corr: entries' nblk blocks each (grid-stride over their windows); select: one
 // block per window (nblk == n), bd threads.
-void launch_seg_screen(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks_corr,
-                       uint16_t* eob_corr, size_t lds_corr, uint32_t nblocks_sel, uint16_t* eob_sel, int bd,
-                       size_t lds_sel, float* Bbuf, size_t bstride, hipStream_t s)
+void launch_seg_screen_corr(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
+                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s)
 {
     if (ntab == 0) return;
     const unsigned ge = std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024));
-    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob_corr);
-    // the select table: the same entries with one block per window
-    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab + ntab, ntab, eob_sel);
-    const dim3 gc(nblocks_corr), gs(nblocks_sel), bc(kBlock), bs(bd);
-    const SegLaunch* ts2 = tab + ntab;
+    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    const dim3 gc(nblocks), bc(kBlock);
+    if (L == 48) hipLaunchKernelGGL(k_screen_corr_seg<48>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
+    else if (L == 64) hipLaunchKernelGGL(k_screen_corr_seg<64>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
+    else if (L == 32) hipLaunchKernelGGL(k_screen_corr_seg<32>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
+    else hipLaunchKernelGGL(k_screen_corr_seg<0>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
+}
+
+// The selection of one class of segments: one block (bd threads, lds bytes of
+// dynamic LDS, the class's largest) per flagged window of the entries.
+void launch_seg_select(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks, uint16_t* eob,
+                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s)
+{
+    if (ntab == 0 || nblocks == 0) return;
+    const unsigned ge = std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024));
+    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    const dim3 gs(nblocks), bs(bd);
     if (L == 48) {
-        hipLaunchKernelGGL(k_screen_corr_seg<48>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
-        allow_lds(k_screen_select_reg_seg<48>, lds_sel);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<48>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<48>, lds);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<48>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
     } else if (L == 64) {
-        hipLaunchKernelGGL(k_screen_corr_seg<64>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
-        allow_lds(k_screen_select_reg_seg<64>, lds_sel);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<64>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<64>, lds);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<64>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
     } else if (L == 32) {
-        hipLaunchKernelGGL(k_screen_corr_seg<32>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
-        allow_lds(k_screen_select_reg_seg<32>, lds_sel);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<32>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<32>, lds);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<32>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
     } else {
-        hipLaunchKernelGGL(k_screen_corr_seg<0>, gc, bc, lds_corr, s, sd, tab, eob_corr, Bbuf, bstride);
-        allow_lds(k_screen_select_reg_seg<0>, lds_sel);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<0>, gs, bs, lds_sel, s, sd, ts2, eob_sel, Bbuf, bstride);
+        allow_lds(k_screen_select_reg_seg<0>, lds);
+        hipLaunchKernelGGL(k_screen_select_reg_seg<0>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
     }
 }
 

@@ -29,10 +29,10 @@ def _separate(seg):
     return bpp.binary_pursuit(seg["voltage"], np.float32, seg["sort_info"], m, n_max_shift_inds=seg["n_max_shift_inds"])
 
 
-@pytest.mark.parametrize("mode,workers", [("fused", 1), ("streams", 1), ("streams", 4)])
-def test_batch_equals_separate_calls(mode, workers):
+@pytest.mark.parametrize("mode,workers,lanes", [("fused", 1, 1), ("fused", 1, 3), ("streams", 1, 1), ("streams", 4, 1)])
+def test_batch_equals_separate_calls(mode, workers, lanes):
     segs = _segments(10, 11)
-    out = batch.binary_pursuit_batch(segs, n_workers=workers, mode=mode)
+    out = batch.binary_pursuit_batch(segs, n_workers=workers, mode=mode, lanes=lanes)
     for seg, got in zip(segs, out):
         ref = _separate(seg)
         assert len(got[0]) == len(ref[0])
@@ -69,6 +69,17 @@ def test_autotune_picks_a_candidate():
     best, rates = batch.autotune_workers(segs, candidates=(1, 2, 4), sample=6)
     assert best in (1, 2, 4) and set(rates) == {1, 2, 4}
     assert all(r > 0 for r in rates.values())
+
+
+def test_lane_autotune_picks_a_candidate_and_lanes_give_the_same_spikes():
+    segs = _segments(8, 16)
+    best, rates = batch.autotune_lanes(segs, candidates=(1, 2, 4), sample=8)
+    assert best in (1, 2, 4) and set(rates) == {1, 2, 4}
+    ref = batch.binary_pursuit_batch(segs, lanes=1)
+    got = batch.binary_pursuit_batch(segs, lanes=4)
+    for a, b in zip(ref, got):
+        np.testing.assert_array_equal(np.asarray(a[0]), np.asarray(b[0]))
+        np.testing.assert_array_equal(np.asarray(a[1]), np.asarray(b[1]))
 
 
 def test_tile_autotune_and_every_shape_gives_the_same_spikes():

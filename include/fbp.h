@@ -80,6 +80,9 @@ typedef struct fbp_stats {
                                           had no channel change in the window's samples since their last
                                           computation and kept their (identical) values */
     int64_t n_likelihood_mfma_launches; /* launches of the matrix-core likelihood pass (mixed precision) */
+    double ms_rescore;            /* deferred exact rescoring of the overlap candidates (part of ms_recheck),
+                                     with FBP_FLAG_TIMING */
+    int64_t n_rescore_launches;
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and

@@ -70,6 +70,8 @@ class FbpStats(ctypes.Structure):
         ("n_select_launches", ctypes.c_int64),
         ("likelihood_pairs_computed", ctypes.c_int64),
         ("n_likelihood_mfma_launches", ctypes.c_int64),
+        ("ms_rescore", ctypes.c_double),
+        ("n_rescore_launches", ctypes.c_int64),
     ]
 
     def as_dict(self):

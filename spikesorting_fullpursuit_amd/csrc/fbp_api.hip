@@ -64,8 +64,10 @@ size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
                         float* Bbuf, hipStream_t s);
-void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
-                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s);
+bool launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s, uint32_t* xcand);
+void launch_screen_rescore(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                           uint32_t nbatch, const uint32_t* xcand, hipStream_t s);
 // MFMA likelihood tables (fbp_likmfma.hip).
 constexpr int kMfMaxBlk = 32;
 constexpr double kMfPadMax = 2.5;  // tile channel rows per template channel row up to which the matrix cores pay
@@ -253,7 +255,7 @@ struct fbp_handle {
     int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand;
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
@@ -404,7 +406,7 @@ struct EventPair {
         if (b) (void)hipEventDestroy(b);
     }
 };
-enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT };
+enum { TM_LIK = 0, TM_MAX, TM_RECHECK, TM_PURSUIT, TM_RESID, TM_OTHER, TM_SELECT, TM_RESCORE };
 
 static ScreenTables tables_view(const fbp_handle* h)
 {
@@ -1298,6 +1300,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
     screen_mf = screen_mf && use_screen;
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
+    if (use_screen) HIP_TRY(h->xcand.ensure((size_t)screen_batch * kXStride * 4), "alloc rescoring records");
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -1369,10 +1372,17 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                             launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
                         }
                         tm.begin(TM_SELECT);
-                        launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(),
-                                             (flags & FBP_FLAG_SELECT_STREAMING) != 0, s);
+                        const bool deferred = launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(),
+                                                                   (flags & FBP_FLAG_SELECT_STREAMING) != 0, s,
+                                                                   h->xcand.as<uint32_t>());
                         tm.end(TM_SELECT);
                         stats.n_select_launches++;
+                        if (deferred) {
+                            tm.begin(TM_RESCORE);
+                            launch_screen_rescore(ts, st, tb, f0, nb, h->xcand.as<uint32_t>(), s);
+                            tm.end(TM_RESCORE);
+                            stats.n_rescore_launches++;
+                        }
                     }
                 } else {
                     HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");
@@ -1433,6 +1443,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     stats.ms_residual = tm.total(TM_RESID);
     stats.ms_other = tm.total(TM_OTHER);
     stats.ms_select = tm.total(TM_SELECT);
+    stats.ms_rescore = tm.total(TM_RESCORE);
     HIP_TRY(hipMemcpy(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost), "read counters");
     stats.recheck_items = h->h_counters[CNT_EVAL];
     stats.recheck_overflow = h->h_counters[CNT_CAND];

@@ -9,6 +9,7 @@ constexpr int kBlock = 256;          // threads per block for the streaming kern
 constexpr int kCompactTile = 1024;   // elements per block in ordered compaction
 constexpr uint32_t kNoItem = 0xffffffffu;
 constexpr int kGB = 8;                 // shifts f per block of the blocked Gram minima
+constexpr int kXStride = 65;           // u32 per flagged window of the deferred-rescoring record
 
 // Device view of the template set.  Rows of the compact layout are the
 // non-zero channels of each template, ascending, zero padded to Lp floats.

@@ -756,6 +756,13 @@ __global__ __launch_bounds__(kBlock) void k_screen_select(TemplateSet ts, PassSt
 // the three phases.  Each wave enumerates the shifts f of its own surviving
 // pairs one pair at a time, so no phase needs a block-wide list.
 // ---------------------------------------------------------------------------
+// Deferred exact rescoring (k_screen_rescore): a window whose candidates fit
+// one wave (at most 64, of at most kRU units, kCV channels with the label's)
+// leaves them in a per-window record [count, keys...]; every other window is
+// finished by the selection kernel itself and marks its record kXDone.
+constexpr uint32_t kXDone = 0xffffffffu;
+constexpr int kRU = 3;
+constexpr int kCV = 32;
 constexpr int kSelR = 24;           // pairs per thread (bits of a 32-bit mask)
 constexpr int kQR = 2;              // (pair, f-quad) items per thread per round of G loads
 constexpr int kSelLR = 12;          // pairs per round of the per-pair loads (divides kSelR)
@@ -827,7 +834,8 @@ __device__ __forceinline__ void apply_parse_at(const TemplateSet& ts, const Pass
 template <int LT>
 __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState st, ScreenTables tb,
                                                                     uint32_t f0, uint32_t nbatch,
-                                                                    const float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx)
+                                                                    const float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx,
+                                                                    uint32_t* __restrict__ xcand)
 {
     extern __shared__ float sm[];
     __shared__ float red[kSelMaxBlock / 64];
@@ -837,7 +845,8 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     __shared__ uint32_t ncand_s;
     __shared__ int plo_sh, phi_sh;  // first / last f with LIK[l][b+f-n] > 0 (S, -1: none)
     __shared__ int pfirst_sh, npl_sh;  // pair ids of the label: pfirst .. pfirst + npl - 1
-    __shared__ uint32_t nsurv_s, nsurvp_s, nrow_s;
+    __shared__ uint32_t nsurv_s, nsurvp_s, nrow_s, nresc_s;
+    __shared__ unsigned long long cun_s;  // channels of the candidates' units (deferred rescoring)
     __shared__ float Mp_sh;  // max over the enumerated shared-channel items of E-hat - delta_u
     const int C = ts.C, U = ts.U;
     const int L = LT > 0 ? LT : ts.L;
@@ -913,7 +922,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
         __syncthreads();
         if (lik_b < thr_l || b >= N - L) {
-            if (tid == 0) st.orc[id] = 0;
+            if (tid == 0) {
+                st.orc[id] = 0;
+                if (xcand != nullptr) xcand[(size_t)wi * kXStride] = kXDone;
+            }
             return;
         }
         const float* Bw = Bbuf + (size_t)wi * bstride;
@@ -1397,6 +1409,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         uint32_t bk = kNoItem;
         if (__syncthreads_or(overflow)) {
             // more candidates than the list holds: enumerate and rescore in place
+            if (xcand != nullptr && tid == 0) xcand[(size_t)wi * kXStride] = kXDone;
             for (int e = tid; e < US; e += bd) {
                 int u, s;
                 divmod_small(e, S, invS, u, s);
@@ -1421,21 +1434,35 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             // the zero-padded rows of their units are staged in LDS, upr units
             // per round
             for (int u = tid; u < U; u += bd) slot_of[u] = -1;
-            if (tid == 0) nrow_s = 0;
+            if (tid == 0) { nrow_s = 0; nresc_s = 0; cun_s = 0ull; }
             __syncthreads();
+            uint32_t* xw = xcand != nullptr ? xcand + (size_t)wi * kXStride : nullptr;
             for (uint32_t i = tid; i < nc; i += bd) {
                 const int u = (int)(cand_k[i] % (uint32_t)U);
                 const float du = Dp[u].x;
                 if (cand_e[i] < fmaxf(Mp - du, bl - du)) continue;
+                if (xw != nullptr) {
+                    const uint32_t q = atomicAdd(&nresc_s, 1u);
+                    if (q < 64u) xw[1 + q] = cand_k[i];
+                }
                 if (atomicCAS(&slot_of[u], -1, -2) == -1) {
                     const int o = (int)atomicAdd(&nrow_s, 1u);
                     slot_nr[o] = ts.nch[u];
                     slot_off[o] = ts.off[u];
                     slot_of[u] = o;
+                    if (xw != nullptr && C <= 64) atomicOr(&cun_s, (unsigned long long)ts.cmask[u]);
                 }
             }
             __syncthreads();
             const int nun = (int)nrow_s;
+            if (xw != nullptr) {
+                // the candidates fit one wave of the rescoring kernel: it finishes the window
+                const uint32_t nr = nresc_s;
+                const bool defer = nr >= 1u && nr <= 64u && nun <= kRU && C <= 64 &&
+                                   __popcll(cun_s | ts.cmask[lab]) <= kCV;
+                if (tid == 0) xw[0] = defer ? nr : kXDone;
+                if (defer) return;  // uniform over the block
+            }
             const int upr = max(1, srows / CM);
             const float invCM = 1.0f / (float)CM;
             for (int r0u = 0; r0u < nun; r0u += upr) {
@@ -1505,9 +1532,153 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
 template <int LT>
 __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg(TemplateSet ts, PassState st, ScreenTables tb,
                                                                     uint32_t f0, uint32_t nbatch,
-                                                                    const float* __restrict__ Bbuf, size_t bstride)
+                                                                    const float* __restrict__ Bbuf, size_t bstride,
+                                                                    uint32_t* __restrict__ xcand)
 {
-    screen_select_reg_body<LT>(ts, st, tb, f0, nbatch, Bbuf, bstride, blockIdx.x, gridDim.x);
+    screen_select_reg_body<LT>(ts, st, tb, f0, nbatch, Bbuf, bstride, blockIdx.x, gridDim.x, xcand);
+}
+
+// ---------------------------------------------------------------------------
+// Deferred exact rescoring: one wave per window whose selection left its
+// candidates (at most 64, of at most kRU units) in its record.  The wave stages
+// the label's and the units' zero-padded rows and the window's samples of
+// their channels (compact slots, ascending channel order), runs one exact FP32
+// chain per candidate (exact_item_padded, the order of the literal kernel),
+// takes the arg-max (value, then lowest item id) and applies the parse cascade:
+// the same result as the selection kernel's own rescoring.
+// ---------------------------------------------------------------------------
+template <int LT>
+__global__ __launch_bounds__(64) void k_screen_rescore(TemplateSet ts, PassState st, ScreenTables tb, uint32_t f0,
+                                                       uint32_t nbatch, const uint32_t* __restrict__ xcand)
+{
+    extern __shared__ __align__(16) float sx[];
+    const uint32_t wi = blockIdx.x, w = f0 + wi;
+    if (wi >= nbatch || w >= st.counters[CNT_F]) return;
+    const uint32_t* xw = xcand + (size_t)wi * kXStride;
+    const uint32_t cnt = xw[0];
+    if (cnt == kXDone || cnt == 0u || cnt > 64u) return;  // uniform over the block
+    const int lane = (int)threadIdx.x;
+    const int U = ts.U, C = ts.C;
+    const int L = LT > 0 ? LT : ts.L;
+    const int S = tb.S, n = tb.n, CM = tb.cmax;
+    const int LP = L + 2 * n, VS = L | 1;
+    float* TlP = sx;                      // [CM][LP] the label's zero-padded rows
+    float* zrow = TlP + CM * LP;          // [LP]
+    float* Tu = zrow + LP;                // [kRU][CM][LP] the units' zero-padded rows
+    float* vW = Tu + kRU * CM * LP;       // [kCV][VS] window samples of the slotted channels
+    int* clS = reinterpret_cast<int*>(vW + kCV * VS);  // [CM] slots of the label's channels
+    int* cuS = clS + CM;                  // [kRU][CM] slots of each unit's channels
+    int* uid = cuS + kRU * CM;            // [kRU] unit of each staging slot
+    const uint32_t id = st.F[w];
+    const uint32_t lab = st.best_lab[id];
+    const int64_t b = st.best_idx[id];
+    const float bl = st.best_lik[id];
+    const int nl = ts.nch[lab], offl = ts.off[lab];
+    const uint32_t key = lane < (int)cnt ? xw[1 + lane] : kNoItem;
+    const int u = key != kNoItem ? (int)(key % (uint32_t)U) : -1;
+    const uint32_t r = key != kNoItem ? key / (uint32_t)U : 0u;
+    const int f = (int)(r % (uint32_t)S), s = (int)(r / (uint32_t)S);
+    // distinct units in lane order: the first lane of each takes a staging slot
+    bool first = u >= 0;
+    for (int j = 0; j < 64; j++) {
+        const int uj = __shfl(u, j, 64);
+        if (j < lane && uj == u) first = false;
+    }
+    const uint64_t fm = __ballot(first);
+    const int nun = min(__popcll(fm), kRU);
+    if (first) {
+        const int o = __popcll(fm & ((1ull << lane) - 1ull));
+        if (o < kRU) uid[o] = u;
+    }
+    // channel slots: the label's and the units' channels, ascending
+    unsigned long long m = u >= 0 ? (unsigned long long)ts.cmask[u] : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off, 64);
+    m |= (unsigned long long)ts.cmask[lab];
+    __syncthreads();
+    // the zero padding of every staged row first (LDS stores only), then the
+    // rows' L samples and the window's samples, eight loads in flight per lane
+    const int nrow_u = nun;  // units staged
+    for (int e = lane; e < (1 + kRU) * CM * LP + LP; e += 64) TlP[e] = 0.0f;  // TlP, zrow, Tu are contiguous
+    if (lane < nl) clS[lane] = __popcll(m & ((1ull << ts.chan[offl + lane]) - 1ull));
+    for (int o = 0; o < nrow_u; o++) {
+        const int uo = uid[o], ro = ts.off[uo];
+        if (lane < ts.nch[uo]) cuS[o * CM + lane] = __popcll(m & ((1ull << ts.chan[ro + lane]) - 1ull));
+    }
+    __syncthreads();
+    {
+        // template samples: rows of the label, then of each staged unit; element e = row * L + i
+        int rows_tot = nl, row_base[kRU + 1];
+        row_base[0] = nl;
+        for (int o = 0; o < nrow_u; o++) {
+            rows_tot += ts.nch[uid[o]];
+            row_base[o + 1] = rows_tot;
+        }
+        const int tot = rows_tot * L;
+        for (int e0 = lane; e0 < tot; e0 += 8 * 64) {
+            float v[8];
+            int dst[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = e0 + q * 64;
+                dst[q] = -1;
+                v[q] = 0.0f;
+                if (e < tot) {
+                    const int row = e / L, i = e - row * L;
+                    int src_row, d0;
+                    if (row < nl) {
+                        src_row = offl + row;
+                        d0 = row * LP;
+                    } else {
+                        int o = 0;
+                        while (o < nrow_u - 1 && row >= row_base[o + 1]) o++;
+                        const int k = row - row_base[o];
+                        src_row = ts.off[uid[o]] + k;
+                        d0 = (int)(Tu - TlP) + o * CM * LP + k * LP;
+                    }
+                    dst[q] = d0 + n + i;
+                    v[q] = ts.rows[(size_t)src_row * ts.Lp + i];
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (dst[q] >= 0) TlP[dst[q]] = v[q];
+        }
+    }
+    {
+        // the window's C x L samples are one contiguous run of the sample-major voltage
+        const float* src = st.V + (size_t)b * C;
+        const int tot = C * L;
+        for (int e0 = lane; e0 < tot; e0 += 8 * 64) {
+            float v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = e0 + q * 64;
+                v[q] = src[e < tot ? e : 0];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int e = e0 + q * 64;
+                if (e >= tot) continue;
+                const int j = e / C, c = e - j * C;
+                if ((m >> c) & 1ull) vW[__popcll(m & ((1ull << c) - 1ull)) * VS + j] = v[q];
+            }
+        }
+    }
+    __syncthreads();
+    float v = 0.0f;
+    uint32_t k = kNoItem;
+    if (u >= 0) {
+        int o = 0;
+        while (o < kRU - 1 && uid[o] != u) o++;
+        v = exact_item_padded<LT>(vW, L, VS, TlP, clS, nl, Tu + (size_t)o * CM * LP, cuS + o * CM, ts.nch[u], zrow, LP,
+                                  f, s, n);
+        k = key;
+    }
+    wave_argmax(v, k);
+    if (lane == 0) apply_parse_at(ts, st, n, id, v, k, bl, lab, b, S);
 }
 
 // ---------------------------------------------------------------------------
@@ -1583,26 +1754,53 @@ static void allow_lds(K kernel, size_t bytes)
 // The register-resident selection: one block per flagged window of the batch.
 template <class K>
 static void launch_select_reg(K kernel, int bd, size_t lds, uint32_t nbatch, hipStream_t s, const TemplateSet& ts,
-                              const PassState& st, const ScreenTables& tb, uint32_t f0, const float* Bbuf)
+                              const PassState& st, const ScreenTables& tb, uint32_t f0, const float* Bbuf,
+                              uint32_t* xcand)
 {
     allow_lds(kernel, lds);
-    hipLaunchKernelGGL(kernel, dim3(nbatch), dim3(bd), lds, s, ts, st, tb, f0, nbatch, Bbuf, (size_t)ts.U * tb.S);
+    hipLaunchKernelGGL(kernel, dim3(nbatch), dim3(bd), lds, s, ts, st, tb, f0, nbatch, Bbuf, (size_t)ts.U * tb.S,
+                       xcand);
 }
 
-void launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
-                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s)
+// Dynamic LDS of the deferred rescoring; 0 when it cannot take this template
+// set (more than 64 channels, or rows that do not fit 64 KiB).
+size_t screen_rescore_lds(const TemplateSet& ts, const ScreenTables& tb)
 {
-    if (nbatch == 0) return;
+    if (ts.C > 64) return 0;
+    const size_t LP = (size_t)ts.L + 2 * (size_t)tb.n, VS = (size_t)(ts.L | 1), CM = (size_t)tb.cmax;
+    const size_t bytes = (CM * LP * (1 + kRU) + LP + (size_t)kCV * VS + CM * (1 + kRU) + kRU) * 4;
+    return bytes <= 64 * 1024 ? bytes : 0;
+}
+
+void launch_screen_rescore(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                           uint32_t nbatch, const uint32_t* xcand, hipStream_t s)
+{
+    const size_t lds = screen_rescore_lds(ts, tb);
+    if (nbatch == 0 || lds == 0) return;
+    const dim3 g(nbatch), b(64);
+    if (ts.L == 48) hipLaunchKernelGGL(k_screen_rescore<48>, g, b, lds, s, ts, st, tb, f0, nbatch, xcand);
+    else if (ts.L == 64) hipLaunchKernelGGL(k_screen_rescore<64>, g, b, lds, s, ts, st, tb, f0, nbatch, xcand);
+    else if (ts.L == 32) hipLaunchKernelGGL(k_screen_rescore<32>, g, b, lds, s, ts, st, tb, f0, nbatch, xcand);
+    else hipLaunchKernelGGL(k_screen_rescore<0>, g, b, lds, s, ts, st, tb, f0, nbatch, xcand);
+}
+
+// Returns whether the selection left windows to the deferred rescoring (xcand
+// given, the register-resident selection taken and the rescoring's LDS fits).
+bool launch_screen_select(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0,
+                          uint32_t nbatch, const float* Bbuf, bool streaming, hipStream_t s, uint32_t* xcand)
+{
+    if (nbatch == 0) return false;
     const int64_t owners = (int64_t)sel_slices(ts.U) * tb.S;  // threads owning pairs
     if (!streaming && owners <= kSelMaxBlock &&
         screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max) <= kSelMaxLds) {
         const int bd = (int)std::max<int64_t>(kBlock, (owners + 63) / 64 * 64);
         const size_t lds = screen_select_reg_lds(ts, tb.S, tb.cmax, tb.npl_max);
-        if (ts.L == 48) launch_select_reg(k_screen_select_reg<48>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
-        else if (ts.L == 64) launch_select_reg(k_screen_select_reg<64>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
-        else if (ts.L == 32) launch_select_reg(k_screen_select_reg<32>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
-        else launch_select_reg(k_screen_select_reg<0>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf);
-        return;
+        uint32_t* xc = screen_rescore_lds(ts, tb) > 0 ? xcand : nullptr;
+        if (ts.L == 48) launch_select_reg(k_screen_select_reg<48>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf, xc);
+        else if (ts.L == 64) launch_select_reg(k_screen_select_reg<64>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf, xc);
+        else if (ts.L == 32) launch_select_reg(k_screen_select_reg<32>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf, xc);
+        else launch_select_reg(k_screen_select_reg<0>, bd, lds, nbatch, s, ts, st, tb, f0, Bbuf, xc);
+        return xc != nullptr;
     }
     const dim3 g(sgrid(nbatch, 1, 1u << 20)), b(kBlock);
     const size_t lds2 = screen_select_lds(ts, tb.S, tb.cmax);
@@ -1619,6 +1817,7 @@ void launch_screen_select(const TemplateSet& ts, const PassState& st, const Scre
         allow_lds(k_screen_select<0>, lds2);
         hipLaunchKernelGGL(k_screen_select<0>, g, b, lds2, s, ts, st, tb, f0, nbatch, Bbuf);
     }
+    return false;
 }
 
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -1668,7 +1867,7 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg_seg(const Se
     const SegLaunch e = tab[eob[blockIdx.x]];
     const SegDesc& d = sd[e.seg];
     screen_select_reg_body<LT>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
-                               blockIdx.x - e.blk0, e.nblk);
+                               blockIdx.x - e.blk0, e.nblk, nullptr);
 }
 
 // Whether the register-resident selection takes this segment, and its block

@@ -286,8 +286,12 @@ struct fbp_handle {
     // host <-> device staging of the voltage (created on first use)
     float* pinned[kStageBufs] = {};
     DevBuf dstage[kStageBufs];
-    hipEvent_t stage_ev[kStageBufs] = {};
+    hipEvent_t stage_ev[kStageBufs] = {};  // the copy that read pinned[b] is done
+    hipEvent_t stage_tr[kStageBufs] = {};  // the transpose that read dstage[b] is done
+    hipEvent_t stage_mark = nullptr;
     bool stage_busy[kStageBufs] = {};
+    bool stage_tr_busy[kStageBufs] = {};
+    hipStream_t cstream = nullptr;         // host-to-device copies of the voltage (default priority)
     std::unique_ptr<CopyPool> pool;
 };
 
@@ -764,7 +768,10 @@ static int ensure_staging(fbp_handle* h)
             HIP_TRY(hipHostMalloc((void**)&h->pinned[b], kStageBytes, hipHostMallocDefault), "pinned staging");
         HIP_TRY(h->dstage[b].ensure(kStageBytes), "alloc device staging");
         if (!h->stage_ev[b]) HIP_TRY(hipEventCreateWithFlags(&h->stage_ev[b], hipEventDisableTiming), "event");
+        if (!h->stage_tr[b]) HIP_TRY(hipEventCreateWithFlags(&h->stage_tr[b], hipEventDisableTiming), "event");
     }
+    if (!h->stage_mark) HIP_TRY(hipEventCreateWithFlags(&h->stage_mark, hipEventDisableTiming), "event");
+    if (!h->cstream) HIP_TRY(hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking), "copy stream");
     return FBP_OK;
 }
 
@@ -787,23 +794,41 @@ static void pool_copy_rows(CopyPool& pool, float* dst, int64_t dld, const float*
 
 // Host voltage [C][N] (row stride ld) into the sample-major device copy dst [N][C],
 // through h's staging buffers on h's stream (C: h's channel count unless given).
-static int stage_in(fbp_handle* h, const float* V, int64_t ld, int64_t N, float* dst, int64_t Cg = -1)
+// Host voltage V [C][N] (row stride ld) into the sample-major device copy dst,
+// piece by piece: the copies run on the copy stream, each piece's transpose on
+// the handle's stream after its copy; landed(x) (when given) is called after
+// the transpose of every piece with the samples [0, x) in place, to queue work
+// on them on the handle's stream while the later pieces are still copied.
+static int stage_in(fbp_handle* h, const float* V, int64_t ld, int64_t N, float* dst, int64_t Cg = -1,
+                    const std::function<int(int64_t)>& landed = nullptr, bool marked = false)
 {
     int rc = ensure_staging(h);
     if (rc != FBP_OK) return rc;
     const int64_t C = Cg > 0 ? Cg : h->C, P = stage_piece(C);
-    hipStream_t s = h->stream;
+    hipStream_t s = h->stream, cs = h->cstream;
+    // the copies come after everything queued on the handle's stream (up to
+    // the caller's own record of stage_mark when `marked`)
+    if (!marked) HIP_TRY(hipEventRecord(h->stage_mark, s), "event");
+    HIP_TRY(hipStreamWaitEvent(cs, h->stage_mark, 0), "stream wait");
     int64_t k = 0;
     for (int64_t x0 = 0; x0 < N; x0 += P, k++) {
         const int64_t np = std::min(P, N - x0);
         const int b = (int)(k % kStageBufs);
         if (h->stage_busy[b]) HIP_TRY(hipEventSynchronize(h->stage_ev[b]), "staging wait");
         pool_copy_rows(*h->pool, h->pinned[b], np, V + x0, ld, C, np);
-        HIP_TRY(hipMemcpyAsync(h->dstage[b].p, h->pinned[b], (size_t)C * np * 4, hipMemcpyHostToDevice, s),
+        if (h->stage_tr_busy[b]) HIP_TRY(hipStreamWaitEvent(cs, h->stage_tr[b], 0), "stream wait");
+        HIP_TRY(hipMemcpyAsync(h->dstage[b].p, h->pinned[b], (size_t)C * np * 4, hipMemcpyHostToDevice, cs),
                 "copy voltage");
-        HIP_TRY(hipEventRecord(h->stage_ev[b], s), "event");
+        HIP_TRY(hipEventRecord(h->stage_ev[b], cs), "event");
         h->stage_busy[b] = true;
+        HIP_TRY(hipStreamWaitEvent(s, h->stage_ev[b], 0), "stream wait");
         launch_to_sample_major(h->dstage[b].as<float>(), np, dst + x0 * C, C, np, s);
+        HIP_TRY(hipEventRecord(h->stage_tr[b], s), "event");
+        h->stage_tr_busy[b] = true;
+        if (landed) {
+            rc = landed(x0 + np);
+            if (rc != FBP_OK) return rc;
+        }
     }
     return FBP_OK;
 }
@@ -1034,13 +1059,16 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
-                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask};
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
         h->dstage[b].release();
         if (h->pinned[b]) (void)hipHostFree(h->pinned[b]);
         if (h->stage_ev[b]) (void)hipEventDestroy(h->stage_ev[b]);
+        if (h->stage_tr[b]) (void)hipEventDestroy(h->stage_tr[b]);
     }
+    if (h->stage_mark) (void)hipEventDestroy(h->stage_mark);
+    if (h->cstream) (void)hipStreamDestroy(h->cstream);
     h->pool.reset();
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1237,11 +1265,47 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     HIP_TRY(hipEventRecord(ev0, s), "event");
 
     PassState st = pass_view(h);
+    if (!v_on_device) {
+        // the voltage copies need not wait for the state fills below
+        int rc = ensure_staging(h);
+        if (rc != FBP_OK) return rc;
+        HIP_TRY(hipEventRecord(h->stage_mark, s), "event");
+    }
+    // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
+    // best_* zeros, flags zeros, window list 0..nwin-1).
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
+    launch_fill_u32(st.best_idx, 0u, nwin, s);
+    launch_fill_u32(st.best_lab, 0u, nwin, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(st.best_lik), 0u, nwin, s);
+    launch_fill_u32(st.obest_idx, 0u, nwin, s);
+    launch_fill_u32(st.obest_lab, 0u, nwin, s);
+    launch_fill_u8(st.check, 0, nwin, s);
+    launch_fill_u8(st.orc, 0, nwin, s);
+    launch_fill_u8(st.giveup, 0, nwin, s);
+    launch_fill_u32(st.counters, 0u, CNT_N, s);
+    if (st.dirty) launch_fill_u32(reinterpret_cast<uint32_t*>(st.dirty), 0u, 2 * nwin, s);
+    launch_iota(st.W, nwin, s);
+    // With host voltage and the vector-unit first pass, the first pass's
+    // likelihood runs on each staged piece's complete windows while the later
+    // pieces are still copied: window w reads the samples [wL, wL + 2L - 1).
+    const bool early_lik = !v_on_device && !mma && !mixed && !(flags & FBP_FLAG_FIRST_PASS_ONLY);
+    int64_t early_w = 0;  // windows whose first-pass likelihood is queued
+    auto early = [&](int64_t x) -> int {
+        const int64_t whi = x >= N ? nwin : (x >= 2 * L - 1 ? std::min<int64_t>(nwin, (x - 2 * L + 1) / L + 1) : 0);
+        if (whi <= early_w) return FBP_OK;
+        PassState sp = st;
+        sp.W = st.W + early_w;
+        tm.begin(TM_LIK);
+        launch_likelihood(ts, sp, (uint32_t)(whi - early_w), nullptr, s);
+        tm.end(TM_LIK);
+        early_w = whi;
+        return FBP_OK;
+    };
     // the residual is kept sample-major ([N][C]): a window is one contiguous block
     if (v_on_device) {
         launch_to_sample_major(Vsrc, ld, st.V, C, N, s);
     } else {
-        int rc = stage_in(h, Vsrc, ld, N, st.V);
+        int rc = stage_in(h, Vsrc, ld, N, st.V, -1, early_lik ? early : std::function<int(int64_t)>(), true);
         if (rc != FBP_OK) return rc;
     }
     if (mixed) launch_absmax(st.V, (int64_t)C * N, h->mf_mbits.as<uint32_t>(), h->mf_vs.as<float>(), s);
@@ -1268,20 +1332,6 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         mv.nbq = ldq / 16;
         launch_quantize(st.V, N, (int)C, mv, mma_req, s);
     }
-    // Initial state: binary_pursuit_parallel.py:528-655 (likelihood zeros,
-    // best_* zeros, flags zeros, window list 0..nwin-1).
-    launch_fill_u32(reinterpret_cast<uint32_t*>(st.LIK), 0u, U * N, s);
-    launch_fill_u32(st.best_idx, 0u, nwin, s);
-    launch_fill_u32(st.best_lab, 0u, nwin, s);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(st.best_lik), 0u, nwin, s);
-    launch_fill_u32(st.obest_idx, 0u, nwin, s);
-    launch_fill_u32(st.obest_lab, 0u, nwin, s);
-    launch_fill_u8(st.check, 0, nwin, s);
-    launch_fill_u8(st.orc, 0, nwin, s);
-    launch_fill_u8(st.giveup, 0, nwin, s);
-    launch_fill_u32(st.counters, 0u, CNT_N, s);
-    if (st.dirty) launch_fill_u32(reinterpret_cast<uint32_t*>(st.dirty), 0u, 2 * nwin, s);
-    launch_iota(st.W, nwin, s);
 
     fbp_stats stats{};
     int64_t nz_sum = 0;
@@ -1307,7 +1357,9 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     int64_t pairs_total = 0;
     h->pass_log.clear();
     for (;;) {
-        launch_fill_u32(st.counters + CNT_PAIRS, 0u, 1, s);
+        // (the staged first pass has counted its pairs since the counters were cleared)
+        const bool staged_pass = passes == 0 && early_lik;
+        if (!staged_pass) launch_fill_u32(st.counters + CNT_PAIRS, 0u, 1, s);
         stats.windows_listed += nW;
         stats.likelihood_positions += (int64_t)nW * L * U;
         stats.likelihood_flops += 2.0 * (double)nW * (double)L * (double)nz_sum * (double)L;
@@ -1332,7 +1384,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                                dirty, w0, std::min(lik_batch, nW - w0), s);
                 stats.n_likelihood_mfma_launches++;
             }
-        } else {
+        } else if (!staged_pass) {
             launch_likelihood(ts, st, nW, dirty, s);
         }
         launch_clear_dirty(st, nW, s);

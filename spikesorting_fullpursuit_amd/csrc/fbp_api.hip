@@ -35,6 +35,7 @@ void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_
 void launch_clear_dirty(const PassState& st, uint32_t nW, hipStream_t s);
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
+void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s);
 void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
                           uint32_t* blockbuf, hipStream_t s);
 void launch_next_windows(const PassState& st, uint32_t* blockbuf, hipStream_t s);
@@ -255,7 +256,7 @@ struct fbp_handle {
     int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand, Fs, lhist;
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
@@ -1059,7 +1060,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
-                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand};
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
         h->dstage[b].release();
@@ -1351,6 +1352,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     screen_mf = screen_mf && use_screen;
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
     if (use_screen) HIP_TRY(h->xcand.ensure((size_t)screen_batch * kXStride * 4), "alloc rescoring records");
+    if (use_screen) HIP_TRY(h->Fs.ensure((size_t)nwin * 4), "alloc flagged windows by label");
+    if (use_screen) HIP_TRY(h->lhist.ensure((size_t)U * 4), "alloc label counts");
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -1408,7 +1411,12 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             stats.windows_flagged += nF;
             if (nF > 0) {
                 tm.begin(TM_RECHECK);
+                // the screened recheck and the reassignment take the flagged windows grouped by label
+                PassState sf = st;
                 if (use_screen) {
+                    sf.F = h->Fs.as<uint32_t>();
+                    launch_sort_by_label(st, nF, U, sf.F, h->lhist.as<uint32_t>(), s);
+                    const PassState& st = sf;
                     ScreenTables tb = tables_view(h);
                     if (screen_mf) {
                         tb.kappa2 = h->scr_k2;
@@ -1444,7 +1452,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
                 tm.end(TM_RECHECK);
                 stats.n_recheck_launches++;
                 tm.begin(TM_OTHER);
-                launch_reassign(ts, st, nF, s);
+                launch_reassign(ts, sf, nF, s);
                 tm.end(TM_OTHER);
             }
         }

@@ -871,6 +871,72 @@ void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, 
                        st.wmax_v, st.wmax_p, st.W, nW, st);
 }
 
+// ---------------------------------------------------------------------------
+// Flagged windows grouped by the label of their best spike (counting sort,
+// order within a label unspecified): the overlap screen of consecutive blocks
+// then reads the same label's Gram rows and minima, which stay in the L2.  The
+// screen, selection and reassignment give every window the same result in any
+// order (each writes only its own window's state).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_label_hist(const uint32_t* __restrict__ F, const uint32_t* __restrict__ cnt_f,
+                                                       const uint32_t* __restrict__ lab, uint32_t* __restrict__ hist)
+{
+    const uint32_t nF = *cnt_f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nF; i += gridDim.x * blockDim.x)
+        atomicAdd(hist + lab[F[i]], 1u);
+}
+
+// hist[U] -> exclusive offsets in place (one block).
+__global__ __launch_bounds__(1024) void k_label_offsets(uint32_t* __restrict__ hist, int U)
+{
+    __shared__ uint32_t part[1024];
+    const int t = (int)threadIdx.x;
+    const int per = (U + 1023) / 1024;
+    uint32_t sum = 0;
+    for (int k = 0; k < per; k++) {
+        const int u = t * per + k;
+        if (u < U) sum += hist[u];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (int k = 0; k < per; k++) {
+        const int u = t * per + k;
+        if (u < U) {
+            const uint32_t h = hist[u];
+            hist[u] = run;
+            run += h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_label_scatter(const uint32_t* __restrict__ F, const uint32_t* __restrict__ cnt_f,
+                                                          const uint32_t* __restrict__ lab, uint32_t* __restrict__ off,
+                                                          uint32_t* __restrict__ out)
+{
+    const uint32_t nF = *cnt_f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nF; i += gridDim.x * blockDim.x) {
+        const uint32_t id = F[i];
+        out[atomicAdd(off + lab[id], 1u)] = id;
+    }
+}
+
+void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s)
+{
+    if (nF == 0) return;
+    launch_fill_u32(hist, 0u, U, s);
+    const unsigned g = (unsigned)std::min<uint32_t>((nF + kBlock - 1) / kBlock, 4096u);
+    hipLaunchKernelGGL(k_label_hist, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist);
+    hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U);
+    hipLaunchKernelGGL(k_label_scatter, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist, out);
+}
+
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s)
 {
     SelFlagged sel{st.W, st.orc, st.F};

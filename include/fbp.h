@@ -49,6 +49,9 @@ extern "C" {
  * (no exact rescoring, so decisions are no longer guaranteed to be those of FP32). */
 #define FBP_FLAG_FIRST_PASS_ONLY 1024u
 #define FBP_FLAG_NO_RESCORE 2048u
+/* with FBP_FLAG_PRECISION_BF16 / _FP8: the matrix cores take every pass (one template per tile over the listed
+ * windows), not only the dense first pass (later passes then run the FP32 kernel on the changed pairs) */
+#define FBP_FLAG_MMA_EVERY_PASS 4096u
 
 typedef struct fbp_handle fbp_handle;
 

@@ -30,6 +30,7 @@ FLAG_PRECISION_BF16 = 256
 FLAG_PRECISION_FP8 = 512
 FLAG_FIRST_PASS_ONLY = 1024
 FLAG_NO_RESCORE = 2048
+FLAG_MMA_EVERY_PASS = 4096
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (

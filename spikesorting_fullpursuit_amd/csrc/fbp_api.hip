@@ -98,6 +98,8 @@ void launch_quantize(const float* V, int64_t N, int C, const MmaVoltage& mv, int
 void launch_requantize(const float* V, int64_t N, int C, int L, const MmaVoltage& mv, int fmt, const uint32_t* sidx,
                        uint32_t nsp, hipStream_t s);
 void launch_vscale_to(const uint32_t* mbits, float* vs, int e_target, hipStream_t s);
+bool launch_likmma_dense(const TemplateSet& ts, const PassState& st, const MmaTables& mt, const MmaVoltage& mv, int fmt,
+                         uint8_t* gmask, hipStream_t s);
 void launch_likmma(const TemplateSet& ts, const PassState& st, const MmaTables& mt, const MmaVoltage& mv, int fmt,
                    uint32_t nW, uint8_t* gmask, const uint64_t* dirty, hipStream_t s);
 void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTables& mt, const void* fragl,
@@ -271,6 +273,8 @@ struct fbp_handle {
     int mma_KB = 0;
     float mma_kl = 0.0f, mma_kb = 0.0f, mma_ka = 0.0f, mma_kd = 0.0f;
     DevBuf mma_frag, mma_tsc, mma_T1, mma_scal, mma_q, mma_bmax, mma_gmask;
+    DevBuf mma_qtmp, mma_qch, mma_rslot;  // template quads of the dense first pass
+    int mma_nq = 0, mma_qc = 0;
     std::vector<float> h_rows;
     std::vector<int> h_off;
     // chunk state
@@ -684,6 +688,56 @@ static int build_mma_tables(fbp_handle* h, int fmt)
     h->mma_kd = (float)(2.0 * K * a * a * (1.0 + 2.0 * gK));
     h->mma_KB = KB;
     h->mma_fmt = fmt;
+    // quads of templates for the dense first pass: templates ordered by first
+    // channel, four at a time, each quad's channels (union, ascending) staged
+    // together; none when L is not a multiple of 16 or a quad spans more than
+    // kQuadCh channels
+    h->mma_nq = 0;
+    h->mma_qc = 0;
+    if (L % 16 == 0 && L <= 64) {
+        const int C = h->C;
+        std::vector<int> first(U, C), ord(U);
+        for (int u = 0; u < U; u++)
+            for (int c = 0; c < C; c++)
+                if (h->h_nz[(size_t)u * C + c]) { first[u] = c; break; }
+        for (int u = 0; u < U; u++) ord[u] = u;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return first[a] < first[b]; });
+        const int nq = (U + 3) / 4;
+        std::vector<int> qtmp((size_t)nq * 4, -1), qch((size_t)nq * kQuadCh, -1), rslot(std::max(Rr, 1), 0);
+        bool ok = true;
+        int qc = 0;
+        for (int q = 0; q < nq && ok; q++) {
+            std::vector<int> chs;
+            for (int j = 0; j < 4 && 4 * q + j < U; j++) {
+                const int u = ord[4 * q + j];
+                qtmp[(size_t)q * 4 + j] = u;
+                for (int c = 0; c < C; c++)
+                    if (h->h_nz[(size_t)u * C + c]) chs.push_back(c);
+            }
+            std::sort(chs.begin(), chs.end());
+            chs.erase(std::unique(chs.begin(), chs.end()), chs.end());
+            if ((int)chs.size() > kQuadCh) { ok = false; break; }
+            qc = std::max(qc, (int)chs.size());
+            for (size_t i = 0; i < chs.size(); i++) qch[(size_t)q * kQuadCh + i] = chs[i];
+            for (int j = 0; j < 4 && 4 * q + j < U; j++) {
+                const int u = ord[4 * q + j];
+                int k = 0;
+                for (int c = 0; c < C; c++)
+                    if (h->h_nz[(size_t)u * C + c])
+                        rslot[h->h_off[u] + k++] = (int)(std::lower_bound(chs.begin(), chs.end(), c) - chs.begin());
+            }
+        }
+        if (ok && qc > 0) {
+            HIP_TRY(h->mma_qtmp.ensure(qtmp.size() * 4), "alloc quads");
+            HIP_TRY(h->mma_qch.ensure(qch.size() * 4), "alloc quads");
+            HIP_TRY(h->mma_rslot.ensure(rslot.size() * 4), "alloc quads");
+            HIP_TRY(hipMemcpy(h->mma_qtmp.p, qtmp.data(), qtmp.size() * 4, hipMemcpyHostToDevice), "copy");
+            HIP_TRY(hipMemcpy(h->mma_qch.p, qch.data(), qch.size() * 4, hipMemcpyHostToDevice), "copy");
+            HIP_TRY(hipMemcpy(h->mma_rslot.p, rslot.data(), rslot.size() * 4, hipMemcpyHostToDevice), "copy");
+            h->mma_nq = nq;
+            h->mma_qc = qc;
+        }
+    }
     return FBP_OK;
 }
 
@@ -701,6 +755,11 @@ static MmaTables mma_view(const fbp_handle* h)
     mt.kb = h->mma_kb;
     mt.ka = h->mma_ka;
     mt.kd = h->mma_kd;
+    mt.qtmp = h->mma_qtmp.as<int>();
+    mt.qch = h->mma_qch.as<int>();
+    mt.rslot = h->mma_rslot.as<int>();
+    mt.nq = h->mma_nq;
+    mt.qc = h->mma_qc;
     return mt;
 }
 
@@ -1060,7 +1119,8 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
-                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist};
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist,
+                      &h->mma_qtmp, &h->mma_qch, &h->mma_rslot};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
         h->dstage[b].release();
@@ -1251,6 +1311,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     // 8-position groups to rescore fit one byte (L <= 64), else the FP32 kernel runs
     const int mma_req = (flags & FBP_FLAG_PRECISION_FP8) ? 1 : ((flags & FBP_FLAG_PRECISION_BF16) ? 0 : -1);
     const bool mma = mma_req >= 0 && L <= 64;
+    const bool mma_every = mma && (flags & FBP_FLAG_MMA_EVERY_PASS) != 0;
     if (mma) {
         mixed = false;
         if (h->mma_fmt != mma_req) {
@@ -1374,12 +1435,18 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         // few templates per window, which the vector-unit kernel skips one by one
         // (the matrix-core kernel only by 16-template tiles over 4 windows).
         // Both kernels leave exact values wherever a decision can depend on them.
-        if (mma) {
-            // matrix-core approximation of every computed pair, then the exact FP32
-            // chain on the 8-position groups it could not settle
-            launch_likmma(ts, st, mma_view(h), mv, mma_req, nW, h->mma_gmask.as<uint8_t>(), dirty, s);
+        if (mma && (dirty == nullptr || mma_every)) {
+            // matrix-core approximation of every computed pair (the dense kernel on the
+            // first pass where it applies), then the exact FP32 chain on the 8-position
+            // groups it could not settle
+            const MmaTables mt = mma_view(h);
+            if (dirty != nullptr || mma_every || !launch_likmma_dense(ts, st, mt, mv, mma_req, h->mma_gmask.as<uint8_t>(), s))
+                launch_likmma(ts, st, mt, mv, mma_req, nW, h->mma_gmask.as<uint8_t>(), dirty, s);
             stats.n_likelihood_mfma_launches++;
             if (!(flags & FBP_FLAG_NO_RESCORE)) launch_likelihood(ts, st, nW, dirty, s, h->mma_gmask.as<uint8_t>());
+        } else if (mma) {
+            // later passes: the few changed pairs per window on the vector units
+            launch_likelihood(ts, st, nW, dirty, s);
         } else if (mixed && dirty == nullptr) {
             const MfTables mt = mf_view(h);
             for (uint32_t w0 = 0; w0 < nW; w0 += lik_batch) {
@@ -1479,7 +1546,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         tm.begin(TM_RESID);
         launch_residual(ts, st.V, N, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, reach, s);
         launch_mark_dirty(ts, st, h->sp_idx.as<uint32_t>() + total, h->sp_lab.as<uint32_t>() + total, nadd, s);
-        if (mma) launch_requantize(st.V, N, (int)C, (int)L, mv, mma_req, h->sp_idx.as<uint32_t>() + total, nadd, s);
+        if (mma && mma_every)
+            launch_requantize(st.V, N, (int)C, (int)L, mv, mma_req, h->sp_idx.as<uint32_t>() + total, nadd, s);
         tm.end(TM_RESID);
         total += nadd;
         nW = h->h_counters[CNT_W];

@@ -103,7 +103,15 @@ struct MmaTables {
     int R;              // rows per window: ceil(L / 16)
     int P;              // windows per tile: 16 / R
     float kl, kb, ka, kd;
+    // dense first pass (k_likmma_dense): templates in quads ordered by first
+    // channel; a quad's channels (union of its templates', ascending) are staged
+    const int* qtmp;    // [nq * 4] template of each quad slot (-1: none)
+    const int* qch;     // [nq][kQuadCh] channels of each quad (-1: none)
+    const int* rslot;   // [R rows] slot of each compact template row in its quad's channel list
+    int nq;             // quads (0: the dense first pass is not available)
+    int qc;             // most channels of any quad
 };
+constexpr int kQuadCh = 16;  // channels of a template quad of the dense first pass at most
 
 struct MmaVoltage {
     void* q;            // [C][ldq] quantized voltage, channel-major (zero beyond N)

@@ -219,7 +219,8 @@ __global__ __launch_bounds__(256) void k_likmma(TemplateSet ts, PassState st, Mm
         const float inv = 1.0f / (ts_t * vs);
         const float bt = ts.bias[t];
         const float dconst = mt.kb * fabsf(bt) + mt.ka * mt.T1[t] / vs + mt.kd * inv;
-        const float ka_s = mt.ka * 64.0f / ts_t;
+        const float ka_s = mt.ka * (float)(32 * KB) / ts_t;  // products per channel and position
+        const int nbr = (L + 14) / 16 + 1;                   // 16-sample blocks a row's 16 positions read
         const int ntile = (nact + P - 1) / P;
         auto row_x = [&](int tile, int prow, int rrow) -> int64_t {
             const bool ok = prow < P && tile * P + prow < nact;
@@ -244,12 +245,19 @@ __global__ __launch_bounds__(256) void k_likmma(TemplateSet ts, PassState st, Mm
         load_a(row_x(0, p_row, rr_row), acur);
         for (int tile = 0; tile < ntile; tile++) {
             if (tile + 1 < ntile) load_a(row_x(tile + 1, p_row, rr_row), anxt);
-            // bound inputs: lane (row m_b, block j_b) loads block j_b of row m_b's four
-            // 16-sample blocks; the row maximum is a 4-lane reduction
+            // bound inputs: lane (row m_b, block j_b) loads blocks j_b, j_b + 4, ... of
+            // the nbr 16-sample blocks row m_b reads; the row maximum is a 4-lane reduction
             const int64_t xb = row_x(tile, p_b, rr_b);
             float vb[G];
 #pragma unroll
-            for (int i = 0; i < G; i++) vb[i] = i < nr ? mv.bmax[(size_t)chs[i] * mv.nbq + (xb >> 4) + j_b] : 0.0f;
+            for (int i = 0; i < G; i++) {
+                vb[i] = 0.0f;
+                if (i < nr) {
+                    const float* bm = mv.bmax + (size_t)chs[i] * mv.nbq + (xb >> 4);
+                    vb[i] = bm[j_b];
+                    for (int jj = j_b + 4; jj < nbr; jj += 4) vb[i] = fmaxf(vb[i], bm[jj]);
+                }
+            }
             floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int i = 0; i < G; i++) {
@@ -317,6 +325,231 @@ __global__ __launch_bounds__(256) void k_likmma(TemplateSet ts, PassState st, Mm
     for (int off = 32; off > 0; off >>= 1) rescored += __shfl_xor(rescored, off, 64);
     if (lane == 0 && rescored) atomicAdd(st.counters + CNT_LIKX, rescored);
     if (lane == 0 && pairs) atomicAdd(st.counters + CNT_PAIRS, pairs);  // the same count in every lane
+}
+
+// ---------------------------------------------------------------------------
+// Dense first pass (every window, every template; L a multiple of 16).
+// A block takes a quad of templates (one per wave; the quads group templates
+// by first channel, so a quad's channels are few) and a run of spans of
+// kDenseW consecutive windows.  Per span the block stages the quantized
+// samples [x0, x0 + kDenseW L + 64) of the quad's channels and their 16-sample
+// block maxima in LDS; each wave holds its template's B fragments in registers
+// for the whole run and takes the span's R row tiles (16 rows of 16 positions,
+// rows = consecutive 16-position groups), A fragments read from LDS.  Bound,
+// LIK values and the 8-position groups to rescore are those of k_likmma.
+// ---------------------------------------------------------------------------
+constexpr int kDenseW = 16;  // windows per span
+
+// Samples staged per channel: rows reach 16 (kDenseW L / 16 - 1) + 32 KB.
+__host__ __device__ inline int dense_span_samples(int L, int KB) { return kDenseW * L + 32 * KB; }
+
+size_t likmma_dense_lds(int L, int KB, int qc, int fmt)
+{
+    const size_t sps = (size_t)dense_span_samples(L, KB);
+    const size_t rowb = ((sps * (fmt == 0 ? 2 : 1)) + 15) & ~(size_t)15;
+    return (size_t)qc * rowb + (size_t)qc * (sps / 16) * 4;
+}
+
+template <int FMT, int KB>
+__global__ __launch_bounds__(256) void k_likmma_dense(TemplateSet ts, PassState st, MmaTables mt, MmaVoltage mv,
+                                                      int64_t nspan, uint8_t* __restrict__ gmask)
+{
+    typedef typename FragT<FMT>::type frag_t;
+    constexpr int G = (kMmaMaxFrag / KB < 8) ? kMmaMaxFrag / KB : 8;  // channels on the matrix-core path
+    constexpr int NF = G * KB;
+    constexpr int ES = FMT == 0 ? 2 : 1;  // bytes per quantized sample
+    extern __shared__ __align__(16) unsigned char lsm[];
+    const int U = ts.U, L = ts.L, R = L / 16;
+    const int64_t N = st.N, nwin = st.nwin;
+    const int sps = dense_span_samples(L, KB);  // samples per staged channel row
+    const int rowb = (sps * ES + 15) & ~15;
+    const int nbk = sps / 16;             // block maxima per staged row
+    const int qc = mt.qc;
+    float* lbm = reinterpret_cast<float*>(lsm + (size_t)qc * rowb);  // [qc][nbk]
+    const int q = (int)(blockIdx.x % (unsigned)mt.nq);
+    const int64_t sb = blockIdx.x / (unsigned)mt.nq, nsb = gridDim.x / (unsigned)mt.nq;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int t = mt.qtmp[q * 4 + wv];
+    const int* qch = mt.qch + (size_t)q * kQuadCh;
+    int nqc = 0;
+    for (int i = 0; i < kQuadCh; i++) nqc += qch[i] >= 0 ? 1 : 0;
+    const int nr = t >= 0 ? ts.nch[t] : 0, r0 = t >= 0 ? ts.off[t] : 0;
+    const bool mm = t >= 0 && nr <= G;    // on the matrix cores (else the exact pass takes every group)
+    const int ngr = L / kPos;
+    const float vs = *mv.vs;
+    // this wave's template: B fragments, channel slots, bound terms
+    frag_t Bf[NF];
+    int slot[G];
+    float rl1[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) {
+        slot[i] = (mm && i < nr) ? mt.rslot[r0 + i] : 0;
+        rl1[i] = (mm && i < nr) ? mt.rowL1[r0 + i] : 0.0f;
+    }
+    {
+        const frag_t* F = static_cast<const frag_t*>(mt.frag) + (size_t)r0 * KB * 64 + lane;
+#pragma unroll
+        for (int i = 0; i < NF; i++) Bf[i] = (mm && i < nr * KB) ? F[(size_t)i * 64] : frag_t{};
+    }
+    const float ts_t = t >= 0 ? mt.tsc[t] : 1.0f;
+    const float inv = 1.0f / (ts_t * vs);
+    const float bt = t >= 0 ? ts.bias[t] : 0.0f;
+    const float dconst = t >= 0 ? mt.kb * fabsf(bt) + mt.ka * mt.T1[t] / vs + mt.kd * inv : 0.0f;
+    const float ka_s = mt.ka * (float)(32 * KB) / ts_t;  // products per channel and position
+    const int q4 = lane >> 4, col = lane & 15;
+    const int m_b = lane >> 2, j_b = lane & 3;
+    const int nbr = (L + 14) / 16 + 1;  // 16-sample blocks a row's 16 positions read
+    uint32_t rescored = 0, pairs = 0;
+    for (int64_t sp = sb; sp < nspan; sp += nsb) {
+        const int64_t w0 = sp * kDenseW;
+        const int64_t x0 = w0 * L;
+        __syncthreads();  // the previous span's rows are consumed
+        // stage: the quad's channels, 16-byte pieces of each row, and the block maxima
+        {
+            // (the last span's rows run past the quantized rows' end: zeros there)
+            // (four 16-byte pieces and four block maxima in flight per thread)
+            const int pieces = rowb / 16;
+            const int64_t rowend = mv.ldq * ES;  // bytes of a quantized channel row
+            const int totp = nqc * pieces, totb = nqc * nbk;
+            for (int e0 = tid; e0 < max(totp, totb); e0 += 4 * 256) {
+                uint4 v[4];
+                float bv[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int e = e0 + j * 256;
+                    v[j] = make_uint4(0u, 0u, 0u, 0u);
+                    bv[j] = 0.0f;
+                    if (e < totp) {
+                        const int ci = e / pieces, pc = e - ci * pieces;
+                        const int64_t boff = x0 * ES + (int64_t)pc * 16;
+                        if (boff + 16 <= rowend)
+                            v[j] = *reinterpret_cast<const uint4*>(static_cast<const unsigned char*>(mv.q) +
+                                                                   (size_t)qch[ci] * rowend + boff);
+                    }
+                    if (e < totb) {
+                        const int ci = e / nbk, bk = e - ci * nbk;
+                        const int64_t gb = (x0 >> 4) + bk;
+                        if (gb < mv.nbq) bv[j] = mv.bmax[(size_t)qch[ci] * mv.nbq + gb];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);  // the loads go out together
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int e = e0 + j * 256;
+                    if (e < totp) {
+                        const int ci = e / pieces, pc = e - ci * pieces;
+                        *reinterpret_cast<uint4*>(lsm + (size_t)ci * rowb + (size_t)pc * 16) = v[j];
+                    }
+                    if (e < totb) lbm[e] = bv[j];
+                }
+            }
+        }
+        __syncthreads();
+        if (t < 0) continue;
+        const int nvw = (int)min((int64_t)kDenseW, nwin - w0);  // windows of this span
+        if (lane == 0) pairs += (uint32_t)nvw;
+        if (!mm) {
+            if (lane < nvw) {
+                gmask[(size_t)(w0 + lane) * U + t] = (uint8_t)((1u << ngr) - 1u);
+                rescored += (uint32_t)(ngr * kPos);
+            }
+            continue;
+        }
+        uint32_t gm = 0;  // lanes < kDenseW: the 8-position groups of window w0 + lane to rescore
+        for (int k = 0; k < R; k++) {
+            // A: row m = col of this tile, group row gr = 16 k + m, samples from 16 gr
+            floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            const int arow = 16 * (16 * k + col) + 8 * KB * q4;
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                if (i >= nr) break;
+                const unsigned char* rb = lsm + (size_t)slot[i] * rowb;
+#pragma unroll
+                for (int kb = 0; kb < KB; kb++) {
+                    const frag_t a = *reinterpret_cast<const frag_t*>(rb + (size_t)(arow + 8 * kb) * ES);
+                    acc = mma<FMT>(a, Bf[i * KB + kb], acc);
+                }
+            }
+            // bound inputs of row m_b: its nbr 16-sample blocks, a 4-lane maximum
+            float Q = 0.0f, S = 0.0f;
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                if (i >= nr) break;
+                const float* bm = lbm + slot[i] * nbk + 16 * k + m_b;
+                float v = bm[j_b];
+                for (int jj = j_b + 4; jj < nbr; jj += 4) v = fmaxf(v, bm[jj]);
+                v = fmaxf(v, __shfl_xor(v, 1, 64));
+                v = fmaxf(v, __shfl_xor(v, 2, 64));
+                Q = fmaf(rl1[i], v, Q);
+                S += v;
+            }
+            uint64_t bal[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int mo = 4 * q4 + r;
+                const float Qm = __shfl(Q, 4 * mo, 64), Sm = __shfl(S, 4 * mo, 64);
+                const int64_t x = x0 + 16 * (16 * k + mo) + col;
+                bool need = false;
+                float v = 0.0f;
+                if (x + L < N) {
+                    v = bt + acc[r] * inv;
+                    const float d = mt.kl * Qm + dconst + ka_s * Sm;
+                    need = !(v <= -d);
+                }
+                if (x < N) st.LIK[(size_t)t * N + x] = v;
+                bal[r] = __ballot(need);
+            }
+            if (lane < nvw) {
+                // rows gr of window `lane` in this tile: [lane R, lane R + R) within [16 k, 16 k + 16)
+                const int g0 = max(lane * R, 16 * k), g1 = min(lane * R + R, 16 * k + 16);
+                for (int gr = g0; gr < g1; gr++) {
+                    const int mo = gr - 16 * k;
+                    const uint32_t bits = (uint32_t)((bal[mo & 3] >> (16 * (mo >> 2))) & 0xffffull);
+                    const int ro = gr - lane * R;
+                    if (bits & 0xffu) gm |= 1u << (2 * ro);
+                    if (bits >> 8) gm |= 1u << (2 * ro + 1);
+                }
+            }
+        }
+        if (lane < nvw) {
+            const size_t o = (size_t)(w0 + lane) * U + t;
+            gmask[o] = (uint8_t)gm;
+            if (gm == 0u) {
+                st.wmax_v[o] = -INFINITY;  // every value of the pair is negative
+                st.wmax_p[o] = 0xffffu;
+            }
+            rescored += (uint32_t)__popc(gm) * kPos;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) rescored += __shfl_xor(rescored, off, 64);
+    if (lane == 0 && rescored) atomicAdd(st.counters + CNT_LIKX, rescored);
+    if (lane == 0 && pairs) atomicAdd(st.counters + CNT_PAIRS, pairs);
+}
+
+// Returns false when the dense first pass cannot take this template set.
+bool launch_likmma_dense(const TemplateSet& ts, const PassState& st, const MmaTables& mt, const MmaVoltage& mv, int fmt,
+                         uint8_t* gmask, hipStream_t s)
+{
+    if (mt.nq <= 0 || ts.L % 16 != 0 || ts.L > 64) return false;
+    const size_t lds = likmma_dense_lds(ts.L, mt.KB, mt.qc, fmt);
+    if (lds > 64 * 1024) return false;
+    const int64_t nspan = (st.nwin + kDenseW - 1) / kDenseW;
+    // a block takes a run of spans; quads vary fastest so the blocks of one span run together
+    // about eight blocks per CU in all: each block keeps its template's B fragments for many spans
+    const int64_t runs = std::max<int64_t>(1, std::min<int64_t>(nspan, (int64_t)2048 / mt.nq + 1));
+    const dim3 g((unsigned)(runs * mt.nq)), b(256);
+#define FBP_MMAD(F, K) hipLaunchKernelGGL((k_likmma_dense<F, K>), g, b, lds, s, ts, st, mt, mv, nspan, gmask)
+    if (fmt == 0) {
+        if (mt.KB == 1) FBP_MMAD(0, 1);
+        else if (mt.KB == 2) FBP_MMAD(0, 2);
+        else FBP_MMAD(0, 3);
+    } else {
+        if (mt.KB == 1) FBP_MMAD(1, 1);
+        else if (mt.KB == 2) FBP_MMAD(1, 2);
+        else FBP_MMAD(1, 3);
+    }
+#undef FBP_MMAD
+    return true;
 }
 
 // Fragment values (already scaled, 8 per lane) to the operand format, with the

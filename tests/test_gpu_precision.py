@@ -1,9 +1,11 @@
 """GPU: the BF16 and FP8 (e4m3) matrix-core likelihood against the oracle.
 
-Every pass's likelihood is approximated on the matrix cores and each
-8-position group that the error bound cannot prove negative is recomputed with
-the exact FP32 chain, so spikes, labels, order and the residual must equal the
-oracle's bit for bit, for both formats and every screen path."""
+The dense first pass (by default; every pass with FLAG_MMA_EVERY_PASS) is
+approximated on the matrix cores and each 8-position group that the error
+bound cannot prove negative is recomputed with the exact FP32 chain (later
+passes run the FP32 kernel on the changed pairs), so spikes, labels, order and
+the residual must equal the oracle's bit for bit, for both formats, both pass
+plans and every screen path."""
 import numpy as np
 import pytest
 
@@ -26,39 +28,46 @@ def handle():
     h.close()
 
 
-def run_fmt(h, V, T, bias, thr, n, fmt, screen="split", recheck=True):
+def run_fmt(h, V, T, bias, thr, n, fmt, screen="split", recheck=True, every=False):
     h.set_templates(T, bias, thr)
     flags = FMTS[fmt] | (_native.FLAG_OVERLAP_RECHECK if recheck else 0)
+    if every:
+        flags |= _native.FLAG_MMA_EVERY_PASS
     if screen == "valu":
         flags |= _native.FLAG_SCREEN_FP32
     elif screen == "literal":
         flags |= _native.FLAG_RECHECK_LITERAL
-    print(f"[precision] {fmt} screen={screen} C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n={n}",
+    print(f"[precision] {fmt} every={every} screen={screen} C={V.shape[0]} N={V.shape[1]} U={T.shape[0]} L={T.shape[2]} n={n}",
           flush=True)
     h.run(V, n, flags)
     idx, lab = h.result()
     return idx, lab, h.residual(V.shape[1])
 
 
+@pytest.mark.parametrize("every", [False, True])
 @pytest.mark.parametrize("fmt", sorted(FMTS))
 @pytest.mark.parametrize("seed", range(6))
-def test_noisy_cases_match_oracle(handle, fmt, seed):
+def test_noisy_cases_match_oracle(handle, fmt, seed, every):
     V, T, bias, thr, on, lab = cases.noisy_case(seed)
     n = T.shape[2] - 1
-    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt), fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt, every=every),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
 @pytest.mark.parametrize("fmt", sorted(FMTS))
 @pytest.mark.parametrize("C,L,U,N", [(2, 8, 3, 600), (5, 20, 17, 1500), (9, 33, 40, 2000), (12, 48, 16, 3000),
                                      (17, 64, 33, 2500), (3, 16, 1, 400), (64, 48, 20, 1200), (4, 17, 6, 900),
                                      (1, 1, 2, 300), (3, 49, 5, 1000)])
-def test_shapes_match_oracle(handle, fmt, C, L, U, N):
+@pytest.mark.parametrize("every", [False, True])
+def test_shapes_match_oracle(handle, fmt, C, L, U, N, every):
     """Rows per window 1..4 (L up to 64), one, two and three k-blocks per channel, templates
     with more channels than one register load of fragments (dense 64-channel ones), tiles
-    that end inside a window, windows at the end of the chunk."""
+    that end inside a window, windows at the end of the chunk; the dense first pass where L
+    is a multiple of 16 (a partial last span of 16 windows), the per-template kernel elsewhere."""
     V, T, bias, thr, on, lab = cases.noisy_case(70 + C + L, C=C, L=L, U=U, N=N, nsp=max(4, N // 25))
     n = L - 1
-    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt), fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt, every=every),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
 @pytest.mark.parametrize("fmt", sorted(FMTS))
@@ -83,16 +92,19 @@ def test_channel_dynamic_range_matches_oracle(handle, fmt, screen, seed):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
+@pytest.mark.parametrize("every", [False, True])
 @pytest.mark.parametrize("fmt", sorted(FMTS))
 @pytest.mark.parametrize("name,dur", [("small", 3.0), ("probe16", 0.5), ("probe64", 0.3)])
-def test_workload_slices_match_oracle(handle, fmt, name, dur):
+def test_workload_slices_match_oracle(handle, fmt, name, dur, every):
     d = synthetic.generate(name, duration_s=dur)
     T, bias, thr = synthetic.kernel_inputs(d)
     n = d["n_max_shift_inds"]
     V = d["voltage"]
-    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt), fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+    assert_same(run_fmt(handle, V, T, bias, thr, n, fmt, every=every),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
     st = handle.stats()
-    assert st["n_likelihood_mfma_launches"] == st["passes"]
+    # the matrix cores take the first pass, or every pass with FLAG_MMA_EVERY_PASS
+    assert st["n_likelihood_mfma_launches"] == (st["passes"] if every else 1)
 
 
 @pytest.mark.parametrize("fmt", sorted(FMTS))

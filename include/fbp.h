@@ -52,6 +52,12 @@ extern "C" {
 /* with FBP_FLAG_PRECISION_BF16 / _FP8: the matrix cores take every pass (one template per tile over the listed
  * windows), not only the dense first pass (later passes then run the FP32 kernel on the changed pairs) */
 #define FBP_FLAG_MMA_EVERY_PASS 4096u
+/* fbp_run_batch only: every pass after the first is one replay of a hipGraph of the pass (its launch
+ * sizes fixed when the graph is built, its item counts written on the device), with one host read per
+ * replay; the graph is rebuilt when a segment lists more windows than it holds or far fewer in all.
+ * Such a call runs only while no other fbp_run / fbp_run_batch is in flight in the process, and other
+ * calls fail with FBP_E_STATE until it returns (graph calls stay on one host thread at a time). */
+#define FBP_FLAG_BATCH_GRAPH 8192u
 
 typedef struct fbp_handle fbp_handle;
 
@@ -86,6 +92,8 @@ typedef struct fbp_stats {
     double ms_rescore;            /* deferred exact rescoring of the overlap candidates (part of ms_recheck),
                                      with FBP_FLAG_TIMING */
     int64_t n_rescore_launches;
+    int64_t n_graph_captures;     /* fbp_run_batch with FBP_FLAG_BATCH_GRAPH: passes captured ... */
+    int64_t n_graph_replays;      /* ... and replayed (counts of the whole batch) */
 } fbp_stats;
 
 /* Create a handle on HIP device `device` (replaces the device search and
@@ -112,7 +120,8 @@ int fbp_run(fbp_handle *h, const float *V, int64_t ld, int64_t N, int32_t v_on_d
 
 /* Batched pursuit: the pass loops of nseg segments run together, each kernel
  * launched once per pass over every segment still converging, with two host
- * reads of all segments' counters per pass.  Segment g is the chunk V[g]
+ * reads of all segments' counters per pass (one per pass, by hipGraph replays,
+ * with FBP_FLAG_BATCH_GRAPH).  Segment g is the chunk V[g]
  * (row stride ld[g], N[g] samples; device pointers when v_on_device != 0)
  * pursued with the templates set on hs[g] and n_max_shift[g]; its spikes,
  * residual, statistics and pass log are read back through hs[g] exactly as

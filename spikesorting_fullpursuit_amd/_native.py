@@ -31,6 +31,7 @@ FLAG_PRECISION_FP8 = 512
 FLAG_FIRST_PASS_ONLY = 1024
 FLAG_NO_RESCORE = 2048
 FLAG_MMA_EVERY_PASS = 4096
+FLAG_BATCH_GRAPH = 8192
 
 # Every symbol include/fbp.h declares.
 EXPORTS = (
@@ -73,6 +74,8 @@ class FbpStats(ctypes.Structure):
         ("n_likelihood_mfma_launches", ctypes.c_int64),
         ("ms_rescore", ctypes.c_double),
         ("n_rescore_launches", ctypes.c_int64),
+        ("n_graph_captures", ctypes.c_int64),
+        ("n_graph_replays", ctypes.c_int64),
     ]
 
     def as_dict(self):

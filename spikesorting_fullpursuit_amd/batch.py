@@ -78,12 +78,13 @@ def release_device_buffers():
     _pools.clear()
 
 
-def _batch_fused(segments, device, stats, tiles, lanes=1):
+def _batch_fused(segments, device, stats, tiles, lanes=1, graph=False):
     """Fused mode of binary_pursuit_batch: the drop-in's argument handling,
     chunk plan and output assembly per segment, one fbp_run_batch per group.
     The chunks are shared out over `lanes` lanes (largest first, each to the
     lane with the least work so far); the lanes run side by side, each its
-    groups in turn on its own handles and stream."""
+    groups in turn on its own handles and stream.  graph: every pass after
+    the first is one replay of a hipGraph-captured pass."""
     t0 = time.perf_counter()
     prepared = []
     for i, seg in enumerate(segments):
@@ -162,7 +163,8 @@ def _batch_fused(segments, device, stats, tiles, lanes=1):
                         chunk = np.float32(chunk)
                     volts.append(chunk)
                 nms.append(int(p["n_max_shift_inds"]))
-            _native.run_batch(hs, volts, nms, flags, block_cap=tiles[0], lik_windows=tiles[1])
+            _native.run_batch(hs, volts, nms, flags | (_native.FLAG_BATCH_GRAPH if graph else 0),
+                              block_cap=tiles[0], lik_windows=tiles[1])
             for h, (i, a, b, _) in zip(hs, group):
                 idx, lab = h.result()
                 if idx.shape[0] > 0:
@@ -199,16 +201,17 @@ def _batch_fused(segments, device, stats, tiles, lanes=1):
     if stats is not None:
         stats.update({"seconds": time.perf_counter() - t0, "mode": "fused",
                       "groups": sum(len(g) for g in lane_groups), "lanes": lanes, "chunks": len(items),
-                      "tiles": tuple(tiles)})
+                      "tiles": tuple(tiles), "graph": bool(graph)})
     return results
 
 
 def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", stats=None, mode="fused",
-                         tiles=(0, 0), lanes=2):
+                         tiles=(0, 0), lanes=2, graph=False):
     """Run binary_pursuit on every segment: all pass loops together ("fused",
     tile shape `tiles` = (blocks per segment and launch, windows per likelihood
-    block), 0 = the library's default; `lanes` fused loops side by side) or on
-    `n_workers` concurrent streams ("streams", `precision` as the drop-in's).
+    block), 0 = the library's default; `lanes` fused loops side by side;
+    `graph`: passes after the first as hipGraph replays) or on `n_workers`
+    concurrent streams ("streams", `precision` as the drop-in's).
 
     segments: list of dicts with the drop-in's arguments: "voltage" ([C][N]
     array or float32 device tensor), "sort_info", "separability_metrics" and
@@ -219,10 +222,12 @@ def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", sta
     `stats` (a dict) receives the wall time and the mode's parameters.
     """
     n = len(segments)
+    if graph and (mode != "fused" or int(lanes) != 1):
+        raise ValueError("graph=True runs the fused mode on one lane (graph calls stay on one host thread)")
     if n == 0:
         return []
     if mode == "fused":
-        return _batch_fused(segments, device, stats, tiles, lanes)
+        return _batch_fused(segments, device, stats, tiles, lanes, graph)
     if mode != "streams":
         raise ValueError("mode must be 'fused' or 'streams'")
     n_workers = max(1, min(int(n_workers), n))
@@ -320,20 +325,25 @@ def autotune_tiles(segments, block_caps=(256, 1024, 4096), lik_windows=(0, 1, 4)
     return best, rates
 
 
-def autotune_lanes(segments, candidates=(1, 2, 3, 4), sample=64, tiles=(0, 0), device=0):
+def autotune_lanes(segments, candidates=(1, 2, 3, 4), sample=64, tiles=(0, 0), device=0, graphs=(False,)):
     """Pick the fused mode's lane count (fused loops side by side) with the
     highest throughput on `sample` segments drawn evenly from the batch's cost
     order, after one untimed warm-up run.  Returns (best, {lanes: Msamples*ch/s}).
-    Results do not depend on the lane count."""
+    With several `graphs` settings the (lanes, graph) pairs are timed (graph
+    mode at one lane only) and the keys are such pairs.  Results do not depend
+    on either."""
     if not segments:
-        return 1, {}
+        return (1, graphs[0]) if len(graphs) > 1 else 1, {}
     sub = _sample(segments, sample)
     work = sum(int(s["voltage"].shape[0]) * int(s["voltage"].shape[1]) for s in sub)
     binary_pursuit_batch(sub, device=device, mode="fused", tiles=tiles, lanes=max(candidates))
     rates = {}
     for k in candidates:
-        st = {}
-        binary_pursuit_batch(sub, device=device, stats=st, mode="fused", tiles=tiles, lanes=k)
-        rates[k] = work / st["seconds"] / 1e6
+        for gr in graphs:
+            if gr and k != 1:
+                continue  # graph mode runs one lane
+            st = {}
+            binary_pursuit_batch(sub, device=device, stats=st, mode="fused", tiles=tiles, lanes=k, graph=gr)
+            rates[(k, gr) if len(graphs) > 1 else k] = work / st["seconds"] / 1e6
     best = max(rates, key=lambda k: rates[k])
     return best, rates

@@ -52,15 +52,20 @@ void launch_screen_tables(const TemplateSet& ts, const ScreenTables& tb, int64_t
 size_t screen_corr_lds(const TemplateSet& ts, int S, int* rp);
 void seg_likelihood_lds(const TemplateSet& ts, int* wb, int* sv, size_t* lds);
 void launch_seg_pass_kernel(int kind, int L, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
-                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s);
+                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s,
+                            GraphRec* rec = nullptr);
 void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
-                        uint32_t nblocks, uint16_t* eob, hipStream_t s);
-void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s);
+                        uint32_t nblocks, uint16_t* eob, hipStream_t s, GraphRec* rec = nullptr);
+void launch_seg_fill_eob(const SegLaunch* tab, uint32_t ntab, uint16_t* eob, hipStream_t s);
+void launch_seg_plan(const SegDesc* sd, const SegDyn* dy, SegLaunch* tab, const SegPlan& p, int phase, hipStream_t s,
+                     GraphRec* rec = nullptr);
+void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s, GraphRec* rec = nullptr);
 bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* bd, size_t* lds);
 void launch_seg_screen_corr(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
-                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s);
+                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s,
+                            GraphRec* rec = nullptr);
 void launch_seg_select(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks, uint16_t* eob,
-                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s);
+                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s, GraphRec* rec = nullptr);
 size_t screen_select_lds(const TemplateSet& ts, int S, int cmax);
 int screen_select_region(const TemplateSet& ts, int S, int cmax, int npl_max);
 void launch_screen_corr(const TemplateSet& ts, const PassState& st, const ScreenTables& tb, uint32_t f0, uint32_t nbatch,
@@ -298,6 +303,33 @@ struct fbp_handle {
     bool stage_tr_busy[kStageBufs] = {};
     hipStream_t cstream = nullptr;         // host-to-device copies of the voltage (default priority)
     std::unique_ptr<CopyPool> pool;
+};
+
+// Pursuit calls in flight in the process.  A batch with FBP_FLAG_BATCH_GRAPH
+// (it builds, launches and destroys hipGraphs) runs only while no other
+// fbp_run / fbp_run_batch is in flight, and none starts until it returns:
+// graph calls are kept on one host thread at a time.
+std::mutex g_run_mu;
+int g_runs = 0;
+bool g_graph_run = false;
+
+struct RunGuard {
+    bool ok = false, graph = false;
+    explicit RunGuard(bool graph_run) : graph(graph_run)
+    {
+        std::lock_guard<std::mutex> lk(g_run_mu);
+        if (g_graph_run || (graph && g_runs > 0)) return;
+        g_runs++;
+        if (graph) g_graph_run = true;
+        ok = true;
+    }
+    ~RunGuard()
+    {
+        if (!ok) return;
+        std::lock_guard<std::mutex> lk(g_run_mu);
+        g_runs--;
+        if (graph) g_graph_run = false;
+    }
 };
 
 static int set_err(fbp_handle* h, int code, const std::string& msg)
@@ -1067,6 +1099,36 @@ struct BatchBufs {
     }
 };
 
+// One pass of a batch captured in a hipGraph (FBP_FLAG_BATCH_GRAPH): launch
+// tables whose blocks are fixed at capture for the segments converging then
+// (at most R[g] listed windows each), their block-to-entry maps, and the
+// executable graph.  k_seg_plan writes the entries' item counts on the device
+// as the pass goes, so a replay runs the whole pass without a host read.
+struct PassGraph {
+    hipGraphExec_t exec = nullptr;
+    SegLaunch* host = nullptr;  // pinned
+    SegLaunch* dev = nullptr;
+    size_t cap = 0;
+    uint16_t* eob = nullptr;
+    size_t eob_cap = 0;
+    std::vector<uint32_t> R;  // listed windows a segment's launches hold (0: not in the graph)
+    uint64_t sumR = 0;
+    SegPlan plan{};
+    int64_t captures = 0, replays = 0;
+    void drop()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+    ~PassGraph()
+    {
+        drop();
+        if (host) (void)hipHostFree(host);
+        if (dev) (void)hipFree(dev);
+        if (eob) (void)hipFree(eob);
+    }
+};
+
 // Append one launch's entries: segment g gets nblk(g) blocks for n(g) items
 // (segments with no items are left out).  Returns the entry count.
 template <class F>
@@ -1250,6 +1312,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     if (!h) return FBP_E_INVALID;
     if (!h->have_templates) return set_err(h, FBP_E_STATE, "fbp_set_templates must be called first");
     if (!Vsrc || N <= 0 || ld < N) return set_err(h, FBP_E_INVALID, "voltage must be non-empty with ld >= N");
+    RunGuard guard(false);
+    if (!guard.ok) return set_err(h, FBP_E_STATE, "a batch with FBP_FLAG_BATCH_GRAPH is running in this process");
     const int64_t L = h->L, C = h->C, U = h->U;
     if (N >= (int64_t)0xffffffffLL) return set_err(h, FBP_E_UNSUPPORTED, "chunk longer than 2^32-1 samples");
     if (L > 65535) return set_err(h, FBP_E_UNSUPPORTED, "template length above 65535 samples");
@@ -1603,6 +1667,11 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         for (int32_t k = 0; k < g; k++)
             if (hs[k] == hs[g]) return set_err(h, FBP_E_INVALID, "a handle appears twice in the batch");
     }
+    RunGuard guard((flags & FBP_FLAG_BATCH_GRAPH) != 0);
+    if (!guard.ok)
+        return set_err(h, FBP_E_STATE, (flags & FBP_FLAG_BATCH_GRAPH)
+                                           ? "FBP_FLAG_BATCH_GRAPH: another pursuit call is running in this process"
+                                           : "a batch with FBP_FLAG_BATCH_GRAPH is running in this process");
     const bool recheck = (flags & FBP_FLAG_OVERLAP_RECHECK) != 0;
     const int L = h->L;
     for (int32_t g = 0; g < nseg; g++) {
@@ -1724,8 +1793,230 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         return e;
     };
+    // ---- captured passes: every pass after the first replays one hipGraph ----
+    const bool use_graph = (flags & FBP_FLAG_BATCH_GRAPH) != 0;
+    PassGraph pg;
+    pg.R.assign(nseg, 0);
+    // Build the graph of one pass for the segments in `active`, segment g's
+    // launches sized for R[g] listed windows (nodes added one by one: no stream
+    // capture, so other host threads' calls cannot interleave with it).
+    auto graph_capture = [&](const std::vector<uint32_t>& R, uint64_t sumR) -> int {
+        pg.drop();
+        pg.R = R;
+        pg.sumR = sumR;
+        if (pg.host == nullptr) {
+            pg.cap = (size_t)nseg * 16 + 64;
+            HIP_TRY(hipHostMalloc((void**)&pg.host, sizeof(SegLaunch) * pg.cap, hipHostMallocDefault), "alloc graph");
+            HIP_TRY(hipMalloc((void**)&pg.dev, sizeof(SegLaunch) * pg.cap), "alloc graph");
+        }
+        pg.plan = SegPlan{};
+        pg.plan.recheck = recheck ? 1 : 0;
+        // rows of each segment's flagged windows in Bbuf
+        std::vector<uint32_t> row0(nseg, 0);
+        uint32_t rows = 0;
+        for (uint32_t g : active) {
+            row0[g] = rows;
+            rows += R[g];
+        }
+        struct GL {
+            size_t o;
+            uint32_t n, nb;
+            size_t eo;
+        };
+        size_t used = 0, eused = 0;
+        bool over = false;
+        auto add = [&](int src, int phase, const std::vector<uint32_t>& segs, auto&& nblk_of) -> GL {
+            GL l{used, 0, 0, eused};
+            if (used + segs.size() > pg.cap) {
+                over = true;
+                return l;
+            }
+            for (uint32_t g : segs) {
+                SegLaunch e{};
+                e.seg = g;
+                e.blk0 = l.nb;
+                e.nblk = std::max<uint32_t>(1, nblk_of(g));
+                e.g0 = row0[g];
+                pg.host[used + l.n++] = e;
+                l.nb += e.nblk;
+            }
+            if (l.n == 0) return l;
+            SegPlan& p = pg.plan;
+            if (p.nr > 0 && p.src[p.nr - 1] == src && p.phase[p.nr - 1] == phase &&
+                p.off[p.nr - 1] + p.cnt[p.nr - 1] == (uint32_t)used) {
+                p.cnt[p.nr - 1] += l.n;
+            } else if (p.nr < kPlanRanges) {
+                p.off[p.nr] = (uint32_t)used;
+                p.cnt[p.nr] = l.n;
+                p.src[p.nr] = (uint8_t)src;
+                p.phase[p.nr] = (uint8_t)phase;
+                p.nr++;
+            } else {
+                over = true;
+            }
+            used += l.n;
+            eused += l.nb;
+            return l;
+        };
+        // the screen's blocks per segment: at most R[g], about 8192 in all
+        auto scr_blocks = [&](uint32_t g) {
+            return std::min<uint32_t>(R[g], std::max<uint32_t>(64, cdiv((uint64_t)R[g] * 8192, std::max<uint64_t>(sumR, 1))));
+        };
+        const GL lik = add(SRC_NW, 0, active, [&](uint32_t g) { return capb(cdiv(R[g], (uint32_t)bb.desc[g].wb)); });
+        const GL clr = add(SRC_NW, 0, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
+        const GL mx = add(SRC_NW, 0, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock / 64)); });
+        GL flg{}, corr{}, sel[kSelClasses]{}, re{};
+        if (recheck) flg = add(SRC_NW, 0, active, [&](uint32_t g) { return cdiv(R[g], kCompactTile); });
+        const GL pr = add(SRC_NW, 0, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
+        const GL sp = add(SRC_NW, 0, active, [&](uint32_t g) { return cdiv(R[g], kCompactTile); });
+        const GL snap = add(SRC_NWIN, 0, active, [&](uint32_t g) { return capb(cdiv(nwin[g], 4 * kBlock)); });
+        const GL nx = add(SRC_NWIN, 0, active, [&](uint32_t g) { return cdiv(nwin[g], kCompactTile); });
+        if (recheck) {
+            corr = add(SRC_NF, 1, active, scr_blocks);
+            for (int c = 0; c < kSelClasses; c++) {
+                std::vector<uint32_t> segs;
+                for (uint32_t g : active)
+                    if (scls[g] == c) segs.push_back(g);
+                sel[c] = add(SRC_NF, 1, segs, scr_blocks);
+            }
+            re = add(SRC_NF, 1, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
+        }
+        const GL dec = add(SRC_DECIDE, 1, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
+        const GL res = add(SRC_NADD, 2, active, [&](uint32_t g) { return capb(R[g]); });
+        const GL mk = add(SRC_NADD, 2, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
+        if (over) return set_err(h, FBP_E_UNSUPPORTED, "captured pass table overflow");
+        if (eused > pg.eob_cap) {
+            if (pg.eob) HIP_TRY(hipFree(pg.eob), "free graph");
+            pg.eob = nullptr;
+            pg.eob_cap = eused + eused / 4;
+            HIP_TRY(hipMalloc((void**)&pg.eob, pg.eob_cap * 2), "alloc graph");
+        }
+        HIP_TRY(hipMemcpyAsync(pg.dev, pg.host, sizeof(SegLaunch) * used, hipMemcpyHostToDevice, s), "copy graph tables");
+        const GL* all[] = {&lik, &clr, &mx, &flg, &pr, &sp, &snap, &nx, &corr, &sel[0], &sel[1], &sel[2], &sel[3],
+                           &re, &dec, &res, &mk};
+        for (const GL* l : all) launch_seg_fill_eob(pg.dev + l->o, l->n, pg.eob + l->eo, s);
+        HIP_TRY(hipGetLastError(), "kernel launch");
+        HIP_TRY(hipStreamSynchronize(s), "sync");
+        auto T = [&](const GL& l) { return pg.dev + l.o; };
+        auto E = [&](const GL& l) { return pg.eob + l.eo; };
+        const SegDesc* sd = bb.d_desc;
+        SegDyn* dy = bb.d_dyn;
+        // the pass's launches as graph nodes, in order; the selection classes
+        // side by side between the correlations and the reassignment
+        GraphRec rec;
+        HIP_TRY(hipGraphCreate(&rec.graph, 0), "create pass graph");
+        GraphRec* r = &rec;
+        launch_seg_plan(sd, dy, pg.dev, pg.plan, 0, s, r);
+        launch_seg_counters(sd, (uint32_t)nseg, nullptr, s, r);  // CNT_PAIRS = 0
+        launch_seg_pass_kernel(SEG_LIKELIHOOD, L, sd, dy, T(lik), lik.n, lik.nb, E(lik), lds_lik, 1, s, r);
+        launch_seg_pass_kernel(SEG_CLEAR_DIRTY, L, sd, dy, T(clr), clr.n, clr.nb, E(clr), 0, 0, s, r);
+        launch_seg_pass_kernel(SEG_WINDOW_MAX, L, sd, dy, T(mx), mx.n, mx.nb, E(mx), 0, 0, s, r);
+        if (recheck) launch_seg_compact(0, sd, dy, T(flg), flg.n, flg.nb, E(flg), s, r);
+        launch_seg_plan(sd, dy, pg.dev, pg.plan, 1, s, r);
+        if (recheck) {
+            launch_seg_screen_corr(L, sd, T(corr), corr.n, corr.nb, E(corr), lds_corr, bb.Bbuf, bstride, s, r);
+            rec.fork();
+            for (int c = 0; c < kSelClasses; c++)
+                launch_seg_select(L, sd, T(sel[c]), sel[c].n, sel[c].nb, E(sel[c]), cls_bd[c], cls_lds[c], bb.Bbuf,
+                                  bstride, s, r);
+            rec.join();
+            launch_seg_pass_kernel(SEG_REASSIGN, L, sd, dy, T(re), re.n, re.nb, E(re), 0, 0, s, r);
+        }
+        launch_seg_pass_kernel(SEG_DECIDE, L, sd, dy, T(dec), dec.n, dec.nb, E(dec), 0, 0, s, r);
+        launch_seg_pass_kernel(SEG_SNAPSHOT, L, sd, dy, T(snap), snap.n, snap.nb, E(snap), 0, 0, s, r);
+        launch_seg_pass_kernel(SEG_PRUNE, L, sd, dy, T(pr), pr.n, pr.nb, E(pr), 0, 0, s, r);
+        launch_seg_compact(1, sd, dy, T(sp), sp.n, sp.nb, E(sp), s, r);
+        launch_seg_compact(2, sd, dy, T(nx), nx.n, nx.nb, E(nx), s, r);
+        launch_seg_plan(sd, dy, pg.dev, pg.plan, 2, s, r);
+        launch_seg_pass_kernel(SEG_RESIDUAL, L, sd, dy, T(res), res.n, res.nb, E(res), 0, 0, s, r);
+        launch_seg_pass_kernel(SEG_MARK_DIRTY, L, sd, dy, T(mk), mk.n, mk.nb, E(mk), 0, 0, s, r);
+        launch_seg_counters(sd, (uint32_t)nseg, bb.d_cnt, s, r);
+        hipError_t ge = rec.err;
+        if (ge == hipSuccess) ge = hipGraphInstantiate(&pg.exec, rec.graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(rec.graph);
+        if (ge != hipSuccess) {
+            pg.exec = nullptr;
+            return hip_err(h, ge, "build pass graph");
+        }
+        pg.captures++;
+        return FBP_OK;
+    };
+    // One pass by a replay (building the graph first when a segment lists more windows
+    // than the graph holds, or the graph holds over 3x the listed windows);
+    // *ran = false when the flagged windows might not fit one screen round (the
+    // pass is then planned on the host).
+    auto graph_pass = [&](bool* ran) -> int {
+        *ran = false;
+        uint64_t want = 0;
+        bool need = pg.exec == nullptr;
+        for (uint32_t g : active) {
+            want += nW[g];
+            if (nW[g] > pg.R[g]) need = true;
+        }
+        if (!need && pg.sumR > 3 * want + 2048) need = true;
+        if (need) {
+            std::vector<uint32_t> R(nseg, 0);
+            uint64_t sumR = 0;
+            for (uint32_t g : active) {
+                R[g] = (uint32_t)std::min<uint64_t>((uint64_t)nwin[g], (uint64_t)nW[g] + nW[g] / 8 + 16);
+                sumR += R[g];
+            }
+            if (recheck && sumR > rows_cap) return FBP_OK;
+            int rc = graph_capture(R, sumR);
+            if (rc != FBP_OK) return rc;
+        }
+        // spike lists: room for one more spike per listed window
+        for (uint32_t g : active) {
+            fbp_handle* hg = hs[g];
+            const size_t need_b = (size_t)(total[g] + nW[g]) * 4;
+            HIP_TRY(grow_keep(hg->sp_idx, need_b, (size_t)total[g] * 4, s), "grow spikes");
+            HIP_TRY(grow_keep(hg->sp_lab, need_b, (size_t)total[g] * 4, s), "grow spikes");
+        }
+        for (int32_t g = 0; g < nseg; g++)
+            bb.h_dyn[g] = SegDyn{hs[g]->sp_idx.as<uint32_t>(), hs[g]->sp_lab.as<uint32_t>(), total[g], nW[g], 0u};
+        for (uint32_t g : active) bb.h_dyn[g].active = 1u;
+        HIP_TRY(hipMemcpyAsync(bb.d_dyn, bb.h_dyn, sizeof(SegDyn) * nseg, hipMemcpyHostToDevice, s), "copy batch");
+        HIP_TRY(hipGraphLaunch(pg.exec, s), "graph launch");
+        HIP_TRY(hipMemcpyAsync(bb.h_cnt, bb.d_cnt, (size_t)nseg * CNT_N * 4, hipMemcpyDeviceToHost, s), "copy counters");
+        HIP_TRY(hipStreamSynchronize(s), "sync");
+        pg.replays++;
+        std::vector<uint32_t> next;
+        for (uint32_t g : active) {
+            const uint32_t* c = bb.h_cnt + (size_t)g * CNT_N;
+            nF[g] = recheck ? c[CNT_F] : 0u;
+            nadd[g] = c[CNT_SPIKES];
+            if (nF[g] > nW[g] || nadd[g] > nW[g])
+                return set_err(h, FBP_E_HIP, "captured pass: counts beyond the listed windows");
+            fbp_stats& sg = stats[g];
+            sg.windows_listed += nW[g];
+            sg.windows_flagged += nF[g];
+            sg.likelihood_positions += (int64_t)nW[g] * L * hs[g]->U;
+            sg.likelihood_pairs_computed += c[CNT_PAIRS];
+            passes[g]++;
+            hs[g]->pass_log.push_back(nadd[g]);
+            hs[g]->pass_log.push_back(nW[g]);
+            hs[g]->pass_log.push_back(nF[g]);
+            if (nadd[g] > 0) next.push_back(g);
+        }
+        for (uint32_t g : next) {
+            total[g] += nadd[g];
+            nW[g] = bb.h_cnt[(size_t)g * CNT_N + CNT_W];
+        }
+        active.swap(next);
+        *ran = true;
+        return FBP_OK;
+    };
     int pass = 0;
     while (!active.empty()) {
+        if (use_graph && pass > 0) {
+            bool ran = false;
+            int rc = graph_pass(&ran);
+            if (rc != FBP_OK) return rc;
+            if (ran) {
+                pass++;
+                continue;
+            }
+        }
         // ---- likelihood, window maxima, flagged windows ----
         SegTables& A = bb.tabs[0];
         A.used = 0;
@@ -1983,6 +2274,8 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         sg.recheck_overflow = c[CNT_CAND];
         sg.screen_shared_pairs = c[CNT_SURV];
         sg.screen_separable_pairs = c[CNT_NSP];
+        sg.n_graph_captures = pg.captures;
+        sg.n_graph_replays = pg.replays;
         hg->stats = sg;
         hg->n_spikes = (int64_t)total[g];
         hg->have_result = true;

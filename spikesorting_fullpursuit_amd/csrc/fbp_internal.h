@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <tuple>
+#include <utility>
+#include <vector>
+
 namespace fbp {
 
 constexpr int kBlock = 256;          // threads per block for the streaming kernels
@@ -137,6 +141,8 @@ struct SegDyn {
     uint32_t* sp_idx;
     uint32_t* sp_lab;
     uint64_t total;
+    uint32_t nW;      // windows listed for this pass (read by k_seg_plan)
+    uint32_t active;  // 1 while the segment converges (read by k_seg_plan)
 };
 
 // One entry of a segmented launch: blocks [blk0, blk0 + nblk) of the grid work
@@ -145,6 +151,82 @@ struct SegDyn {
 struct SegLaunch {
     uint32_t seg, blk0, nblk, n, a0, g0;
 };
+
+// A captured pass: launch tables whose blocks are fixed at capture and whose
+// item counts a plan kernel writes on the device before each launch reads
+// them.  Range r is cnt[r] entries from off[r]; src[r] says where their n
+// comes from; phase[r] which plan launch of the pass writes it.
+enum SegSrc { SRC_NW = 0, SRC_NWIN, SRC_NF, SRC_DECIDE, SRC_NADD };
+constexpr int kPlanRanges = 16;
+struct SegPlan {
+    uint32_t off[kPlanRanges], cnt[kPlanRanges];
+    uint8_t src[kPlanRanges], phase[kPlanRanges];
+    int nr;
+    int recheck;
+};
+
+// Kernel launches recorded as the nodes of a hipGraph being built, instead of
+// launched on a stream: each node depends on the nodes of `tail` (the last
+// launch, or every launch of a fork once it is joined).  Between fork() and
+// join() the nodes all depend on the fork's base and run side by side.
+struct GraphRec {
+    hipGraph_t graph = nullptr;
+    std::vector<hipGraphNode_t> tail, base, branch;
+    bool forking = false;
+    hipError_t err = hipSuccess;
+    void fork()
+    {
+        base = tail;
+        branch.clear();
+        forking = true;
+    }
+    void join()
+    {
+        if (!branch.empty()) tail = branch;
+        forking = false;
+    }
+    void add(const hipKernelNodeParams& p)
+    {
+        if (err != hipSuccess) return;
+        const std::vector<hipGraphNode_t>& deps = forking ? base : tail;
+        hipGraphNode_t n = nullptr;
+        err = hipGraphAddKernelNode(&n, graph, deps.empty() ? nullptr : deps.data(), deps.size(), &p);
+        if (err != hipSuccess) return;
+        if (forking) {
+            branch.push_back(n);
+        } else {
+            tail.assign(1, n);
+        }
+    }
+};
+
+template <class Tup, size_t... I>
+inline void arg_addrs(Tup& t, void** a, std::index_sequence<I...>)
+{
+    ((a[I] = (void*)&std::get<I>(t)), ...);
+}
+
+// Launch kernel k on stream s, or record it in r (arguments converted to the
+// kernel's parameter types; the graph copies their values).
+template <class... P, class... A>
+inline void launch_or_record(GraphRec* r, void (*k)(P...), dim3 g, dim3 b, size_t lds, hipStream_t s, A&&... a)
+{
+    if (r == nullptr) {
+        hipLaunchKernelGGL(k, g, b, lds, s, std::forward<A>(a)...);
+        return;
+    }
+    std::tuple<P...> vals(std::forward<A>(a)...);
+    void* args[sizeof...(P)];
+    arg_addrs(vals, args, std::index_sequence_for<P...>{});
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(k);
+    p.gridDim = g;
+    p.blockDim = b;
+    p.sharedMemBytes = (unsigned)lds;
+    p.kernelParams = args;
+    p.extra = nullptr;
+    r->add(p);
+}
 
 // Kernels of a segmented pass launch (launch_seg_pass_kernel).
 enum SegKind { SEG_LIKELIHOOD = 0, SEG_CLEAR_DIRTY, SEG_WINDOW_MAX, SEG_REASSIGN, SEG_DECIDE, SEG_SNAPSHOT, SEG_PRUNE,

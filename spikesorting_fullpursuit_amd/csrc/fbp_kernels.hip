@@ -1109,6 +1109,7 @@ __global__ __launch_bounds__(kBlock) void k_reassign_seg(const SegDesc* __restri
                                                          const uint16_t* __restrict__ eob)
 {
     SEG_ENTRY();
+    if (e.n == 0) return;  // a segment of a captured pass with no flagged windows
     reassign_body(d.ts.L, d.st, bx, gx);
 }
 
@@ -1126,6 +1127,7 @@ __global__ __launch_bounds__(kBlock) void k_check_snapshot_seg(const SegDesc* __
                                                                const uint16_t* __restrict__ eob)
 {
     SEG_ENTRY();
+    if (e.n == 0) return;  // a segment of a captured pass that has converged
     copy_bytes_body(d.st.check, d.st.check_snap, d.st.nwin, bx, gx);
 }
 
@@ -1168,6 +1170,37 @@ __global__ __launch_bounds__(kBlock) void k_seg_counters(const SegDesc* __restri
         }
     }
 }
+
+// Item counts of a captured pass's entries (phase 0 at the start of the pass,
+// 1 after the flagged windows are counted, 2 after this pass's spikes are):
+// segments that have converged get none.
+__global__ __launch_bounds__(kBlock) void k_seg_plan(const SegDesc* __restrict__ sd, const SegDyn* __restrict__ dy,
+                                                     SegLaunch* __restrict__ tab, SegPlan p, int phase)
+{
+    for (int r = 0; r < p.nr; r++) {
+        if (p.phase[r] != phase) continue;
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.cnt[r]; i += gridDim.x * blockDim.x) {
+            SegLaunch& e = tab[p.off[r] + i];
+            const SegDyn y = dy[e.seg];
+            const uint32_t* c = sd[e.seg].st.counters;
+            uint32_t n = 0, a0 = e.a0;
+            if (y.active) {
+                switch (p.src[r]) {
+                case SRC_NW: n = y.nW; break;
+                case SRC_NWIN: n = (uint32_t)sd[e.seg].st.nwin; break;
+                case SRC_NF: n = c[CNT_F]; break;
+                case SRC_DECIDE:
+                    n = y.nW;
+                    a0 = (p.recheck && c[CNT_F] > 0) ? 1u : 0u;
+                    break;
+                default: n = c[CNT_SPIKES]; break;
+                }
+            }
+            e.n = n;
+            e.a0 = a0;
+        }
+    }
+}
 #undef SEG_ENTRY
 
 // Launchers of the segmented kernels.  `tab` (device) holds ntab entries whose
@@ -1184,58 +1217,71 @@ void seg_likelihood_lds(const TemplateSet& ts, int* wb, int* sv, size_t* lds)
     *lds = (size_t)(*wb) * likelihood_lds_per_window(ts, *sv);
 }
 
+void launch_seg_fill_eob(const SegLaunch* tab, uint32_t ntab, uint16_t* eob, hipStream_t s)
+{
+    if (ntab > 0) fill_eob(tab, ntab, eob, s);
+}
+
+void launch_seg_plan(const SegDesc* sd, const SegDyn* dy, SegLaunch* tab, const SegPlan& p, int phase, hipStream_t s,
+                     GraphRec* rec)
+{
+    launch_or_record(rec, k_seg_plan, dim3(4), dim3(kBlock), 0, s, sd, dy, tab, p, phase);
+}
+
+// rec: record the launch in a graph being built (eob then already holds this
+// table's block-to-entry map); otherwise fill eob and launch on s.
 void launch_seg_pass_kernel(int kind, int L, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
-                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s)
+                            uint32_t nblocks, uint16_t* eob, size_t lds, int flag, hipStream_t s, GraphRec* rec)
 {
     if (ntab == 0 || nblocks == 0) return;
-    fill_eob(tab, ntab, eob, s);
+    if (rec == nullptr) fill_eob(tab, ntab, eob, s);
+    const uint16_t* m = eob;
     const dim3 g(nblocks), b(kBlock);
     switch (kind) {
     case SEG_LIKELIHOOD:
-        if (L == 48) hipLaunchKernelGGL(k_likelihood_seg<48>, g, b, lds, s, sd, tab, eob, flag);
-        else if (L == 64) hipLaunchKernelGGL(k_likelihood_seg<64>, g, b, lds, s, sd, tab, eob, flag);
-        else if (L == 32) hipLaunchKernelGGL(k_likelihood_seg<32>, g, b, lds, s, sd, tab, eob, flag);
-        else hipLaunchKernelGGL(k_likelihood_seg<0>, g, b, lds, s, sd, tab, eob, flag);
+        if (L == 48) launch_or_record(rec, k_likelihood_seg<48>, g, b, lds, s, sd, tab, m, flag);
+        else if (L == 64) launch_or_record(rec, k_likelihood_seg<64>, g, b, lds, s, sd, tab, m, flag);
+        else if (L == 32) launch_or_record(rec, k_likelihood_seg<32>, g, b, lds, s, sd, tab, m, flag);
+        else launch_or_record(rec, k_likelihood_seg<0>, g, b, lds, s, sd, tab, m, flag);
         break;
-    case SEG_CLEAR_DIRTY: hipLaunchKernelGGL(k_clear_dirty_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_WINDOW_MAX: hipLaunchKernelGGL(k_window_max_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_REASSIGN: hipLaunchKernelGGL(k_reassign_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_DECIDE: hipLaunchKernelGGL(k_pursuit_decide_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_SNAPSHOT: hipLaunchKernelGGL(k_check_snapshot_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_PRUNE: hipLaunchKernelGGL(k_pursuit_prune_seg, g, b, 0, s, sd, tab, eob); break;
-    case SEG_RESIDUAL: hipLaunchKernelGGL(k_residual_seg, g, b, 0, s, sd, dy, tab, eob); break;
-    case SEG_MARK_DIRTY: hipLaunchKernelGGL(k_mark_dirty_seg, g, b, 0, s, sd, dy, tab, eob); break;
+    case SEG_CLEAR_DIRTY: launch_or_record(rec, k_clear_dirty_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_WINDOW_MAX: launch_or_record(rec, k_window_max_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_REASSIGN: launch_or_record(rec, k_reassign_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_DECIDE: launch_or_record(rec, k_pursuit_decide_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_SNAPSHOT: launch_or_record(rec, k_check_snapshot_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_PRUNE: launch_or_record(rec, k_pursuit_prune_seg, g, b, 0, s, sd, tab, m); break;
+    case SEG_RESIDUAL: launch_or_record(rec, k_residual_seg, g, b, 0, s, sd, dy, tab, m); break;
+    case SEG_MARK_DIRTY: launch_or_record(rec, k_mark_dirty_seg, g, b, 0, s, sd, dy, tab, m); break;
     default: break;
     }
 }
 
 // Ordered compaction of kind K over every entry: one block per tile of
 // kCompactTile items (the entries' nblk), one scan block per entry.
-void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
-                        uint32_t nblocks, uint16_t* eob, hipStream_t s)
+template <int K>
+static void seg_compact(const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
+                        const uint16_t* eob, hipStream_t s, GraphRec* rec)
 {
-    if (ntab == 0) return;
-    if (nblocks > 0) fill_eob(tab, ntab, eob, s);
     const dim3 g(nblocks), b(kBlock);
-    if (kind == 0) {
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<0>, g, b, 0, s, sd, dy, tab, eob);
-        hipLaunchKernelGGL(k_compact_scan_seg<0>, dim3(ntab), dim3(1024), 0, s, sd, tab);
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<0>, g, b, 0, s, sd, dy, tab, eob);
-    } else if (kind == 1) {
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<1>, g, b, 0, s, sd, dy, tab, eob);
-        hipLaunchKernelGGL(k_compact_scan_seg<1>, dim3(ntab), dim3(1024), 0, s, sd, tab);
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<1>, g, b, 0, s, sd, dy, tab, eob);
-    } else {
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_count_seg<2>, g, b, 0, s, sd, dy, tab, eob);
-        hipLaunchKernelGGL(k_compact_scan_seg<2>, dim3(ntab), dim3(1024), 0, s, sd, tab);
-        if (nblocks > 0) hipLaunchKernelGGL(k_compact_scatter_seg<2>, g, b, 0, s, sd, dy, tab, eob);
-    }
+    if (nblocks > 0) launch_or_record(rec, k_compact_count_seg<K>, g, b, 0, s, sd, dy, tab, eob);
+    launch_or_record(rec, k_compact_scan_seg<K>, dim3(ntab), dim3(1024), 0, s, sd, tab);
+    if (nblocks > 0) launch_or_record(rec, k_compact_scatter_seg<K>, g, b, 0, s, sd, dy, tab, eob);
 }
 
-void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s)
+void launch_seg_compact(int kind, const SegDesc* sd, const SegDyn* dy, const SegLaunch* tab, uint32_t ntab,
+                        uint32_t nblocks, uint16_t* eob, hipStream_t s, GraphRec* rec)
+{
+    if (ntab == 0) return;
+    if (rec == nullptr && nblocks > 0) fill_eob(tab, ntab, eob, s);
+    if (kind == 0) seg_compact<0>(sd, dy, tab, ntab, nblocks, eob, s, rec);
+    else if (kind == 1) seg_compact<1>(sd, dy, tab, ntab, nblocks, eob, s, rec);
+    else seg_compact<2>(sd, dy, tab, ntab, nblocks, eob, s, rec);
+}
+
+void launch_seg_counters(const SegDesc* sd, uint32_t nseg, uint32_t* out, hipStream_t s, GraphRec* rec)
 {
     if (nseg == 0) return;
-    hipLaunchKernelGGL(k_seg_counters, dim3(grid_for(nseg, kBlock, 1024)), dim3(kBlock), 0, s, sd, nseg, out);
+    launch_or_record(rec, k_seg_counters, dim3(grid_for(nseg, kBlock, 1024)), dim3(kBlock), 0, s, sd, nseg, out);
 }
 
 }  // namespace fbp

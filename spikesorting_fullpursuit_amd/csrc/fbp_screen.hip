@@ -831,7 +831,9 @@ __device__ __forceinline__ void apply_parse_at(const TemplateSet& ts, const Pass
     }
 }
 
-template <int LT>
+// PERSIST: the block takes windows bx, bx + gx, ... of the batch in turn
+// (otherwise the grid covers the batch, one window per block).
+template <int LT, bool PERSIST = false>
 __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState st, ScreenTables tb,
                                                                     uint32_t f0, uint32_t nbatch,
                                                                     const float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx,
@@ -894,11 +896,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const float invS = 1.0f / (float)S;
     const uint32_t nF = st.counters[CNT_F];
-    // one window per block (the grid covers the batch)
-    {
-        const uint32_t wi = bx;
+    for (uint32_t wi = bx; wi < nbatch; wi = PERSIST ? wi + gx : nbatch) {
         const uint32_t w = f0 + wi;
-        if (wi >= nbatch || w >= nF) return;
+        if (w >= nF) return;
+        if (PERSIST && wi != bx) __syncthreads();  // the previous window's LDS reads are done
         const int tid = (int)threadIdx.x;
         const bool owner = tid < KU * S;
         int u0, s_t;  // the thread's first unit and its shift
@@ -926,7 +927,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                 st.orc[id] = 0;
                 if (xcand != nullptr) xcand[(size_t)wi * kXStride] = kXDone;
             }
-            return;
+            continue;
         }
         const float* Bw = Bbuf + (size_t)wi * bstride;
         // Staging: every global load below is independent of the others, and
@@ -1461,7 +1462,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                 const bool defer = nr >= 1u && nr <= 64u && nun <= kRU && C <= 64 &&
                                    __popcll(cun_s | ts.cmask[lab]) <= kCV;
                 if (tid == 0) xw[0] = defer ? nr : kXDone;
-                if (defer) return;  // uniform over the block
+                if (defer) continue;  // uniform over the block
             }
             const int upr = max(1, srows / CM);
             const float invCM = 1.0f / (float)CM;
@@ -1866,8 +1867,8 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg_seg(const Se
 {
     const SegLaunch e = tab[eob[blockIdx.x]];
     const SegDesc& d = sd[e.seg];
-    screen_select_reg_body<LT>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
-                               blockIdx.x - e.blk0, e.nblk, nullptr);
+    screen_select_reg_body<LT, true>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
+                                     blockIdx.x - e.blk0, e.nblk, nullptr);
 }
 
 // Whether the register-resident selection takes this segment, and its block
@@ -1880,42 +1881,45 @@ bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* 
     return owners <= kSelMaxBlock && *lds <= kSelMaxLds;
 }
 
-// corr: entries' nblk blocks each (grid-stride over their windows); select: one
-// block per window (nblk == n), bd threads.
+// corr: entries' nblk blocks each (grid-stride over their windows); select:
+// bd threads, one block per window when nblk == n, else each block takes
+// windows nblk apart in turn.
 void launch_seg_screen_corr(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
-                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s)
+                            uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s, GraphRec* rec)
 {
     if (ntab == 0) return;
     const unsigned ge = std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024));
-    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    if (rec == nullptr) hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    const uint16_t* m = eob;
     const dim3 gc(nblocks), bc(kBlock);
-    if (L == 48) hipLaunchKernelGGL(k_screen_corr_seg<48>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
-    else if (L == 64) hipLaunchKernelGGL(k_screen_corr_seg<64>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
-    else if (L == 32) hipLaunchKernelGGL(k_screen_corr_seg<32>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
-    else hipLaunchKernelGGL(k_screen_corr_seg<0>, gc, bc, lds, s, sd, tab, eob, Bbuf, bstride);
+    if (L == 48) launch_or_record(rec, k_screen_corr_seg<48>, gc, bc, lds, s, sd, tab, m, Bbuf, bstride);
+    else if (L == 64) launch_or_record(rec, k_screen_corr_seg<64>, gc, bc, lds, s, sd, tab, m, Bbuf, bstride);
+    else if (L == 32) launch_or_record(rec, k_screen_corr_seg<32>, gc, bc, lds, s, sd, tab, m, Bbuf, bstride);
+    else launch_or_record(rec, k_screen_corr_seg<0>, gc, bc, lds, s, sd, tab, m, Bbuf, bstride);
 }
 
-// The selection of one class of segments: one block (bd threads, lds bytes of
-// dynamic LDS, the class's largest) per flagged window of the entries.
+// The selection of one class of segments: bd threads and lds bytes of dynamic
+// LDS (the class's largest) per block.
 void launch_seg_select(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks, uint16_t* eob,
-                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s)
+                       int bd, size_t lds, const float* Bbuf, size_t bstride, hipStream_t s, GraphRec* rec)
 {
     if (ntab == 0 || nblocks == 0) return;
     const unsigned ge = std::max<uint32_t>(1, std::min<uint32_t>(ntab, 1024));
-    hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    if (rec == nullptr) hipLaunchKernelGGL(k_fill_eob_scr, dim3(ge), dim3(kBlock), 0, s, tab, ntab, eob);
+    const uint16_t* m = eob;
     const dim3 gs(nblocks), bs(bd);
     if (L == 48) {
         allow_lds(k_screen_select_reg_seg<48>, lds);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<48>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
+        launch_or_record(rec, k_screen_select_reg_seg<48>, gs, bs, lds, s, sd, tab, m, Bbuf, bstride);
     } else if (L == 64) {
         allow_lds(k_screen_select_reg_seg<64>, lds);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<64>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
+        launch_or_record(rec, k_screen_select_reg_seg<64>, gs, bs, lds, s, sd, tab, m, Bbuf, bstride);
     } else if (L == 32) {
         allow_lds(k_screen_select_reg_seg<32>, lds);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<32>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
+        launch_or_record(rec, k_screen_select_reg_seg<32>, gs, bs, lds, s, sd, tab, m, Bbuf, bstride);
     } else {
         allow_lds(k_screen_select_reg_seg<0>, lds);
-        hipLaunchKernelGGL(k_screen_select_reg_seg<0>, gs, bs, lds, s, sd, tab, eob, Bbuf, bstride);
+        launch_or_record(rec, k_screen_select_reg_seg<0>, gs, bs, lds, s, sd, tab, m, Bbuf, bstride);
     }
 }
 

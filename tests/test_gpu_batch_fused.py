@@ -116,6 +116,62 @@ def test_batch_mixed_shift_ranges_and_lengths():
         h.close()
 
 
+GRAPH = _native.FLAG_BATCH_GRAPH
+
+
+def _graph_stats(hs):
+    st = hs[0].stats()
+    print(f"[batch-graph] captures {st['n_graph_captures']} replays {st['n_graph_replays']}", flush=True)
+    return st
+
+
+def test_graph_batch_of_20_equals_separate_runs():
+    """Captured passes (one hipGraph replay per pass after the first), 20
+    segments, C in {1, 4, 8, 16, 32}, U in [4, 60]: batch == separate run,
+    pass logs included."""
+    segs = _segments(20, 31)
+    hs = _handles(segs)
+    _native.run_batch(hs, [s["V"] for s in segs], [s["n"] for s in segs], FLAGS | GRAPH)
+    st = _graph_stats(hs)
+    assert st["n_graph_captures"] >= 1
+    assert st["n_graph_replays"] >= 1
+    _check(segs, hs, FLAGS)
+    for h in hs:
+        h.close()
+
+
+def test_graph_batch_matches_oracle():
+    segs = _segments(16, 32, units=(4, 16))
+    for s in segs:
+        s["n"] = 15
+    hs = _handles(segs)
+    _native.run_batch(hs, [s["V"] for s in segs], [s["n"] for s in segs], FLAGS | GRAPH)
+    assert _graph_stats(hs)["n_graph_replays"] >= 1
+    for g, (s, h) in enumerate(zip(segs, hs)):
+        idx, lab = h.result()
+        oi, ol, oR = fbp_oracle.pursuit(s["V"], s["T"], s["bias"], s["thr"], s["n"], return_residual=True)
+        np.testing.assert_array_equal(idx, oi)
+        np.testing.assert_array_equal(lab, ol)
+        np.testing.assert_array_equal(h.residual(s["V"].shape[1]), oR)
+        h.close()
+
+
+def test_graph_batch_without_recheck_mixed_lengths_repeated():
+    """No recheck; shift ranges 0..63 and one long segment; the same handles
+    run twice (graphs of the second call are built afresh): both == separate."""
+    segs = _segments(8, 34, units=(4, 24))
+    for g, s in enumerate(segs):
+        s["n"] = (0, 1, 7, 63)[g % 4]
+    segs.append(_segments(1, 35, dur=(4.0, 4.0), units=(30, 30))[0])
+    hs = _handles(segs)
+    for flags in (GRAPH, FLAGS | GRAPH):
+        _native.run_batch(hs, [s["V"] for s in segs], [s["n"] for s in segs], flags)
+        _graph_stats(hs)
+        _check(segs, hs, flags & ~GRAPH)
+    for h in hs:
+        h.close()
+
+
 def test_batch_rejects_mismatched_template_lengths():
     segs = _segments(2, 36, units=(4, 8))
     hs = _handles(segs)

@@ -173,6 +173,17 @@ def test_batch_and_refine_have_no_cpu_fallback():
         refine.forward_residual(torch.zeros(2, 1000), torch.zeros(1, 2, 8), torch.zeros(0), table)
 
 
+def test_batch_graph_mode_runs_one_lane():
+    """Graph mode keeps its graph calls on one host thread: more lanes, or the
+    streams mode, are refused before any device work."""
+    from spikesorting_fullpursuit_amd import batch, _native
+    assert _native.FLAG_BATCH_GRAPH == 8192
+    for kw in ({"lanes": 2}, {"lanes": 1, "mode": "streams"}):
+        with pytest.raises(ValueError, match="one lane"):
+            batch.binary_pursuit_batch([], graph=True, **kw)
+    assert batch.binary_pursuit_batch([], graph=True, lanes=1) == []
+
+
 def test_batch_workloads_cover_the_configured_mix():
     from spikesorting_fullpursuit_amd import synthetic
     ws = synthetic.batch_workloads()

@@ -1877,7 +1877,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
                 std::vector<uint32_t> segs;
                 for (uint32_t g : active)
                     if (scls[g] == c) segs.push_back(g);
-                sel[c] = add(SRC_NF, 1, segs, scr_blocks);
+                sel[c] = add(SRC_NF, 1, segs, [&](uint32_t g) { return R[g]; });  // one block per window
             }
             re = add(SRC_NF, 1, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
         }

@@ -831,9 +831,7 @@ __device__ __forceinline__ void apply_parse_at(const TemplateSet& ts, const Pass
     }
 }
 
-// PERSIST: the block takes windows bx, bx + gx, ... of the batch in turn
-// (otherwise the grid covers the batch, one window per block).
-template <int LT, bool PERSIST = false>
+template <int LT>
 __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState st, ScreenTables tb,
                                                                     uint32_t f0, uint32_t nbatch,
                                                                     const float* __restrict__ Bbuf, size_t bstride, uint32_t bx, uint32_t gx,
@@ -896,10 +894,11 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     const int wv = threadIdx.x >> 6, lane = lane_id();
     const float invS = 1.0f / (float)S;
     const uint32_t nF = st.counters[CNT_F];
-    for (uint32_t wi = bx; wi < nbatch; wi = PERSIST ? wi + gx : nbatch) {
+    // one window per block (the grid covers the batch; blocks beyond it return)
+    {
+        const uint32_t wi = bx;
         const uint32_t w = f0 + wi;
-        if (w >= nF) return;
-        if (PERSIST && wi != bx) __syncthreads();  // the previous window's LDS reads are done
+        if (wi >= nbatch || w >= nF) return;
         const int tid = (int)threadIdx.x;
         const bool owner = tid < KU * S;
         int u0, s_t;  // the thread's first unit and its shift
@@ -927,7 +926,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                 st.orc[id] = 0;
                 if (xcand != nullptr) xcand[(size_t)wi * kXStride] = kXDone;
             }
-            continue;
+            return;
         }
         const float* Bw = Bbuf + (size_t)wi * bstride;
         // Staging: every global load below is independent of the others, and
@@ -1462,7 +1461,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                 const bool defer = nr >= 1u && nr <= 64u && nun <= kRU && C <= 64 &&
                                    __popcll(cun_s | ts.cmask[lab]) <= kCV;
                 if (tid == 0) xw[0] = defer ? nr : kXDone;
-                if (defer) continue;  // uniform over the block
+                if (defer) return;  // uniform over the block
             }
             const int upr = max(1, srows / CM);
             const float invCM = 1.0f / (float)CM;
@@ -1867,8 +1866,8 @@ __global__ __launch_bounds__(kSelMaxBlock) void k_screen_select_reg_seg(const Se
 {
     const SegLaunch e = tab[eob[blockIdx.x]];
     const SegDesc& d = sd[e.seg];
-    screen_select_reg_body<LT, true>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
-                                     blockIdx.x - e.blk0, e.nblk, nullptr);
+    screen_select_reg_body<LT>(d.ts, d.st, d.tb, e.a0, e.n, Bbuf + (size_t)e.g0 * bstride, bstride,
+                               blockIdx.x - e.blk0, e.nblk, nullptr);
 }
 
 // Whether the register-resident selection takes this segment, and its block
@@ -1881,9 +1880,8 @@ bool screen_select_reg_plan(const TemplateSet& ts, const ScreenTables& tb, int* 
     return owners <= kSelMaxBlock && *lds <= kSelMaxLds;
 }
 
-// corr: entries' nblk blocks each (grid-stride over their windows); select:
-// bd threads, one block per window when nblk == n, else each block takes
-// windows nblk apart in turn.
+// corr: entries' nblk blocks each (grid-stride over their windows); select: one
+// block per window (nblk >= n; the blocks beyond n return at once), bd threads.
 void launch_seg_screen_corr(int L, const SegDesc* sd, const SegLaunch* tab, uint32_t ntab, uint32_t nblocks,
                             uint16_t* eob, size_t lds, float* Bbuf, size_t bstride, hipStream_t s, GraphRec* rec)
 {

@@ -908,9 +908,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         const uint32_t lab = st.best_lab[id];
         const int64_t b = st.best_idx[id];
         const float bl = st.best_lik[id];
-        // a window whose best spike fell below its threshold or too near the
-        // chunk end takes no overlap recheck (binary_pursuit.cl:518-527); the
-        // label's row count and first row are read with the test's values
+        // the skip test's values, the label's row count and first row, and the
+        // loads that need only the window's label and onset go out together; a
+        // window whose best spike fell below its threshold or too near the
+        // chunk end takes no overlap recheck (binary_pursuit.cl:518-527)
         const float lik_b = st.LIK[(size_t)lab * N + b];
         const float thr_l = ts.thr[lab];
         const int nl = ts.nch[lab];
@@ -918,16 +919,6 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         // first row and row count of unit u = tid (the bound terms below)
         const int q_r0 = tid < U ? ts.off[tid] : 0;
         const int q_nr = tid < U ? ts.nch[tid] : 0;
-        if (tid == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
-        for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
-        __syncthreads();
-        if (lik_b < thr_l || b >= N - L) {
-            if (tid == 0) {
-                st.orc[id] = 0;
-                if (xcand != nullptr) xcand[(size_t)wi * kXStride] = kXDone;
-            }
-            return;
-        }
         const float* Bw = Bbuf + (size_t)wi * bstride;
         // Staging: every global load below is independent of the others, and
         // each load's first round is issued before any value is used, so they
@@ -948,9 +939,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             lv[i] = st.LIK[(size_t)uc * N + (ok ? as_t : 0)];
             bv[i] = Bw[ok ? uc * S + s_t : 0];
         }
-        // first round of the window's samples (eight per thread), of the
-        // label's zero-padded rows (eight per thread), its pair ids and its
-        // A row with the sign of LIK[l][b+f-n]
+        // first round of the window's samples (eight per thread), its pair ids
+        // and its A row with the sign of LIK[l][b+f-n] (b + L <= N - 1 for a
+        // window that is not skipped; clamped otherwise)
+        const int64_t bs = b < N - L ? b : 0;
         int wj = tid / C, wc = tid - wj * C;
         const int wdj = bd / C, wdc = bd - wdj * C;
         float wvv[8];
@@ -960,9 +952,25 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             const bool ok = tid + q * bd < C * L;
             wdst[q] = ok ? wc * VS + wj : -1;
             wcq[q] = wc;
-            wvv[q] = st.V[ok ? (size_t)(b + wj) * C + wc : (size_t)b * C];
+            wvv[q] = st.V[ok && bs + wj < N ? (size_t)(bs + wj) * C + wc : 0];
             qr_step(wj, wc, wdj, wdc, C);
         }
+        const int pid0 = tid < U ? tb.pid[(size_t)lab * U + tid] : -1;
+        const int64_t af0 = b + tid - n;
+        const bool fin0 = tid < S && af0 >= 0 && af0 < N - L;
+        const float la0 = st.LIK[(size_t)lab * N + (fin0 ? af0 : b)];
+        const float A0 = Bw[(int)lab * S + (tid < S ? tid : 0)];
+        if (tid == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
+        for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
+        __syncthreads();
+        if (lik_b < thr_l || b >= N - L) {
+            if (tid == 0) {
+                st.orc[id] = 0;
+                if (xcand != nullptr) xcand[(size_t)wi * kXStride] = kXDone;
+            }
+            return;
+        }
+        // the label's zero-padded rows (eight per thread) and channels
         int rk = tid / LP, ri = tid - rk * LP;
         const int rdk = bd / LP, rdi = bd - rdk * LP;
         float rvv[8];
@@ -976,11 +984,6 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             rvv[q] = in ? rvv[q] : 0.0f;
             qr_step(rk, ri, rdk, rdi, LP);
         }
-        const int pid0 = tid < U ? tb.pid[(size_t)lab * U + tid] : -1;
-        const int64_t af0 = b + tid - n;
-        const bool fin0 = tid < S && af0 >= 0 && af0 < N - L;
-        const float la0 = st.LIK[(size_t)lab * N + (fin0 ? af0 : b)];
-        const float A0 = Bw[(int)lab * S + (tid < S ? tid : 0)];
         const int cl0 = tid < nl ? ts.chan[offl + tid] : 0;
         __builtin_amdgcn_sched_barrier(0);  // every first-round load goes out before any is used
 #pragma unroll
@@ -1068,6 +1071,17 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             if (pl) { atomicMin(&plo_sh, f); atomicMax(&phi_sh, f); }
             A[f] = Bw[(int)lab * S + f];
         }
+        // channels and L1 norms of the first eight rows of unit u = tid (the
+        // bound terms below): in flight across the barrier
+        int qch[8];
+        float ql1[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int r = q_r0 + min(j, max(q_nr, 1) - 1);
+            const bool ok = tid < U && q_nr > 0;
+            qch[j] = ok ? ts.chan[r] : 0;
+            ql1[j] = ok ? tb.rowL1[r] : 0.0f;
+        }
         __syncthreads();  // staging, pair ids, plo / phi, pfirst / npl are complete
         // Gram minima, the range maxima of A and the per-unit bound terms are
         // independent of each other: one phase
@@ -1147,9 +1161,14 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                     float l1[8];
 #pragma unroll
                     for (int j = 0; j < 8; j++) {
-                        const int r = r0 + min(k0 + j, nr - 1);
-                        ch[j] = ts.chan[r];
-                        l1[j] = tb.rowL1[r];
+                        if (k0 == 0 && u == tid) {
+                            ch[j] = qch[j];
+                            l1[j] = ql1[j];
+                        } else {
+                            const int r = r0 + min(k0 + j, nr - 1);
+                            ch[j] = ts.chan[r];
+                            l1[j] = tb.rowL1[r];
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < 8; j++) {

@@ -886,6 +886,24 @@ __global__ __launch_bounds__(kBlock) void k_label_hist(const uint32_t* __restric
         atomicAdd(hist + lab[F[i]], 1u);
 }
 
+// The same with a block-local histogram in LDS (U words): one global atomic
+// per (block, label) instead of one per window.
+__global__ __launch_bounds__(kBlock) void k_label_hist_lds(const uint32_t* __restrict__ F,
+                                                           const uint32_t* __restrict__ cnt_f,
+                                                           const uint32_t* __restrict__ lab, uint32_t* __restrict__ hist,
+                                                           int U)
+{
+    extern __shared__ uint32_t lh[];
+    const uint32_t nF = *cnt_f;
+    for (int u = threadIdx.x; u < U; u += blockDim.x) lh[u] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nF; i += gridDim.x * blockDim.x)
+        atomicAdd(lh + lab[F[i]], 1u);
+    __syncthreads();
+    for (int u = threadIdx.x; u < U; u += blockDim.x)
+        if (lh[u] != 0u) atomicAdd(hist + u, lh[u]);
+}
+
 // hist[U] -> exclusive offsets in place (one block).
 __global__ __launch_bounds__(1024) void k_label_offsets(uint32_t* __restrict__ hist, int U)
 {
@@ -927,10 +945,47 @@ __global__ __launch_bounds__(kBlock) void k_label_scatter(const uint32_t* __rest
     }
 }
 
+// The same with block-local counts in LDS: each block counts its windows per
+// label, reserves one range per label with a single global atomic, and places
+// its windows within the ranges by LDS atomics.  The block takes the same
+// windows in both of its passes (grid-stride, fixed grid).
+__global__ __launch_bounds__(kBlock) void k_label_scatter_lds(const uint32_t* __restrict__ F,
+                                                              const uint32_t* __restrict__ cnt_f,
+                                                              const uint32_t* __restrict__ lab,
+                                                              uint32_t* __restrict__ off, uint32_t* __restrict__ out,
+                                                              int U)
+{
+    extern __shared__ uint32_t lh[];  // [U] counts, then each label's next slot
+    const uint32_t nF = *cnt_f;
+    for (int u = threadIdx.x; u < U; u += blockDim.x) lh[u] = 0u;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nF; i += gridDim.x * blockDim.x)
+        atomicAdd(lh + lab[F[i]], 1u);
+    __syncthreads();
+    for (int u = threadIdx.x; u < U; u += blockDim.x)
+        if (lh[u] != 0u) lh[u] = atomicAdd(off + u, lh[u]);
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nF; i += gridDim.x * blockDim.x) {
+        const uint32_t id = F[i];
+        out[atomicAdd(lh + lab[id], 1u)] = id;
+    }
+}
+
 void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s)
 {
     if (nF == 0) return;
     launch_fill_u32(hist, 0u, U, s);
+    if (U <= 16384) {
+        // about 16 windows per thread: few blocks, so few global atomics per label
+        const unsigned g = (unsigned)std::min<uint32_t>((nF + 16 * kBlock - 1) / (16 * kBlock), 1024u);
+        const size_t lds = (size_t)U * 4;
+        hipLaunchKernelGGL(k_label_hist_lds, dim3(g), dim3(kBlock), lds, s, st.F, st.counters + CNT_F, st.best_lab,
+                           hist, U);
+        hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U);
+        hipLaunchKernelGGL(k_label_scatter_lds, dim3(g), dim3(kBlock), lds, s, st.F, st.counters + CNT_F, st.best_lab,
+                           hist, out, U);
+        return;
+    }
     const unsigned g = (unsigned)std::min<uint32_t>((nF + kBlock - 1) / kBlock, 4096u);
     hipLaunchKernelGGL(k_label_hist, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist);
     hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U);

@@ -35,6 +35,8 @@ void launch_mark_dirty(const TemplateSet& ts, const PassState& st, const uint32_
 void launch_clear_dirty(const PassState& st, uint32_t nW, hipStream_t s);
 void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, hipStream_t s);
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
+void launch_merge_regions(uint32_t* F, const uint32_t* Fs, const uint32_t* rcnt, int nreg, uint32_t rwin, uint32_t* cnt,
+                          hipStream_t s);
 void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s);
 void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
                           uint32_t* blockbuf, hipStream_t s);
@@ -263,7 +265,7 @@ struct fbp_handle {
     int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand, Fs, lhist;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand, Fs, lhist, rcnt;
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
@@ -1181,7 +1183,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
-                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist,
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist, &h->rcnt,
                       &h->mma_qtmp, &h->mma_qch, &h->mma_rslot};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
@@ -1390,6 +1392,26 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     hipEvent_t ev0 = evp.a, ev1 = evp.b;
     HIP_TRY(hipEventRecord(ev0, s), "event");
 
+    fbp_stats stats{};
+    int64_t nz_sum = 0;
+    for (int u = 0; u < h->U; u++) nz_sum += h->h_nch[u];
+    const int reach = (int)(3 + (2 * (int64_t)n_max_shift + L) / L);
+    const uint32_t nchunk = recheck ? recheck_literal_chunks(ts, n_max_shift) : 0;
+    bool use_screen = false;
+    if (recheck && !(flags & FBP_FLAG_RECHECK_LITERAL)) {
+        if (h->tab_n != (int64_t)n_max_shift) {
+            int rc = build_screen_tables(h, n_max_shift, &use_screen);
+            if (rc != FBP_OK) return rc;
+        }
+        use_screen = h->screen_ok;
+    }
+    const uint32_t screen_batch =
+        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
+    screen_mf = screen_mf && use_screen;
+    if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
+    if (use_screen) HIP_TRY(h->xcand.ensure((size_t)screen_batch * kXStride * 4), "alloc rescoring records");
+    if (use_screen) HIP_TRY(h->Fs.ensure((size_t)nwin * 4), "alloc flagged windows by label");
+    if (use_screen) HIP_TRY(h->lhist.ensure((size_t)U * 4), "alloc label counts");
     PassState st = pass_view(h);
     if (!v_on_device) {
         // the voltage copies need not wait for the state fills below
@@ -1416,6 +1438,74 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     // pieces are still copied: window w reads the samples [wL, wL + 2L - 1).
     const bool early_lik = !v_on_device && !mma && !mixed && !(flags & FBP_FLAG_FIRST_PASS_ONLY);
     int64_t early_w = 0;  // windows whose first-pass likelihood is queued
+    // The screened recheck of the flagged windows sp.F[0 .. n) (n read on the
+    // device from sp.counters[CNT_F], at most nF_hi): grouped by label into Fs,
+    // then correlations, selection and deferred rescoring in batches.
+    auto screened = [&](const PassState& sp, uint32_t nF_hi, uint32_t* Fs, hipStream_t q) {
+        launch_sort_by_label(sp, nF_hi, U, Fs, h->lhist.as<uint32_t>(), q);
+        PassState sf = sp;
+        sf.F = Fs;
+        ScreenTables tb = tables_view(h);
+        if (screen_mf) {
+            tb.kappa2 = h->scr_k2;
+            tb.kappa3 = h->scr_k3;
+            tb.kappa4 = h->scr_k4;
+        }
+        for (uint32_t f0 = 0; f0 < nF_hi; f0 += screen_batch) {
+            const uint32_t nb = std::min(screen_batch, nF_hi - f0);
+            if (screen_mf) {
+                launch_screen_mfma(ts, sf, mf_view(h), h->mf_fragl.p, tb.Ntr, tb.S, tb.n, f0, nb, h->Bbuf.as<float>(), q);
+            } else {
+                launch_screen_corr(ts, sf, tb, f0, nb, h->Bbuf.as<float>(), q);
+            }
+            tm.begin(TM_SELECT);
+            const bool deferred = launch_screen_select(ts, sf, tb, f0, nb, h->Bbuf.as<float>(),
+                                                       (flags & FBP_FLAG_SELECT_STREAMING) != 0, q,
+                                                       h->xcand.as<uint32_t>());
+            tm.end(TM_SELECT);
+            stats.n_select_launches++;
+            if (deferred) {
+                tm.begin(TM_RESCORE);
+                launch_screen_rescore(ts, sf, tb, f0, nb, h->xcand.as<uint32_t>(), q);
+                tm.end(TM_RESCORE);
+                stats.n_rescore_launches++;
+            }
+        }
+    };
+    // With host voltage, the first pass's window maxima and screened recheck
+    // also run region by region (kRegionWindows windows each) as soon as a
+    // region's windows and `reach` windows past it have their likelihood; the
+    // regions' flagged lists are merged after the last one.  (On a stream of
+    // their own beside the likelihood and the transposes they made the step
+    // slower: 571 -> 596 ms at 64 channels, 600 s.)
+    constexpr int64_t kRegionWindows = 16384;
+    const bool early_rc = early_lik && use_screen;
+    int64_t rdone = 0;  // windows whose region is queued
+    int nreg = 0;
+    if (early_rc) {
+        HIP_TRY(h->rcnt.ensure((size_t)((nwin + kRegionWindows - 1) / kRegionWindows) * CNT_N * 4), "alloc region counters");
+        launch_fill_u32(h->rcnt.as<uint32_t>(), 0u, ((nwin + kRegionWindows - 1) / kRegionWindows) * CNT_N, s);
+    }
+    auto region = [&](int64_t w1) -> int {
+        const uint32_t n = (uint32_t)(w1 - rdone);
+        PassState sp = st;
+        sp.W = st.W + rdone;
+        sp.F = st.F + rdone;
+        sp.counters = h->rcnt.as<uint32_t>() + (size_t)nreg * CNT_N;
+        const hipStream_t q = s;
+        tm.begin(TM_MAX);
+        launch_window_max(ts, sp, n, q);
+        tm.end(TM_MAX);
+        tm.begin(TM_OTHER);
+        launch_flag_compact(sp, n, h->blockbuf.as<uint32_t>(), q);
+        tm.end(TM_OTHER);
+        tm.begin(TM_RECHECK);
+        screened(sp, n, h->Fs.as<uint32_t>() + rdone, q);
+        tm.end(TM_RECHECK);
+        nreg++;
+        rdone = w1;
+        return FBP_OK;
+    };
     auto early = [&](int64_t x) -> int {
         const int64_t whi = x >= N ? nwin : (x >= 2 * L - 1 ? std::min<int64_t>(nwin, (x - 2 * L + 1) / L + 1) : 0);
         if (whi <= early_w) return FBP_OK;
@@ -1425,6 +1515,13 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         launch_likelihood(ts, sp, (uint32_t)(whi - early_w), nullptr, s);
         tm.end(TM_LIK);
         early_w = whi;
+        // regions whose windows, and `reach` windows past them, have their likelihood
+        while (early_rc && rdone < nwin) {
+            const int64_t w1 = std::min<int64_t>(nwin, rdone + kRegionWindows);
+            if (early_w < std::min<int64_t>(nwin, w1 + reach)) break;
+            const int rc = region(w1);
+            if (rc != FBP_OK) return rc;
+        }
         return FBP_OK;
     };
     // the residual is kept sample-major ([N][C]): a window is one contiguous block
@@ -1433,6 +1530,13 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     } else {
         int rc = stage_in(h, Vsrc, ld, N, st.V, -1, early_lik ? early : std::function<int(int64_t)>(), true);
         if (rc != FBP_OK) return rc;
+        if (early_rc) {
+            if (rdone != nwin) return set_err(h, FBP_E_HIP, "first-pass regions incomplete");
+            tm.begin(TM_OTHER);
+            launch_merge_regions(st.F, h->Fs.as<uint32_t>(), h->rcnt.as<uint32_t>(), nreg, (uint32_t)kRegionWindows,
+                                 st.counters, s);
+            tm.end(TM_OTHER);
+        }
     }
     if (mixed) launch_absmax(st.V, (int64_t)C * N, h->mf_mbits.as<uint32_t>(), h->mf_vs.as<float>(), s);
     MmaVoltage mv{};
@@ -1459,26 +1563,6 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         launch_quantize(st.V, N, (int)C, mv, mma_req, s);
     }
 
-    fbp_stats stats{};
-    int64_t nz_sum = 0;
-    for (int u = 0; u < h->U; u++) nz_sum += h->h_nch[u];
-    const int reach = (int)(3 + (2 * (int64_t)n_max_shift + L) / L);
-    const uint32_t nchunk = recheck ? recheck_literal_chunks(ts, n_max_shift) : 0;
-    bool use_screen = false;
-    if (recheck && !(flags & FBP_FLAG_RECHECK_LITERAL)) {
-        if (h->tab_n != (int64_t)n_max_shift) {
-            int rc = build_screen_tables(h, n_max_shift, &use_screen);
-            if (rc != FBP_OK) return rc;
-        }
-        use_screen = h->screen_ok;
-    }
-    const uint32_t screen_batch =
-        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
-    screen_mf = screen_mf && use_screen;
-    if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
-    if (use_screen) HIP_TRY(h->xcand.ensure((size_t)screen_batch * kXStride * 4), "alloc rescoring records");
-    if (use_screen) HIP_TRY(h->Fs.ensure((size_t)nwin * 4), "alloc flagged windows by label");
-    if (use_screen) HIP_TRY(h->lhist.ensure((size_t)U * 4), "alloc label counts");
     uint32_t nW = (uint32_t)nwin;
     int64_t total = 0;
     int passes = 0;
@@ -1528,14 +1612,20 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             passes = 1;
             break;
         }
-        tm.begin(TM_MAX);
-        launch_window_max(ts, st, nW, s);
-        tm.end(TM_MAX);
+        // (the staged first pass has run its window maxima and screened recheck by regions)
+        const bool region_pass = staged_pass && early_rc;
+        if (!region_pass) {
+            tm.begin(TM_MAX);
+            launch_window_max(ts, st, nW, s);
+            tm.end(TM_MAX);
+        }
         uint32_t nF = 0;
         if (recheck) {
-            tm.begin(TM_OTHER);
-            launch_flag_compact(st, nW, h->blockbuf.as<uint32_t>(), s);
-            tm.end(TM_OTHER);
+            if (!region_pass) {
+                tm.begin(TM_OTHER);
+                launch_flag_compact(st, nW, h->blockbuf.as<uint32_t>(), s);
+                tm.end(TM_OTHER);
+            }
             HIP_TRY(hipMemcpyAsync(h->h_counters, st.counters, CNT_N * 4, hipMemcpyDeviceToHost, s), "read counters");
             HIP_TRY(hipStreamSynchronize(s), "sync");
             nF = h->h_counters[CNT_F];
@@ -1543,37 +1633,12 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
             if (nF > 0) {
                 tm.begin(TM_RECHECK);
                 // the screened recheck and the reassignment take the flagged windows grouped by label
+                // (by region, then label, after the region pass)
                 PassState sf = st;
                 if (use_screen) {
-                    sf.F = h->Fs.as<uint32_t>();
-                    launch_sort_by_label(st, nF, U, sf.F, h->lhist.as<uint32_t>(), s);
-                    const PassState& st = sf;
-                    ScreenTables tb = tables_view(h);
-                    if (screen_mf) {
-                        tb.kappa2 = h->scr_k2;
-                        tb.kappa3 = h->scr_k3;
-                        tb.kappa4 = h->scr_k4;
-                    }
-                    for (uint32_t f0 = 0; f0 < nF; f0 += screen_batch) {
-                        const uint32_t nb = std::min(screen_batch, nF - f0);
-                        if (screen_mf) {
-                            launch_screen_mfma(ts, st, mf_view(h), h->mf_fragl.p, tb.Ntr, tb.S, tb.n, f0, nb,
-                                               h->Bbuf.as<float>(), s);
-                        } else {
-                            launch_screen_corr(ts, st, tb, f0, nb, h->Bbuf.as<float>(), s);
-                        }
-                        tm.begin(TM_SELECT);
-                        const bool deferred = launch_screen_select(ts, st, tb, f0, nb, h->Bbuf.as<float>(),
-                                                                   (flags & FBP_FLAG_SELECT_STREAMING) != 0, s,
-                                                                   h->xcand.as<uint32_t>());
-                        tm.end(TM_SELECT);
-                        stats.n_select_launches++;
-                        if (deferred) {
-                            tm.begin(TM_RESCORE);
-                            launch_screen_rescore(ts, st, tb, f0, nb, h->xcand.as<uint32_t>(), s);
-                            tm.end(TM_RESCORE);
-                            stats.n_rescore_launches++;
-                        }
+                    if (!region_pass) {
+                        sf.F = h->Fs.as<uint32_t>();
+                        screened(st, nF, sf.F, s);
                     }
                 } else {
                     HIP_TRY(h->part_v.ensure((size_t)nF * nchunk * 4), "alloc recheck partials");

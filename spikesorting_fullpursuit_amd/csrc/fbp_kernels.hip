@@ -992,6 +992,38 @@ void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out
     hipLaunchKernelGGL(k_label_scatter, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist, out);
 }
 
+// The first pass's flagged windows, rechecked region by region while the
+// voltage is still being copied (region r: windows [r * rwin, (r + 1) * rwin),
+// its flagged windows grouped by label at Fs + r * rwin, its counters at
+// rcnt + r * CNT_N): one list F in region order, the pass's flagged count, and
+// the regions' recheck statistics added to the run's counters.  Block r copies
+// region r.
+__global__ __launch_bounds__(kBlock) void k_merge_regions(uint32_t* __restrict__ F, const uint32_t* __restrict__ Fs,
+                                                          const uint32_t* __restrict__ rcnt, int nreg, uint32_t rwin,
+                                                          uint32_t* __restrict__ cnt)
+{
+    const int r = blockIdx.x;
+    uint32_t o = 0;
+    for (int q = 0; q < r; q++) o += rcnt[q * CNT_N + CNT_F];
+    const uint32_t c = rcnt[r * CNT_N + CNT_F];
+    for (uint32_t i = threadIdx.x; i < c; i += blockDim.x) F[o + i] = Fs[(size_t)r * rwin + i];
+    if (r == nreg - 1 && threadIdx.x == 0) {
+        cnt[CNT_F] = o + c;
+        const int add[] = {CNT_CAND, CNT_EVAL, CNT_SURV, CNT_NSP};
+        for (int k : add) {
+            uint32_t t = 0;
+            for (int q = 0; q < nreg; q++) t += rcnt[q * CNT_N + k];
+            cnt[k] += t;
+        }
+    }
+}
+
+void launch_merge_regions(uint32_t* F, const uint32_t* Fs, const uint32_t* rcnt, int nreg, uint32_t rwin, uint32_t* cnt,
+                          hipStream_t s)
+{
+    if (nreg > 0) hipLaunchKernelGGL(k_merge_regions, dim3(nreg), dim3(kBlock), 0, s, F, Fs, rcnt, nreg, rwin, cnt);
+}
+
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s)
 {
     SelFlagged sel{st.W, st.orc, st.F};

@@ -404,9 +404,9 @@ def run_batch(args):
 
     tiles, trates = batch.autotune_tiles(segs, sample=32)
     log(f"autotune: tiles {tiles}, Msamples*ch/s by (blocks per segment, windows per block) {trates}")
-    lanes, lrates = batch.autotune_lanes(segs, sample=64, tiles=tiles)
+    lanes, lrates = batch.autotune_lanes(segs, candidates=(2, 4, 6, 8), sample=64, tiles=tiles)
     log(f"autotune: lanes {lanes}, Msamples*ch/s by lanes {lrates}")
-    workers, wrates = batch.autotune_workers(segs, candidates=(2, 4, 8), sample=12)
+    workers, wrates = batch.autotune_workers(segs, candidates=(2, 4, 6, 8), sample=12)
     log(f"autotune: workers {workers}, Msamples*ch/s by workers {wrates}")
     for _ in range(args.warmup):
         batch.binary_pursuit_batch(segs[:16], tiles=tiles, lanes=lanes)

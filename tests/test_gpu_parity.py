@@ -428,3 +428,35 @@ def test_host_input_staged_in_pieces_matches_device_input(handle, name, dur, off
         m = min(L, N - p)
         ref[:, p:p + m] -= T[u, :, :m]
     assert np.max(np.abs(hR - ref)) <= 1e-3 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("literal", ["split", "valu"])
+def test_host_input_first_pass_regions_equal_device_input(handle, literal):
+    """Host input rechecks the first pass region by region (16 384 windows each)
+    as the voltage lands; device input in one launch (both with the vector-unit
+    first likelihood pass, which the regions need).  The 16-channel
+    configuration on 60 s (37 500 windows, three regions): same spikes, labels,
+    order, residual, per-pass trace and recheck statistics."""
+    import torch
+    d = synthetic.generate("probe16", duration_s=60.0)
+    T, bias, thr = synthetic.kernel_inputs(d)
+    n = d["n_max_shift_inds"]
+    V = np.ascontiguousarray(d["voltage"], dtype=np.float32)
+    host = gpu_run(handle, V, T, bias, thr, n, literal=literal)
+    host_log, host_st = handle.pass_log(), handle.stats()
+    assert (V.shape[1] + T.shape[2] - 1) // T.shape[2] > 2 * 16384
+    Vd = torch.tensor(V, device="cuda")
+    flags = _native.FLAG_OVERLAP_RECHECK
+    if literal == "valu":
+        flags |= _native.FLAG_SCREEN_FP32
+    torch.cuda.synchronize()
+    handle.run(None, n, flags, ld=Vd.stride(0), n_samples=Vd.shape[1], device_ptr=Vd.data_ptr())
+    idx, lab = handle.result()
+    np.testing.assert_array_equal(host[0], idx)
+    np.testing.assert_array_equal(host[1], lab)
+    np.testing.assert_array_equal(host[2], handle.residual(V.shape[1]))
+    np.testing.assert_array_equal(host_log, handle.pass_log())
+    dev_st = handle.stats()
+    for k in ("windows_flagged", "recheck_items", "screen_shared_pairs", "screen_separable_pairs"):
+        assert host_st[k] == dev_st[k], k
+    assert len(idx) > 1000

@@ -878,7 +878,6 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     float* zrow = TlP + CM * LP;          // [LP] zeros (a template channel that is absent)
     int* cl = reinterpret_cast<int*>(zrow + LP);  // [CM]
     float* reg = reinterpret_cast<float*>(cl + CM);  // [RG], in turn:
-    float* gmB = reg;                     //   [npl][S] least G over the blocks of f with LIK[l] > 0
     float* gmA = reg + RG / 2;            //   [npl][S] least G over every f
     uint32_t* surv_e = reinterpret_cast<uint32_t*>(reg);  //   survivor lists: pair ...
     float* surv_b = reg + RG / 2;                          //   ... and its B
@@ -1085,27 +1084,11 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         __syncthreads();  // staging, pair ids, plo / phi, pfirst / npl are complete
         // Gram minima, the range maxima of A and the per-unit bound terms are
         // independent of each other: one phase
-        // least G of every pair of the label over every f (gmA) and over the
-        // blocks of f that hold the f with LIK[l][b+f-n] > 0 (gmB; the whole
-        // row when they span > 3 blocks, unset when there is no such f)
+        // least G of every pair of the label over every f (gmA)
         {
-            const int plo = plo_sh, phi = phi_sh, pf = pfirst_sh, np = npl_sh;
-            const int j1 = max(plo, 0) / kGB, j3 = max(phi, 0) / kGB, j2 = min(j1 + 1, j3);
-            const bool wide = j3 - j1 > 2, some = phi >= 0;
+            const int pf = pfirst_sh, np = npl_sh;
 #pragma unroll 4
-            for (int k = tid; k < np * S; k += bd) {
-                const size_t row = (size_t)pf * S + k;
-                const float ga = tb.minG[row];
-                gmA[k] = ga;
-                if (some) {
-                    if (wide) {
-                        gmB[k] = ga;
-                    } else {
-                        const float* g = tb.minGb + row * tb.nbg;
-                        gmB[k] = fminf(fminf(g[j1], g[j2]), g[j3]);
-                    }
-                }
-            }
+            for (int k = tid; k < np * S; k += bd) gmA[k] = tb.minG[(size_t)pf * S + k];
         }
         // range maxima of A over f in [s-L, s+L]: all in-bound f / those with LIK > 0
         if (n <= L) {
@@ -1228,10 +1211,17 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         // each, tracks M' and keeps items with E-hat >= tp_u.  Bound: the
         // largest A over the allowed f within L of s (any in-bound f when
         // LIK[u][b+s-n] > 0, else those with LIK[l][b+f-n] > 0) plus B minus
-        // the least G over the blocks of f that can hold an allowed f.
+        // the least G over the blocks of f that can hold an allowed f.  That
+        // least G is the least over every f (gmA) when every f is allowed or
+        // the f with LIK[l][b+f-n] > 0 span more than 3 blocks, else the least
+        // over their (at most 3) blocks; gmA <= it, so the bound with gmA is
+        // tested first and the block minima are read only for the pairs that
+        // pass it (the same survivors as the block bound alone).
         uint32_t sv = 0;
         {
-            const int pf = pfirst_sh;
+            const int pf = pfirst_sh, plo = plo_sh, phi = phi_sh;
+            const int j1 = max(plo, 0) / kGB, j3 = max(phi, 0) / kGB, j2 = min(j1 + 1, j3);
+            const bool narrow = phi >= 0 && j3 - j1 <= 2;
             const float2* dpu = pair_base();
 #pragma unroll
             for (int i = 0; i < kSelR; i++) {
@@ -1240,9 +1230,12 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                 const bool pu = (pum >> i) & 1u;
                 const float ra = pu ? rall : rpos;
                 const int kk = pw >= 0 ? (pw - pf) * S + s_t : 0;  // row of (label, u) in the tables
-                const float gm = (pu ? gmA : gmB)[kk];
                 const float tp = fmaxf(Mns - dp.x, bl - dp.x);
-                const bool hit = pw >= 0 && ra != -INFINITY && !((ra + bv[i]) - gm < tp);
+                bool hit = pw >= 0 && ra != -INFINITY && !((ra + bv[i]) - gmA[kk] < tp);
+                if (hit && !pu && narrow) {
+                    const float* g = tb.minGb + ((size_t)pf * S + kk) * tb.nbg;
+                    hit = !((ra + bv[i]) - fminf(fminf(g[j1], g[j2]), g[j3]) < tp);
+                }
                 sv |= hit ? 1u << i : 0u;
             }
         }

@@ -92,7 +92,7 @@ typedef struct fbp_stats {
     double ms_rescore;            /* deferred exact rescoring of the overlap candidates (part of ms_recheck),
                                      with FBP_FLAG_TIMING */
     int64_t n_rescore_launches;
-    int64_t n_graph_captures;     /* fbp_run_batch with FBP_FLAG_BATCH_GRAPH: passes captured ... */
+    int64_t n_graph_builds;       /* fbp_run_batch with FBP_FLAG_BATCH_GRAPH: pass graphs built ... */
     int64_t n_graph_replays;      /* ... and replayed (counts of the whole batch) */
 } fbp_stats;
 

@@ -74,7 +74,7 @@ class FbpStats(ctypes.Structure):
         ("n_likelihood_mfma_launches", ctypes.c_int64),
         ("ms_rescore", ctypes.c_double),
         ("n_rescore_launches", ctypes.c_int64),
-        ("n_graph_captures", ctypes.c_int64),
+        ("n_graph_builds", ctypes.c_int64),
         ("n_graph_replays", ctypes.c_int64),
     ]
 

@@ -84,7 +84,7 @@ def _batch_fused(segments, device, stats, tiles, lanes=1, graph=False):
     The chunks are shared out over `lanes` lanes (largest first, each to the
     lane with the least work so far); the lanes run side by side, each its
     groups in turn on its own handles and stream.  graph: every pass after
-    the first is one replay of a hipGraph-captured pass."""
+    the first is one replay of a hipGraph of the pass."""
     t0 = time.perf_counter()
     prepared = []
     for i, seg in enumerate(segments):

@@ -1101,8 +1101,8 @@ struct BatchBufs {
     }
 };
 
-// One pass of a batch captured in a hipGraph (FBP_FLAG_BATCH_GRAPH): launch
-// tables whose blocks are fixed at capture for the segments converging then
+// One pass of a batch as a hipGraph (FBP_FLAG_BATCH_GRAPH): launch tables
+// whose blocks are fixed when the graph is built, for the segments converging then
 // (at most R[g] listed windows each), their block-to-entry maps, and the
 // executable graph.  k_seg_plan writes the entries' item counts on the device
 // as the pass goes, so a replay runs the whole pass without a host read.
@@ -1116,7 +1116,7 @@ struct PassGraph {
     std::vector<uint32_t> R;  // listed windows a segment's launches hold (0: not in the graph)
     uint64_t sumR = 0;
     SegPlan plan{};
-    int64_t captures = 0, replays = 0;
+    int64_t builds = 0, replays = 0;
     void drop()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1858,14 +1858,14 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         return e;
     };
-    // ---- captured passes: every pass after the first replays one hipGraph ----
+    // ---- graph passes: every pass after the first replays one hipGraph ----
     const bool use_graph = (flags & FBP_FLAG_BATCH_GRAPH) != 0;
     PassGraph pg;
     pg.R.assign(nseg, 0);
     // Build the graph of one pass for the segments in `active`, segment g's
     // launches sized for R[g] listed windows (nodes added one by one: no stream
     // capture, so other host threads' calls cannot interleave with it).
-    auto graph_capture = [&](const std::vector<uint32_t>& R, uint64_t sumR) -> int {
+    auto graph_build = [&](const std::vector<uint32_t>& R, uint64_t sumR) -> int {
         pg.drop();
         pg.R = R;
         pg.sumR = sumR;
@@ -1949,7 +1949,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         const GL dec = add(SRC_DECIDE, 1, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
         const GL res = add(SRC_NADD, 2, active, [&](uint32_t g) { return capb(R[g]); });
         const GL mk = add(SRC_NADD, 2, active, [&](uint32_t g) { return capb(cdiv(R[g], kBlock)); });
-        if (over) return set_err(h, FBP_E_UNSUPPORTED, "captured pass table overflow");
+        if (over) return set_err(h, FBP_E_UNSUPPORTED, "graph pass table overflow");
         if (eused > pg.eob_cap) {
             if (pg.eob) HIP_TRY(hipFree(pg.eob), "free graph");
             pg.eob = nullptr;
@@ -2003,7 +2003,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
             pg.exec = nullptr;
             return hip_err(h, ge, "build pass graph");
         }
-        pg.captures++;
+        pg.builds++;
         return FBP_OK;
     };
     // One pass by a replay (building the graph first when a segment lists more windows
@@ -2027,7 +2027,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
                 sumR += R[g];
             }
             if (recheck && sumR > rows_cap) return FBP_OK;
-            int rc = graph_capture(R, sumR);
+            int rc = graph_build(R, sumR);
             if (rc != FBP_OK) return rc;
         }
         // spike lists: room for one more spike per listed window
@@ -2051,7 +2051,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
             nF[g] = recheck ? c[CNT_F] : 0u;
             nadd[g] = c[CNT_SPIKES];
             if (nF[g] > nW[g] || nadd[g] > nW[g])
-                return set_err(h, FBP_E_HIP, "captured pass: counts beyond the listed windows");
+                return set_err(h, FBP_E_HIP, "graph pass: counts beyond the listed windows");
             fbp_stats& sg = stats[g];
             sg.windows_listed += nW[g];
             sg.windows_flagged += nF[g];
@@ -2339,7 +2339,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         sg.recheck_overflow = c[CNT_CAND];
         sg.screen_shared_pairs = c[CNT_SURV];
         sg.screen_separable_pairs = c[CNT_NSP];
-        sg.n_graph_captures = pg.captures;
+        sg.n_graph_builds = pg.builds;
         sg.n_graph_replays = pg.replays;
         hg->stats = sg;
         hg->n_spikes = (int64_t)total[g];

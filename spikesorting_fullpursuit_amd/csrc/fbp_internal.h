@@ -152,9 +152,9 @@ struct SegLaunch {
     uint32_t seg, blk0, nblk, n, a0, g0;
 };
 
-// A captured pass: launch tables whose blocks are fixed at capture and whose
-// item counts a plan kernel writes on the device before each launch reads
-// them.  Range r is cnt[r] entries from off[r]; src[r] says where their n
+// A pass replayed as a hipGraph: launch tables whose blocks are fixed when the
+// graph is built and whose item counts a plan kernel writes on the device
+// before each launch reads them.  Range r is cnt[r] entries from off[r]; src[r] says where their n
 // comes from; phase[r] which plan launch of the pass writes it.
 enum SegSrc { SRC_NW = 0, SRC_NWIN, SRC_NF, SRC_DECIDE, SRC_NADD };
 constexpr int kPlanRanges = 16;

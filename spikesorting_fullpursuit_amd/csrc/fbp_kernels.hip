@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(kBlock) void k_reassign_seg(const SegDesc* __restri
                                                          const uint16_t* __restrict__ eob)
 {
     SEG_ENTRY();
-    if (e.n == 0) return;  // a segment of a captured pass with no flagged windows
+    if (e.n == 0) return;  // a segment of a graph pass with no flagged windows
     reassign_body(d.ts.L, d.st, bx, gx);
 }
 
@@ -1214,7 +1214,7 @@ __global__ __launch_bounds__(kBlock) void k_check_snapshot_seg(const SegDesc* __
                                                                const uint16_t* __restrict__ eob)
 {
     SEG_ENTRY();
-    if (e.n == 0) return;  // a segment of a captured pass that has converged
+    if (e.n == 0) return;  // a segment of a graph pass that has converged
     copy_bytes_body(d.st.check, d.st.check_snap, d.st.nwin, bx, gx);
 }
 
@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_counters(const SegDesc* __restri
     }
 }
 
-// Item counts of a captured pass's entries (phase 0 at the start of the pass,
+// Item counts of a graph pass's entries (phase 0 at the start of the pass,
 // 1 after the flagged windows are counted, 2 after this pass's spikes are):
 // segments that have converged get none.
 __global__ __launch_bounds__(kBlock) void k_seg_plan(const SegDesc* __restrict__ sd, const SegDyn* __restrict__ dy,

@@ -848,6 +848,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     __shared__ uint32_t nsurv_s, nsurvp_s, nrow_s, nresc_s;
     __shared__ unsigned long long cun_s;  // channels of the candidates' units (deferred rescoring)
     __shared__ float Mp_sh;  // max over the enumerated shared-channel items of E-hat - delta_u
+    __shared__ uint32_t nstat_s[2];  // the window's shared-channel / separable survivors (statistics)
     const int C = ts.C, U = ts.U;
     const int L = LT > 0 ? LT : ts.L;
     const int S = tb.S, n = tb.n, CM = tb.cmax;
@@ -959,7 +960,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         const bool fin0 = tid < S && af0 >= 0 && af0 < N - L;
         const float la0 = st.LIK[(size_t)lab * N + (fin0 ? af0 : b)];
         const float A0 = Bw[(int)lab * S + (tid < S ? tid : 0)];
-        if (tid == 0) { ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0; }
+        if (tid == 0) {
+            ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0;
+            nstat_s[0] = 0u; nstat_s[1] = 0u;
+        }
         for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
         __syncthreads();
         if (lik_b < thr_l || b >= N - L) {
@@ -1253,7 +1257,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         const int plo = plo_sh, phi = phi_sh;
         {
             const int c = wave_sum_u32(__popc(sv));
-            if (lane == 0 && c) atomicAdd(st.counters + CNT_SURV, (uint32_t)c);
+            if (lane == 0 && c) atomicAdd(&nstat_s[0], (uint32_t)c);
         }
         // Surviving pairs go to two lists with their B: pairs with
         // LIK[u][b+s-n] <= 0 allow only the f with LIK[l][b+f-n] > 0, all in
@@ -1320,7 +1324,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                     sv |= hit ? 1u << i : 0u;
                 }
                 const int c = wave_sum_u32(__popc(sv));
-                if (lane == 0 && c) atomicAdd(st.counters + CNT_NSP, (uint32_t)c);
+                if (lane == 0 && c) atomicAdd(&nstat_s[1], (uint32_t)c);
             }
             if (tid == 0) { nsurv_s = 0; nsurvp_s = 0; }
             __syncthreads();
@@ -1416,6 +1420,11 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         }
         const float Mp = fmaxf(Mns, Mp_sh);
         __syncthreads();
+        // one global add per window and statistic (not one per wave: every block adds to the same words)
+        if (tid == 0) {
+            if (nstat_s[0]) atomicAdd(st.counters + CNT_SURV, nstat_s[0]);
+            if (nstat_s[1]) atomicAdd(st.counters + CNT_NSP, nstat_s[1]);
+        }
         const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
         float bv_best = 0.0f;
         uint32_t bk = kNoItem;

@@ -963,8 +963,10 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         if (tid == 0) {
             ncand_s = 0; plo_sh = S; phi_sh = -1; pfirst_sh = 0x7fffffff; npl_sh = 0;
             nstat_s[0] = 0u; nstat_s[1] = 0u;
+            nrow_s = 0; nresc_s = 0; cun_s = 0ull;  // the rescoring's staging slots (its own LDS)
         }
         for (int c = tid; c < C; c += bd) vmax[c] = 0.0f;
+        for (int u = tid; u < U; u += bd) slot_of[u] = -1;
         __syncthreads();
         if (lik_b < thr_l || b >= N - L) {
             if (tid == 0) {
@@ -1454,9 +1456,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             // candidates reaching theta_u, each rescored exactly by one thread;
             // the zero-padded rows of their units are staged in LDS, upr units
             // per round
-            for (int u = tid; u < U; u += bd) slot_of[u] = -1;
-            if (tid == 0) { nrow_s = 0; nresc_s = 0; cun_s = 0ull; }
-            __syncthreads();
+            // (slot_of, nrow_s, nresc_s and cun_s were reset with the window's other state)
             uint32_t* xw = xcand != nullptr ? xcand + (size_t)wi * kXStride : nullptr;
             for (uint32_t i = tid; i < nc; i += bd) {
                 const int u = (int)(cand_k[i] % (uint32_t)U);

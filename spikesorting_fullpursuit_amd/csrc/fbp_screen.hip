@@ -847,7 +847,6 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
     __shared__ int pfirst_sh, npl_sh;  // pair ids of the label: pfirst .. pfirst + npl - 1
     __shared__ uint32_t nsurv_s, nsurvp_s, nrow_s, nresc_s;
     __shared__ unsigned long long cun_s;  // channels of the candidates' units (deferred rescoring)
-    __shared__ float Mp_sh;  // max over the enumerated shared-channel items of E-hat - delta_u
     __shared__ uint32_t nstat_s[2];  // the window's shared-channel / separable survivors (statistics)
     const int C = ts.C, U = ts.U;
     const int L = LT > 0 ? LT : ts.L;
@@ -1268,10 +1267,11 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         // Phase 2 (shared channel, G term, threshold tp_u, tracks M') and
         // phase 3 (no shared channel, threshold theta_u) use the same code.
         const uint32_t capN = survcap * 3 / 4, capP = survcap - capN;
+        float Msh = -INFINITY;  // max over the enumerated shared-channel items of E-hat - delta_u (every thread)
 #pragma unroll 1
         for (int ph = 2; ph <= 3; ph++) {
             const bool shared = ph == 2;
-            const float Mref = shared ? Mns : fmaxf(Mns, Mp_sh);
+            const float Mref = shared ? Mns : fmaxf(Mns, Msh);
             auto thr_u = [&](int u) { return fmaxf(Mref - Dp[u].x, bl - Dp[u].x); };
             auto gterm = [&](int u, int s, int f) -> float {
                 return shared ? tb.G[((size_t)__float_as_int(Dp[u].y) * S + s) * tb.sg + f] : 0.0f;
@@ -1414,14 +1414,11 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
                     eval_pl(e, Bu);
                 }
             }
-            if (shared) {
-                const float m2 = block_max_n(msh, red, nw);
-                if (tid == 0) Mp_sh = m2;
-                __syncthreads();
-            }
+            if (shared) Msh = block_max_n(msh, red, nw);  // every thread gets the block's maximum
         }
-        const float Mp = fmaxf(Mns, Mp_sh);
-        __syncthreads();
+        const float Mp = fmaxf(Mns, Msh);
+        // the barrier of the overflow vote also completes the candidate list and the statistics
+        const bool ovf = __syncthreads_or(overflow);
         // one global add per window and statistic (not one per wave: every block adds to the same words)
         if (tid == 0) {
             if (nstat_s[0]) atomicAdd(st.counters + CNT_SURV, nstat_s[0]);
@@ -1430,7 +1427,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
         const uint32_t nc = min(ncand_s, (uint32_t)kCandCap);
         float bv_best = 0.0f;
         uint32_t bk = kNoItem;
-        if (__syncthreads_or(overflow)) {
+        if (ovf) {
             // more candidates than the list holds: enumerate and rescore in place
             if (xcand != nullptr && tid == 0) xcand[(size_t)wi * kXStride] = kXDone;
             for (int e = tid; e < US; e += bd) {
@@ -1536,7 +1533,7 @@ __device__ __forceinline__ void screen_select_reg_body(TemplateSet ts, PassState
             }
         }
         wave_argmax(bv_best, bk);
-        __syncthreads();
+        // (red was last read in the phase-2 maximum, several barriers ago)
         if (lane == 0) { red[wv] = bv_best; redk[wv] = bk; }
         __syncthreads();
         if (tid == 0) {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <condition_variable>
 #include <cstdio>
 #include <cmath>
@@ -78,7 +79,7 @@ void launch_screen_rescore(const TemplateSet& ts, const PassState& st, const Scr
                            uint32_t nbatch, const uint32_t* xcand, hipStream_t s);
 // MFMA likelihood tables (fbp_likmfma.hip).
 constexpr int kMfMaxBlk = 32;
-constexpr double kMfPadMax = 2.5;  // tile channel rows per template channel row up to which the matrix cores pay
+constexpr double kMfPadMax = 2.0;  // tile channel rows per template channel row up to which the matrix cores pay
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 struct MfTables {
     const half8* frag;
@@ -87,6 +88,7 @@ struct MfTables {
     const int* nblk;
     const int* aoff;
     const int* chb;
+    const int* nunit;
     const float* T1;
     const int* tileof;
     int ntiles;
@@ -269,7 +271,7 @@ struct fbp_handle {
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
-    DevBuf mf_frag, mf_fragl, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_T1, mf_mbits, mf_vs, mf_tileof;
+    DevBuf mf_frag, mf_fragl, mf_units, mf_nkb, mf_nblk, mf_aoff, mf_chb, mf_nunit, mf_T1, mf_mbits, mf_vs, mf_tileof;
     int mf_ntiles = 0;
     float mf_tscale = 1.0f, mf_kl = 0.0f, mf_kb = 0.0f, mf_ka = 0.0f;
     float scr_k2 = 0.0f, scr_k3 = 0.0f, scr_k4 = 0.0f;  // screen bound terms of the split-fp16 B
@@ -485,8 +487,10 @@ static ScreenTables tables_view(const fbp_handle* h)
 }
 
 // Matrix-core tables (likelihood and overlap screen): templates ordered by
-// their first non-zero channel, cut into tiles of 16; each tile covers the
-// contiguous 8-channel blocks from its first to its last non-zero channel; the
+// their first non-zero channel, cut into tiles of at most 16 where the k-blocks
+// summed over the tiles are fewest (dynamic programming over the cut points);
+// each tile covers the contiguous 8-channel blocks from its first to its last
+// non-zero channel; the
 // A operand in fragment order (tile, k-block of 32, lane, 8 values), k =
 // (sample j, block b, channel in block) with run 4 * kb + lane / 16 =
 // j * blocks + b, as fp16 hi parts and the lo parts of the remainder.  mf_ok
@@ -500,21 +504,49 @@ static int build_mf_tables(fbp_handle* h)
     if (L > 64 || (C + 7) / 8 > kMfMaxBlk) return FBP_OK;
     std::vector<float> hT((size_t)U * C * L);
     HIP_TRY(hipMemcpy(hT.data(), h->T.p, hT.size() * 4, hipMemcpyDeviceToHost), "copy templates");
-    std::vector<int> first(U, C), order(U);
+    std::vector<int> first(U, C), lastc(U, -1), order(U);
     for (int u = 0; u < U; u++) {
         for (int c = 0; c < C; c++)
-            if (h->h_nz[(size_t)u * C + c]) { first[u] = c; break; }
+            if (h->h_nz[(size_t)u * C + c]) {
+                if (first[u] == C) first[u] = c;
+                lastc[u] = c;
+            }
         order[u] = u;
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return first[a] < first[b]; });
-    const int ntiles = (U + 15) / 16;
+    // cost[j]: fewest k-blocks of the first j sorted templates; cut[j]: start of the last tile
+    // (ties go to the larger tile)
+    std::vector<int> cost(U + 1, INT_MAX), cut(U + 1, 0);
+    cost[0] = 0;
+    for (int j = 1; j <= U; j++) {
+        int blo = kMfMaxBlk, bhi = -1;
+        for (int i = j - 1; i >= std::max(0, j - 16); i--) {
+            const int u = order[i];
+            if (lastc[u] >= 0) {
+                blo = std::min(blo, first[u] / 8);
+                bhi = std::max(bhi, lastc[u] / 8);
+            }
+            const int nb = bhi < 0 ? 1 : bhi - blo + 1;
+            const int c = cost[i] + (L * nb + 3) / 4;
+            if (c <= cost[j]) {
+                cost[j] = c;
+                cut[j] = i;
+            }
+        }
+    }
+    std::vector<std::pair<int, int>> runs;  // [begin, end) of each tile in the sorted order
+    for (int j = U; j > 0; j = cut[j]) runs.emplace_back(cut[j], j);
+    std::reverse(runs.begin(), runs.end());
+    const int ntiles = (int)runs.size();
     if (likmfma_lds(template_view(h), ntiles) > kMaxLdsSelect) return FBP_OK;  // beyond 160 KiB: FP32 kernels
-    std::vector<int> units((size_t)ntiles * 16, -1), nkb(ntiles), nblk(ntiles), aoff(ntiles), chb(ntiles, 0);
+    std::vector<int> units((size_t)ntiles * 16, -1), nkb(ntiles), nblk(ntiles), aoff(ntiles), chb(ntiles, 0),
+        nunit(ntiles);
     int total_kb = 0, maxnb = 1;
     for (int t = 0; t < ntiles; t++) {
         int blo = kMfMaxBlk, bhi = -1;
-        for (int r = 0; r < 16 && t * 16 + r < U; r++) {
-            const int u = order[t * 16 + r];
+        nunit[t] = runs[t].second - runs[t].first;
+        for (int r = 0; r < nunit[t]; r++) {
+            const int u = order[runs[t].first + r];
             units[(size_t)t * 16 + r] = u;
             for (int c = 0; c < C; c++)
                 if (h->h_nz[(size_t)u * C + c]) {
@@ -570,6 +602,7 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(h->mf_nblk.ensure((size_t)ntiles * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_aoff.ensure((size_t)ntiles * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_chb.ensure(chb.size() * 4), "alloc likelihood tables");
+    HIP_TRY(h->mf_nunit.ensure(nunit.size() * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_T1.ensure((size_t)U * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_tileof.ensure((size_t)U * 4), "alloc likelihood tables");
     HIP_TRY(h->mf_mbits.ensure(4), "alloc likelihood tables");
@@ -581,6 +614,7 @@ static int build_mf_tables(fbp_handle* h)
     HIP_TRY(hipMemcpy(h->mf_nblk.p, nblk.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_aoff.p, aoff.data(), (size_t)ntiles * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_chb.p, chb.data(), chb.size() * 4, hipMemcpyHostToDevice), "copy");
+    HIP_TRY(hipMemcpy(h->mf_nunit.p, nunit.data(), nunit.size() * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_T1.p, T1.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
     HIP_TRY(hipMemcpy(h->mf_tileof.p, tileof.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy");
     h->mf_ntiles = ntiles;
@@ -641,6 +675,7 @@ static MfTables mf_view(const fbp_handle* h)
     mt.nblk = h->mf_nblk.as<int>();
     mt.aoff = h->mf_aoff.as<int>();
     mt.chb = h->mf_chb.as<int>();
+    mt.nunit = h->mf_nunit.as<int>();
     mt.T1 = h->mf_T1.as<float>();
     mt.tileof = h->mf_tileof.as<int>();
     mt.ntiles = h->mf_ntiles;
@@ -1182,7 +1217,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->counters, &h->blockbuf, &h->part_v, &h->part_k, &h->sp_idx, &h->sp_lab, &h->clips,
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
-                      &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
+                      &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->mf_nunit, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
                       &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist, &h->rcnt,
                       &h->mma_qtmp, &h->mma_qch, &h->mma_rslot};
     for (DevBuf* b : bufs) b->release();

@@ -44,6 +44,7 @@ struct MfTables {
     const int* nblk;    // [ntiles] 8-channel blocks
     const int* aoff;    // [ntiles] first k-block of the tile in frag
     const int* chb;     // [ntiles] first 8-channel block (the tile's blocks are contiguous)
+    const int* nunit;   // [ntiles] templates of the tile (its first rows)
     const float* T1;    // [U] sum of |T| over the template (rounded up)
     const int* tileof;  // [U] tile holding each template
     int ntiles;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(kMfThreads) __attribute__((amdgpu_waves_per_eu(2, 2
             // (window, template) pairs computed, for the statistics
             int nwv_ok = 0, units_on = 0;
             for (int k = 0; k < kMfWin; k++) nwv_ok += wid_s[k] >= 0 ? 1 : 0;
-            for (int i = 0; i < na; i++) units_on += min(16, U - 16 * tlist[i]);
+            for (int i = 0; i < na; i++) units_on += mt.nunit[tlist[i]];
             if (na) atomicAdd(st.counters + CNT_PAIRS, (uint32_t)(units_on * nwv_ok));
         }
         __syncthreads();

@@ -18,7 +18,8 @@ namespace fbp {
 // register window over the voltage row staged in LDS.
 // ---------------------------------------------------------------------------
 
-constexpr int kLikMaxWb = 16;  // windows per likelihood block at most
+constexpr int kLikMaxWb = 16;  // windows per likelihood block at most (pair entries keep the window in 4 bits)
+static_assert(kLikMaxWb <= 16, "pair entries t * 16 + b");
 
 template <int LT>
 __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __restrict__ V, int64_t N,
@@ -30,8 +31,8 @@ __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __r
                                                        const uint8_t* __restrict__ gmask, uint32_t bx, uint32_t gx)
 {
     // [wb][C][sv] voltage rows, then per-item maxima [wb][U][ngr] (value, position),
-    // then the block's (window, template) pairs to compute [wb][U], then the
-    // channels each window stages [wb][C]
+    // then the block's (window b, template t) pairs to compute [wb][U] (entries
+    // t * 16 + b), then the channels each window stages [wb][C]
     extern __shared__ __align__(16) float sV[];
     __shared__ uint32_t npair_s;
     __shared__ unsigned long long cun_s[kLikMaxWb];
@@ -56,17 +57,27 @@ __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __r
         // (every pair without a dirty map); the others keep LIK and maxima
         // gmask (exact pass after a matrix-core approximation): only the pairs
         // with 8-position groups to rescore, and only those groups
-        for (int pr = threadIdx.x; pr < nb * U; pr += blockDim.x) {
-            const int b = pr / U, t = pr - b * U;
-            if (gmask != nullptr && gmask[(size_t)W[w0 + b] * U + t] == 0) continue;
+        // (b, t) of pr = b * U + t advanced by the block size: one division per thread
+        const int bd = blockDim.x, db = bd / U, dt = bd - db * U;
+        int rb = threadIdx.x / U, rt = threadIdx.x - (int)(threadIdx.x / U) * U;
+        for (int pr = threadIdx.x; pr < nb * U; pr += bd) {
+            const int bb = rb, tt = rt;
+            rt += dt;
+            rb += db;
+            if (rt >= U) {
+                rt -= U;
+                ++rb;
+            }
+            if (gmask != nullptr && gmask[(size_t)W[w0 + bb] * U + tt] == 0) continue;
+            const uint32_t ent = (uint32_t)tt * 16u + (uint32_t)bb;
             if (dirty == nullptr) {
-                plist[atomicAdd(&npair_s, 1u)] = (uint32_t)pr;
-                if (gmask != nullptr && C <= 64) atomicOr(&cun_s[b], (unsigned long long)ts.cmask[t]);
+                plist[atomicAdd(&npair_s, 1u)] = ent;
+                if (gmask != nullptr && C <= 64) atomicOr(&cun_s[bb], (unsigned long long)ts.cmask[tt]);
             } else {
-                const uint64_t m = ts.cmask[t];
-                if ((m & dirty[W[w0 + b]]) != 0ull) {
-                    plist[atomicAdd(&npair_s, 1u)] = (uint32_t)pr;
-                    atomicOr(&cun_s[b], (unsigned long long)m);
+                const uint64_t m = ts.cmask[tt];
+                if ((m & dirty[W[w0 + bb]]) != 0ull) {
+                    plist[atomicAdd(&npair_s, 1u)] = ent;
+                    atomicOr(&cun_s[bb], (unsigned long long)m);
                 }
             }
         }
@@ -97,25 +108,38 @@ __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __r
         // store channel rows; several loads in flight per thread
         for (int b = 0; b < nb; b++) {
             const int ncl = ncl_s[b];
+            if (ncl == 0) continue;  // uniform over the block
             const int tot = ncl * sv;
             const int64_t xb = (int64_t)W[w0 + b] * L;
+            // element e = j * ncl + qi (sample j, listed quad or channel qi), advanced by the
+            // block size one step at a time: two divisions per window, none per element
+            const int dj = bd / ncl, dq = bd - dj * ncl;
+            int j = threadIdx.x / ncl, qi = threadIdx.x - (int)(threadIdx.x / ncl) * ncl;
+            auto step = [&]() {
+                qi += dq;
+                j += dj;
+                if (qi >= ncl) {
+                    qi -= ncl;
+                    ++j;
+                }
+            };
             if (quads) {
-                for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * blockDim.x) {
+                for (int e0 = threadIdx.x; e0 < tot; e0 += 4 * bd) {
                     float4 v[4];
                     int c0[4], jj[4];
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
-                        const int e = e0 + q * blockDim.x;
+                        const int e = e0 + q * bd;
                         v[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         c0[q] = -1;
                         jj[q] = 0;
                         if (e < tot) {
-                            const int j = e / ncl, qi = e - j * ncl;
                             const int64_t x = xb + j;
                             c0[q] = 4 * clist[b * C + qi];
                             jj[q] = j;
                             if (x < N) v[q] = *reinterpret_cast<const float4*>(V + (size_t)x * C + c0[q]);
                         }
+                        step();
                     }
 #pragma unroll
                     for (int q = 0; q < 4; q++) {
@@ -129,33 +153,32 @@ __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __r
                 }
                 continue;
             }
-            for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * blockDim.x) {
+            for (int e0 = threadIdx.x; e0 < tot; e0 += 8 * bd) {
                 float v[8];
+                int dst[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
-                    const int e = e0 + q * blockDim.x;
+                    const int e = e0 + q * bd;
                     v[q] = 0.0f;
+                    dst[q] = -1;
                     if (e < tot) {
-                        const int j = e / ncl, ci = e - j * ncl;
+                        const int c = clist[b * C + qi];
                         const int64_t x = xb + j;
-                        if (x < N) v[q] = V[(size_t)x * C + clist[b * C + ci]];
+                        dst[q] = (b * C + c) * sv + j;
+                        if (x < N) v[q] = V[(size_t)x * C + c];
                     }
+                    step();
                 }
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    const int e = e0 + q * blockDim.x;
-                    if (e < tot) {
-                        const int j = e / ncl, ci = e - j * ncl;
-                        sV[(b * C + clist[b * C + ci]) * sv + j] = v[q];
-                    }
-                }
+                for (int q = 0; q < 8; q++)
+                    if (dst[q] >= 0) sV[dst[q]] = v[q];
             }
         }
         __syncthreads();
         for (int it = threadIdx.x; it < npair * ngr; it += blockDim.x) {
             const int j = it / ngr, g = it - j * ngr;
-            const int pr = (int)plist[j];
-            const int b = pr / U, t = pr - b * U;
+            const uint32_t pr = plist[j];
+            const int b = (int)(pr & 15u), t = (int)(pr >> 4);
             if (gmask != nullptr && ((gmask[(size_t)W[w0 + b] * U + t] >> g) & 1) == 0) {
                 pv[it] = -INFINITY;  // its approximations are negative: no window maximum there
                 pp[it] = 0xffffu;
@@ -193,8 +216,8 @@ __device__ __forceinline__ void likelihood_body(TemplateSet ts, const float* __r
         __syncthreads();
         // per computed (window, template): first position of the window maximum
         for (int j = threadIdx.x; j < npair; j += blockDim.x) {
-            const int pr = (int)plist[j];
-            const int b = pr / U, t = pr - b * U;
+            const uint32_t pr = plist[j];
+            const int b = (int)(pr & 15u), t = (int)(pr >> 4);
             float mv = -INFINITY;
             uint32_t mp = 0xffffu;
             for (int g = 0; g < ngr; g++) {

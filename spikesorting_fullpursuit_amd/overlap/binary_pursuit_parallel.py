@@ -190,11 +190,17 @@ def assemble_outputs(secret_spike_indices, secret_spike_labels, secret_spike_boo
     """The drop-in's return arrays from its per-chunk lists
     (binary_pursuit_parallel.py:1506-1532)."""
     if len(secret_spike_indices) > 0:
-        event_indices = np.int64(np.hstack(secret_spike_indices))
-        neuron_labels = np.int64(np.hstack(secret_spike_labels))
+        # one pass per output array: a single chunk is converted directly (new arrays,
+        # the inputs are not modified), several are joined first
+        if len(secret_spike_indices) == 1:
+            event_indices = np.add(secret_spike_indices[0], clip_init_samples, dtype=np.int64)
+            neuron_labels = np.asarray(secret_spike_labels[0]).astype(np.int64)
+        else:
+            event_indices = np.hstack(secret_spike_indices).astype(np.int64, copy=False)
+            event_indices += clip_init_samples
+            neuron_labels = np.hstack(secret_spike_labels).astype(np.int64, copy=False)
         adjusted_clips = (np.vstack(adjusted_spike_clips)).astype(v_dtype)
         binary_pursuit_spike_bool = np.hstack(secret_spike_bool)
-        event_indices += clip_init_samples
         return event_indices, neuron_labels, binary_pursuit_spike_bool, adjusted_clips
     return [], [], [], []
 
@@ -264,7 +270,10 @@ def binary_pursuit(voltage, v_dtype, sort_info, separability_metrics, n_max_shif
             residuals.append(h.residual(stop_index - start_index))
         if idx.shape[0] > 0:
             clips = h.clips(idx.shape[0]) if get_adjusted_clips else []
-            secret_spike_indices.append(idx.astype(np.int64) + start_index)
+            chunk_indices = idx.astype(np.int64)
+            if start_index:
+                chunk_indices += start_index
+            secret_spike_indices.append(chunk_indices)
             secret_spike_labels.append(lab)
             adjusted_spike_clips.append(clips)
             secret_spike_bool.append(np.ones(idx.shape[0], dtype="bool"))

@@ -53,8 +53,11 @@ def add_n_spikes(separability_metrics, neuron_labels):
     labels = np.asarray(neuron_labels)
     counts = np.zeros(n_templates)
     if labels.size:
-        valid = labels[(labels >= 0) & (labels < n_templates)].astype(np.int64)
-        counts += np.bincount(valid, minlength=n_templates)[:n_templates]
+        if np.issubdtype(labels.dtype, np.integer) and labels.min() >= 0 and labels.max() < n_templates:
+            counts += np.bincount(labels, minlength=n_templates)[:n_templates]  # every label valid
+        else:
+            valid = labels[(labels >= 0) & (labels < n_templates)].astype(np.int64)
+            counts += np.bincount(valid, minlength=n_templates)[:n_templates]
     separability_metrics["n_spikes"] = counts
     return None
 

@@ -266,8 +266,8 @@ class Handle:
     def result(self):
         n = ctypes.c_int64(0)
         self._check(self._lib.fbp_result_count(self._h, ctypes.byref(n)))
-        idx = np.zeros(n.value, np.uint32)
-        lab = np.zeros(n.value, np.uint32)
+        idx = np.empty(n.value, np.uint32)  # every entry is written by the copy
+        lab = np.empty(n.value, np.uint32)
         self._check(self._lib.fbp_copy_result(self._h, idx.ctypes.data, lab.ctypes.data))
         return idx, lab
 

@@ -186,6 +186,30 @@ def test_mixed_precision_shapes_match_oracle(handle, C, L, U, N):
                 fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
 
 
+@pytest.mark.parametrize("groups", [(12, 12), (5, 20, 9)])
+def test_mixed_precision_partial_tiles_match_oracle(handle, groups):
+    """Templates on disjoint 8-channel blocks, in groups that do not fill a
+    16-template tile: the matrix-core tables cut the tiles where their summed
+    k-blocks are fewest, so tiles with unused rows sit between full ones (first
+    likelihood pass and screen correlations on the matrix cores)."""
+    rng = np.random.default_rng(sum(groups))
+    U, C, L, N = sum(groups), 8 * len(groups), 24, 4000
+    T = cases.dog_templates(U, C, L, 3).astype(np.float32)
+    u0 = 0
+    for g, cnt in enumerate(groups):  # group g on channels 8g .. 8g + 7 only
+        keep = np.zeros(C, bool)
+        keep[8 * g:8 * g + 8] = True
+        T[u0:u0 + cnt][:, ~keep, :] = 0.0
+        u0 += cnt
+    V = rng.normal(size=(C, N)).astype(np.float32) * 0.05
+    for k, p in enumerate(range(30, N - 2 * L, 37)):
+        V[:, p:p + L] += T[(7 * k) % U] * np.float32(0.8 + 0.4 * rng.random())
+    bias, thr = cases.bias_thr(T.astype(np.float64), 0.05)
+    n = L - 1
+    assert_same(gpu_run(handle, V, T, bias, thr, n, literal="split", mixed=True),
+                fbp_oracle.pursuit(V, T, bias, thr, n, return_residual=True))
+
+
 @pytest.mark.parametrize("literal", ["split", "valu"])
 @pytest.mark.parametrize("seed", [0, 1])
 def test_channel_dynamic_range_matches_oracle(handle, seed, literal):

@@ -118,6 +118,11 @@ void launch_screen_mfma(const TemplateSet& ts, const PassState& st, const MfTabl
 // Limits of the screened recheck (beyond them the literal recheck runs).
 static const size_t kMaxGramBytes = (size_t)8 << 30;
 static const size_t kMaxBbufBytes = (size_t)2 << 30;
+// One recording's screen batch: flagged windows whose B rows a batch holds at most (each
+// batch ends its three kernels on a partial round of blocks, and the label-grouped order
+// puts a label's costly windows together, so fewer, larger batches pay)
+static const size_t kScreenBatchBytes = (size_t)8 << 30;
+static const size_t kScreenBatchWindows = 262144;
 static const size_t kMaxLds = 64 * 1024;
 static const size_t kMaxLdsSelect = 160 * 1024;
 
@@ -1441,7 +1446,8 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         use_screen = h->screen_ok;
     }
     const uint32_t screen_batch =
-        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(65536, kMaxBbufBytes / ((size_t)U * S * 4))) : 0;
+        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(kScreenBatchWindows, kScreenBatchBytes / ((size_t)U * S * 4)))
+                   : 0;
     screen_mf = screen_mf && use_screen;
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");
     if (use_screen) HIP_TRY(h->xcand.ensure((size_t)screen_batch * kXStride * 4), "alloc rescoring records");

@@ -39,6 +39,8 @@ def release_device_buffers():
 # Limits of libfbp_hip.so's buffers (csrc/fbp_api.hip) used by library_device_bytes.
 _MAX_GRAM_BYTES = 8 << 30
 _MAX_SCREEN_BUFFER_BYTES = 2 << 30
+_SCREEN_BATCH_BYTES = 8 << 30
+_SCREEN_BATCH_WINDOWS = 262144
 _STAGING_BYTES = 3 * (32 << 20)
 # Bytes per window besides the per-template maxima: best index / label / value and
 # the overlap copies (5 x 4), five flag bytes, four window lists (4 x 4), the
@@ -59,7 +61,8 @@ def library_device_bytes(n_chans, n_samples, n_templates, template_samples_per_c
         gram = 0  # the literal recheck runs instead
     screen = 0
     if U * S * 4 <= _MAX_SCREEN_BUFFER_BYTES:
-        screen = min(65536, max(1, _MAX_SCREEN_BUFFER_BYTES // (U * S * 4))) * U * S * 4 + U * S * 4
+        batch = min(_SCREEN_BATCH_WINDOWS, max(1, _SCREEN_BATCH_BYTES // (U * S * 4)))
+        screen = batch * U * S * 4 + U * S * 4 + batch * 65 * 4  # B rows, Ntr, deferred-rescoring records
     templates = 3 * U * C * L * 4
     constant = gram + screen + templates + _STAGING_BYTES
     # BF16 likelihood (the largest of the precisions): the quantized voltage (2 C), its

@@ -1445,8 +1445,10 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
         }
         use_screen = h->screen_ok;
     }
+    // (no more rows than the chunk has windows)
     const uint32_t screen_batch =
-        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>(kScreenBatchWindows, kScreenBatchBytes / ((size_t)U * S * 4)))
+        use_screen ? (uint32_t)std::max<size_t>(1, std::min<size_t>({kScreenBatchWindows, (size_t)nwin,
+                                                                     kScreenBatchBytes / ((size_t)U * S * 4)}))
                    : 0;
     screen_mf = screen_mf && use_screen;
     if (use_screen) HIP_TRY(h->Bbuf.ensure((size_t)screen_batch * U * S * 4), "alloc screen buffer");

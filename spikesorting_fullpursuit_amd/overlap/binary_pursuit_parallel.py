@@ -61,7 +61,8 @@ def library_device_bytes(n_chans, n_samples, n_templates, template_samples_per_c
         gram = 0  # the literal recheck runs instead
     screen = 0
     if U * S * 4 <= _MAX_SCREEN_BUFFER_BYTES:
-        batch = min(_SCREEN_BATCH_WINDOWS, max(1, _SCREEN_BATCH_BYTES // (U * S * 4)))
+        n_windows = -(-max(int(n_samples), 1) // L)  # the library holds no more rows than windows
+        batch = max(1, min(_SCREEN_BATCH_WINDOWS, n_windows, _SCREEN_BATCH_BYTES // (U * S * 4)))
         screen = batch * U * S * 4 + U * S * 4 + batch * 65 * 4  # B rows, Ntr, deferred-rescoring records
     templates = 3 * U * C * L * 4
     constant = gram + screen + templates + _STAGING_BYTES

@@ -1858,7 +1858,7 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
     for (int64_t w : nwin) sum_nwin += w;
     const uint32_t rows_cap =
         recheck ? (uint32_t)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(sum_nwin, 1 << 20),
-                                                                  (int64_t)(kMaxBbufBytes / (bstride * 4))))
+                                                                  (int64_t)(kScreenBatchBytes / (bstride * 4))))
                 : 0;
     int64_t max_tiles = 0;
     for (int64_t w : nwin) max_tiles = std::max(max_tiles, (w + kCompactTile - 1) / kCompactTile);

@@ -224,6 +224,8 @@ def binary_pursuit_batch(segments, n_workers=4, device=0, precision="mixed", sta
     n = len(segments)
     if graph and (mode != "fused" or int(lanes) != 1):
         raise ValueError("graph=True runs the fused mode on one lane (graph calls stay on one host thread)")
+    if graph:
+        raise NotImplementedError("graph=True is disabled in this build (the library refuses FBP_FLAG_BATCH_GRAPH)")
     if n == 0:
         return []
     if mode == "fused":

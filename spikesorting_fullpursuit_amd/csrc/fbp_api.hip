@@ -38,7 +38,8 @@ void launch_window_max(const TemplateSet& ts, const PassState& st, uint32_t nW, 
 void launch_flag_compact(const PassState& st, uint32_t nW, uint32_t* blockbuf, hipStream_t s);
 void launch_merge_regions(uint32_t* F, const uint32_t* Fs, const uint32_t* rcnt, int nreg, uint32_t rwin, uint32_t* cnt,
                           hipStream_t s);
-void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s);
+void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s,
+                          const uint32_t* order = nullptr);
 void launch_spike_compact(const PassState& st, uint32_t nW, uint32_t* out_idx, uint32_t* out_lab, uint64_t base,
                           uint32_t* blockbuf, hipStream_t s);
 void launch_next_windows(const PassState& st, uint32_t* blockbuf, hipStream_t s);
@@ -272,7 +273,7 @@ struct fbp_handle {
     int npl_max = 0;  // most units sharing a channel with one unit
     int64_t npairs = 0;
     bool screen_ok = false;
-    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand, Fs, lhist, rcnt;
+    DevBuf pid, pair_l, pair_u, G, minG, minGb, Ntr, Til, rowL1, SS, Bbuf, xcand, Fs, lhist, rcnt, lorder;
     // matrix-core tables: fp16 fragments of 16-template tiles (hi parts for the
     // likelihood; hi and lo parts for the overlap screen's B)
     bool mf_built = false;
@@ -865,14 +866,23 @@ static int build_screen_tables(fbp_handle* h, int64_t n, bool* ok)
     const int64_t np = (int64_t)pl.size();
     if ((size_t)np * S * S * 4 > kMaxGramBytes) return FBP_OK;
     h->npl_max = 0;
+    std::vector<int> partners(U);
     for (int l = 0; l < U; l++) {
         int c = 0;
         for (int u = 0; u < U; u++) c += pid[(size_t)l * U + u] >= 0;
+        partners[l] = c;
         h->npl_max = std::max(h->npl_max, c);
     }
+    // Label order of the grouped flagged windows: most channel-sharing partners first, so
+    // that a screen launch ends on the labels whose windows enumerate the fewest items
+    std::vector<uint32_t> lorder(U);
+    for (int l = 0; l < U; l++) lorder[l] = (uint32_t)l;
+    std::stable_sort(lorder.begin(), lorder.end(), [&](uint32_t a, uint32_t b) { return partners[a] > partners[b]; });
     if (pl.empty()) { pl.push_back(0); pu.push_back(0); }
     HIP_TRY(hipStreamSynchronize(h->stream), "sync");
     HIP_TRY(h->pid.ensure(pid.size() * 4), "alloc pid");
+    HIP_TRY(h->lorder.ensure((size_t)U * 4), "alloc label order");
+    HIP_TRY(hipMemcpy(h->lorder.p, lorder.data(), (size_t)U * 4, hipMemcpyHostToDevice), "copy label order");
     HIP_TRY(h->pair_l.ensure(pl.size() * 4), "alloc pairs");
     HIP_TRY(h->pair_u.ensure(pu.size() * 4), "alloc pairs");
     HIP_TRY(h->G.ensure((size_t)std::max<int64_t>(np, 1) * S * ((S + 3) & ~3) * 4), "alloc gram table");
@@ -1223,7 +1233,7 @@ int fbp_destroy(fbp_handle* h)
                       &h->pid, &h->pair_l, &h->pair_u, &h->G, &h->minG, &h->minGb, &h->Ntr, &h->rowL1, &h->SS, &h->Bbuf,
                       &h->mf_frag, &h->mf_fragl, &h->mf_units, &h->mf_nkb, &h->mf_nblk, &h->mf_aoff, &h->mf_chb, &h->mf_T1,
                       &h->mf_mbits, &h->mf_vs, &h->mf_tileof, &h->mf_nunit, &h->cmask, &h->dirty, &h->mma_frag, &h->mma_tsc,
-                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist, &h->rcnt,
+                      &h->mma_T1, &h->mma_scal, &h->mma_q, &h->mma_bmax, &h->mma_gmask, &h->Til, &h->xcand, &h->Fs, &h->lhist, &h->rcnt, &h->lorder,
                       &h->mma_qtmp, &h->mma_qch, &h->mma_rslot};
     for (DevBuf* b : bufs) b->release();
     for (int b = 0; b < kStageBufs; b++) {
@@ -1485,7 +1495,7 @@ int fbp_run(fbp_handle* h, const float* Vsrc, int64_t ld, int64_t N, int32_t v_o
     // device from sp.counters[CNT_F], at most nF_hi): grouped by label into Fs,
     // then correlations, selection and deferred rescoring in batches.
     auto screened = [&](const PassState& sp, uint32_t nF_hi, uint32_t* Fs, hipStream_t q) {
-        launch_sort_by_label(sp, nF_hi, U, Fs, h->lhist.as<uint32_t>(), q);
+        launch_sort_by_label(sp, nF_hi, U, Fs, h->lhist.as<uint32_t>(), q, h->lorder.as<uint32_t>());
         PassState sf = sp;
         sf.F = Fs;
         ScreenTables tb = tables_view(h);
@@ -1775,6 +1785,10 @@ int fbp_run_batch(fbp_handle* const* hs, int32_t nseg, const float* const* V, co
         for (int32_t k = 0; k < g; k++)
             if (hs[k] == hs[g]) return set_err(h, FBP_E_INVALID, "a handle appears twice in the batch");
     }
+    // Graph replays are switched off: a one-lane batch with them ended in a host
+    // segmentation fault whose cause is not known (the code below stays for when it is).
+    if (flags & FBP_FLAG_BATCH_GRAPH)
+        return set_err(h, FBP_E_UNSUPPORTED, "FBP_FLAG_BATCH_GRAPH is disabled in this build");
     RunGuard guard((flags & FBP_FLAG_BATCH_GRAPH) != 0);
     if (!guard.ok)
         return set_err(h, FBP_E_STATE, (flags & FBP_FLAG_BATCH_GRAPH)

@@ -927,16 +927,18 @@ __global__ __launch_bounds__(kBlock) void k_label_hist_lds(const uint32_t* __res
         if (lh[u] != 0u) atomicAdd(hist + u, lh[u]);
 }
 
-// hist[U] -> exclusive offsets in place (one block).
-__global__ __launch_bounds__(1024) void k_label_offsets(uint32_t* __restrict__ hist, int U)
+// hist[U] -> exclusive offsets in place (one block), the labels laid out in the
+// order order[0 .. U) (label ids ascending without it).
+__global__ __launch_bounds__(1024) void k_label_offsets(uint32_t* __restrict__ hist, int U,
+                                                        const uint32_t* __restrict__ order)
 {
     __shared__ uint32_t part[1024];
     const int t = (int)threadIdx.x;
     const int per = (U + 1023) / 1024;
     uint32_t sum = 0;
     for (int k = 0; k < per; k++) {
-        const int u = t * per + k;
-        if (u < U) sum += hist[u];
+        const int r = t * per + k;
+        if (r < U) sum += hist[order ? order[r] : (uint32_t)r];
     }
     part[t] = sum;
     __syncthreads();
@@ -948,8 +950,9 @@ __global__ __launch_bounds__(1024) void k_label_offsets(uint32_t* __restrict__ h
     }
     uint32_t run = part[t] - sum;
     for (int k = 0; k < per; k++) {
-        const int u = t * per + k;
-        if (u < U) {
+        const int r = t * per + k;
+        if (r < U) {
+            const uint32_t u = order ? order[r] : (uint32_t)r;
             const uint32_t h = hist[u];
             hist[u] = run;
             run += h;
@@ -994,7 +997,8 @@ __global__ __launch_bounds__(kBlock) void k_label_scatter_lds(const uint32_t* __
     }
 }
 
-void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s)
+void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out, uint32_t* hist, hipStream_t s,
+                          const uint32_t* order)
 {
     if (nF == 0) return;
     launch_fill_u32(hist, 0u, U, s);
@@ -1004,14 +1008,14 @@ void launch_sort_by_label(const PassState& st, uint32_t nF, int U, uint32_t* out
         const size_t lds = (size_t)U * 4;
         hipLaunchKernelGGL(k_label_hist_lds, dim3(g), dim3(kBlock), lds, s, st.F, st.counters + CNT_F, st.best_lab,
                            hist, U);
-        hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U);
+        hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U, order);
         hipLaunchKernelGGL(k_label_scatter_lds, dim3(g), dim3(kBlock), lds, s, st.F, st.counters + CNT_F, st.best_lab,
                            hist, out, U);
         return;
     }
     const unsigned g = (unsigned)std::min<uint32_t>((nF + kBlock - 1) / kBlock, 4096u);
     hipLaunchKernelGGL(k_label_hist, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist);
-    hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U);
+    hipLaunchKernelGGL(k_label_offsets, dim3(1), dim3(1024), 0, s, hist, U, order);
     hipLaunchKernelGGL(k_label_scatter, dim3(g), dim3(kBlock), 0, s, st.F, st.counters + CNT_F, st.best_lab, hist, out);
 }
 

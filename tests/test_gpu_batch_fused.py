@@ -16,6 +16,10 @@ CHANNELS = (1, 4, 8, 16, 32)
 FLAGS = _native.FLAG_OVERLAP_RECHECK | _native.FLAG_SCREEN_FP32
 
 
+_GRAPH_OFF = pytest.mark.skip(reason="hipGraph batch replays are switched off: a one-lane graph batch ended in a host "
+                               "segmentation fault whose cause is not known")
+
+
 def _segments(n, seed, dur=(0.3, 1.0), units=(4, 60)):
     """n segments of the batched workload's kind, channel counts cycling over
     1, 4, 8, 16 and 32, unit counts drawn from `units`."""
@@ -125,6 +129,7 @@ def _graph_stats(hs):
     return st
 
 
+@_GRAPH_OFF
 def test_graph_batch_of_20_equals_separate_runs():
     """Captured passes (one hipGraph replay per pass after the first), 20
     segments, C in {1, 4, 8, 16, 32}, U in [4, 60]: batch == separate run,
@@ -140,6 +145,7 @@ def test_graph_batch_of_20_equals_separate_runs():
         h.close()
 
 
+@_GRAPH_OFF
 def test_graph_batch_matches_oracle():
     segs = _segments(16, 32, units=(4, 16))
     for s in segs:
@@ -156,6 +162,7 @@ def test_graph_batch_matches_oracle():
         h.close()
 
 
+@_GRAPH_OFF
 def test_graph_batch_without_recheck_mixed_lengths_repeated():
     """No recheck; shift ranges 0..63 and one long segment; the same handles
     run twice (graphs of the second call are built afresh): both == separate."""

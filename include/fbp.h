@@ -56,7 +56,9 @@ extern "C" {
  * sizes fixed when the graph is built, its item counts written on the device), with one host read per
  * replay; the graph is rebuilt when a segment lists more windows than it holds or far fewer in all.
  * Such a call runs only while no other fbp_run / fbp_run_batch is in flight in the process, and other
- * calls fail with FBP_E_STATE until it returns (graph calls stay on one host thread at a time). */
+ * calls fail with FBP_E_STATE until it returns (graph calls stay on one host thread at a time).
+ * Switched off in this build: fbp_run_batch returns FBP_E_UNSUPPORTED for it (a one-lane graph batch ended
+ * in a host segmentation fault whose cause is not known). */
 #define FBP_FLAG_BATCH_GRAPH 8192u
 
 typedef struct fbp_handle fbp_handle;
